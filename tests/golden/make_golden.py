@@ -1,0 +1,141 @@
+"""Write the golden fixtures under tests/golden/ from the reference built from
+source (oracle/_ref/ref_dump, see oracle/Makefile).
+
+Run in a container that has the reference checkout:
+
+    make -C oracle ref && python tests/golden/make_golden.py
+
+Each case is a seeded synthetic field (bigwhoop_amd.fields) plus codec
+settings. Stored per case (only data: inputs are regenerated from the seed and
+checked against their SHA-256):
+  * cases.json          settings, input digest, codestream digest and size
+  * <case>.bwc          the full reference codestream (small cases)
+  * <case>.t1.npz       per-codeblock tier-1 records: K, Kmsbs, Z, S, L and a
+                        SHA-256 of the MQ bytes; tile slope bounds
+  * <case>.dwt.npz      SHA-256 of the float64 DWT coefficients plus every
+                        97th coefficient
+  * <case>.geo.npz      per-codeblock geometry and subband constants
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parents[1]
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "tests"))
+
+from bigwhoop_amd.fields import grf, modes_field  # noqa: E402
+import oracle_lib as ol  # noqa: E402
+
+CASES = [
+    # the CPU-runnable workload: 64^3 f64, 4 modes + noise, 3 levels, cb 16^3, 1 layer at 4 bpv
+    dict(name="c1_modes64", gen=dict(kind="modes", shape=[64, 64, 64], seed=1), npar=1, prec=8, rate="4",
+         decomp=[3, 3, 3, 0], cb=[4, 4, 4, 0], store=True, stages=True),
+    # the headline settings (5 levels, cb 32x32x8) on a smaller GRF, single and 4 layers
+    dict(name="grf64_l5_r2", gen=dict(kind="grf", shape=[64, 64, 64], seed=11, slope=-5 / 3), npar=1, prec=8,
+         rate="2", decomp=[5, 5, 5, 0], cb=[5, 5, 3, 0], store=True, stages=True),
+    dict(name="grf64_l5_r1248", gen=dict(kind="grf", shape=[64, 64, 64], seed=11, slope=-5 / 3), npar=1, prec=8,
+         rate="1,2,4,8", decomp=[5, 5, 5, 0], cb=[5, 5, 3, 0], store=True, stages=False),
+    dict(name="grf128_l5_r2", gen=dict(kind="grf", shape=[128, 128, 128], seed=5, slope=-5 / 3), npar=1, prec=8,
+         rate="2", decomp=[5, 5, 5, 0], cb=[5, 5, 3, 0], store=False, stages=False),
+    # odd sizes, defaults, float32 input, two variables
+    dict(name="grf_odd_f32_2var", gen=dict(kind="grf", shape=[37, 29, 19], seed=3, slope=-7 / 3, npar=2),
+         npar=2, prec=4, rate="0.5,3", store=True, stages=True),
+    # 4-D with temporal levels and codeblock 16x16x8x4 (the 4-D workload's settings)
+    dict(name="grf4d_small", gen=dict(kind="grf", shape=[32, 32, 16, 8], seed=9, slope=-5 / 3), npar=1, prec=8,
+         rate="2", decomp=[3, 3, 3, 2], cb=[4, 4, 3, 2], store=True, stages=True),
+    # tiles + precincts + error resilience
+    dict(name="grf_tiles_erres", gen=dict(kind="grf", shape=[48, 40, 24], seed=21, slope=-2.0), npar=1, prec=8,
+         rate="1.5,6", decomp=[3, 3, 2, 0], cb=[4, 4, 3, 0], precincts=[4, 4, 3, 0], tiles=[32, 24, 16, 0],
+         erres=True, store=True, stages=True),
+    # 1-D and 2-D
+    dict(name="grf1d", gen=dict(kind="grf", shape=[4099], seed=4, slope=-1.5), npar=1, prec=8, rate="3",
+         store=True, stages=True),
+    dict(name="grf2d_qm40", gen=dict(kind="grf", shape=[96, 80], seed=8, slope=-2.5), npar=1, prec=8,
+         rate="2,5", qm=40, cb=[5, 4, 0, 0], store=True, stages=True),
+]
+
+
+def make_input(gen):
+    npar = gen.get("npar", 1)
+    parts = []
+    for p in range(npar):
+        if gen["kind"] == "modes":
+            parts.append(modes_field(gen["shape"], seed=gen["seed"] + 1000 * p))
+        else:
+            parts.append(grf(gen["shape"], slope=gen["slope"], seed=gen["seed"] + 1000 * p))
+    return np.concatenate([a.reshape(-1) for a in parts])
+
+
+def codec_kw(case):
+    kw = {}
+    for k in ("decomp", "cb", "precincts", "qm", "tiles", "erres"):
+        if k in case:
+            kw[k] = case[k]
+    return kw
+
+
+def main():
+    if not ol.have_ref():
+        raise SystemExit("oracle/_ref/ref_dump missing: run `make -C oracle ref` where the reference exists")
+    meta = []
+    for case in CASES:
+        data = make_input(case["gen"])
+        if case["prec"] == 4:
+            data = data.astype(np.float32)
+        shape = case["gen"]["shape"]
+        kw = codec_kw(case)
+        out = ol.ref_run("full", data, shape, npar=case["npar"], prec=case["prec"], rate=case["rate"], **kw)
+        cs = out["bwc"]
+        ent = {k: v for k, v in case.items() if k not in ("store", "stages")}
+        ent["input_sha256"] = hashlib.sha256(data.tobytes()).hexdigest()
+        ent["codestream_sha256"] = hashlib.sha256(cs).hexdigest()
+        ent["codestream_size"] = len(cs)
+        ent["files"] = []
+        if case["store"]:
+            (HERE / f"{case['name']}.bwc").write_bytes(cs)
+            ent["files"].append(f"{case['name']}.bwc")
+        if case["stages"]:
+            st = ol.ref_run("stages", data, shape, npar=case["npar"], prec=case["prec"], rate=case["rate"], **kw)
+            assert st["bwc"] == cs, "stage-driven codestream differs from bwc_compress"
+            geo = ol.parse_geo(st["geo"])
+            # per (tile, param) codeblock counts follow the geometry records' order
+            o = ol.Oracle(shape, npar=case["npar"], prec=case["prec"], rate=case["rate"], **kw)
+            counts = [o.ncblk(t, p) for t in range(o.ntiles) for p in range(case["npar"])]
+            o.close()
+            assert sum(counts) == len(geo)
+            recs, slopes = ol.parse_t1(st["t1"], counts)
+            flat = [r for rr in recs for r in rr]
+            n = len(flat)
+            K = np.array([r["K"] for r in flat], np.uint8)
+            Kmsbs = np.array([r["Kmsbs"] for r in flat], np.uint8)
+            Z = np.array([r["Z"] for r in flat], np.uint8)
+            S = np.zeros((n, 191), np.uint16)
+            L = np.zeros((n, 190), np.uint64)
+            dig = np.zeros((n, 32), np.uint8)
+            for i, r in enumerate(flat):
+                S[i, : r["Z"] + 1] = r["S"]
+                L[i, : r["Z"]] = r["L"]
+                dig[i] = np.frombuffer(hashlib.sha256(r["data"]).digest(), np.uint8)
+            np.savez_compressed(HERE / f"{case['name']}.t1.npz", K=K, Kmsbs=Kmsbs, Z=Z, S=S, L=L, data_sha256=dig,
+                                counts=np.array(counts, np.uint64), slopes=np.array(slopes, np.uint16))
+            dwt = np.frombuffer(st["dwt"], np.float64)
+            np.savez_compressed(HERE / f"{case['name']}.dwt.npz",
+                                sha256=np.frombuffer(hashlib.sha256(dwt.tobytes()).digest(), np.uint8),
+                                sample=dwt[::97].copy(), n=np.array([dwt.size], np.uint64))
+            np.savez_compressed(HERE / f"{case['name']}.geo.npz", box=geo["box"].copy(), sb=geo["sb"].copy(),
+                                meta=geo["meta"].copy())
+            ent["files"] += [f"{case['name']}.t1.npz", f"{case['name']}.dwt.npz", f"{case['name']}.geo.npz"]
+        meta.append(ent)
+        print(case["name"], len(cs), "bytes", "stages" if case["stages"] else "")
+    (HERE / "cases.json").write_text(json.dumps(meta, indent=1))
+
+
+if __name__ == "__main__":
+    main()
