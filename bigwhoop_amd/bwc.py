@@ -1,0 +1,348 @@
+"""Python mirror of the reference's ctypes wrapper (src/interfaces/python/bwc.py)
+over the MI355X libbwc.so.
+
+Same function names and argument meaning as the reference wrapper
+(init_stream, alloc_coder, alloc_decoder, free_codec, set_codeblocks,
+set_decomp, set_qm, set_tiles, set_precincts, create_compression, compress,
+create_decompression, decompress), with argument types matching the C API
+(include/bwc.h). The library must have been built (python -m
+bigwhoop_amd.build); loading fails loudly otherwise — there is no CPU
+fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from enum import IntEnum
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libbwc.so"
+
+
+class DWT(IntEnum):
+    CDF_9_7 = 0
+    LeGall_5_3 = 1
+    Haar = 2
+
+
+class PROG(IntEnum):
+    LRCP = 0
+    RLCP = 1
+
+
+class TILE(IntEnum):
+    SIZEOF = 0
+    NUMBEROF = 1
+
+
+class PREC(IntEnum):
+    single = 4
+    double = 8
+
+
+class MODE(IntEnum):
+    comp = 0
+    decomp = 1
+
+
+_lib = None
+
+
+def lib():
+    """Load libbwc.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m bigwhoop_amd.build`")
+        L = C.CDLL(str(LIB_PATH))
+        vp, u8, u16, u32, u64 = C.c_void_p, C.c_uint8, C.c_uint16, C.c_uint32, C.c_uint64
+        sig = {
+            "bwc_init_stream": (vp, [vp, vp, C.c_int]),
+            "bwc_set_com": (C.c_ubyte, [vp, C.c_char_p, u16]),
+            "bwc_set_aux": (C.c_ubyte, [vp, C.c_char_p, u32]),
+            "bwc_alloc_coder": (vp, [u64, u64, u64, u64, u8, C.c_int]),
+            "bwc_alloc_decoder": (vp, []),
+            "bwc_free_codec": (None, [vp]),
+            "bwc_set_error_resilience": (None, [vp]),
+            "bwc_set_decomp": (None, [vp, u8, u8, u8, u8]),
+            "bwc_set_precincts": (None, [vp, u8, u8, u8, u8]),
+            "bwc_set_codeblocks": (None, [vp, u8, u8, u8, u8]),
+            "bwc_set_qm": (None, [vp, u8]),
+            "bwc_set_tiles": (None, [vp, u64, u64, u64, u64, C.c_int]),
+            "set_kernels": (None, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+            "set_progression": (None, [vp, C.c_int]),
+            "set_quant_style": (None, [vp, C.c_int]),
+            "set_quant_step_size": (None, [vp, C.c_double]),
+            "bwc_create_compression": (C.c_ubyte, [vp, vp, C.c_char_p]),
+            "bwc_compress": (C.c_size_t, [vp, vp]),
+            "bwc_create_decompression": (C.c_ubyte, [vp, vp, u8]),
+            "bwc_decompress": (C.c_ubyte, [vp, vp]),
+            "bwc_open_header": (vp, [vp]),
+            "bwc_close_header": (None, [vp]),
+            "bwc_amd_version": (C.c_int, []),
+            "bwc_amd_device_count": (C.c_int, []),
+            "bwc_amd_last_error": (C.c_char_p, []),
+            "bwc_amd_reset_gain_lut": (None, []),
+            "bwc_amd_ncblk": (u64, [vp, u64, u8]),
+            "bwc_amd_param_dims": (C.c_int, [vp, u64, C.POINTER(u64)]),
+            "bwc_amd_stage_dwt": (C.c_int, [vp, u64, u8, vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+            "bwc_amd_stage_idwt": (C.c_int, [vp, u64, u8, C.POINTER(C.c_double)]),
+            "bwc_amd_stage_t1": (C.c_int, [vp, u64, u8, C.POINTER(C.c_double), C.POINTER(u8), C.POINTER(u8),
+                                           C.POINTER(u8), C.POINTER(u32), C.POINTER(u16), C.POINTER(u8), u64,
+                                           C.POINTER(u64), C.POINTER(u16)]),
+            "bwc_amd_last_timings": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+            "bwc_amd_cblk_info": (C.c_int, [vp, u64, u8, C.POINTER(u32), C.POINTER(C.c_double), C.POINTER(u32)]),
+            "bwc_amd_begin_records": (None, [vp]),
+            "bwc_amd_set_records": (C.c_int, [vp, u64, u8, C.POINTER(u8), C.POINTER(u8), C.POINTER(u8),
+                                              C.POINTER(u32), C.POINTER(u16), C.POINTER(u8), C.POINTER(u64),
+                                              C.POINTER(C.c_double)]),
+            "bwc_amd_encode_records": (C.c_size_t, [vp, vp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    e = lib().bwc_amd_last_error()
+    return e.decode() if e else ""
+
+
+# ---- reference wrapper names -------------------------------------------------
+def init_stream(inpbuf, outbuf, mode):
+    return lib().bwc_init_stream(inpbuf, outbuf, int(mode))
+
+
+def alloc_coder(nX, nY, nZ, nTS, nPar, prec):
+    return lib().bwc_alloc_coder(nX, nY, nZ, nTS, nPar, int(prec))
+
+
+def alloc_decoder():
+    return lib().bwc_alloc_decoder()
+
+
+def free_codec(codec):
+    lib().bwc_free_codec(codec)
+
+
+def set_codeblocks(codec, cbX, cbY, cbZ, cbTS):
+    lib().bwc_set_codeblocks(codec, cbX, cbY, cbZ, cbTS)
+
+
+def set_decomp(codec, decompX, decompY, decompZ, decompTS):
+    lib().bwc_set_decomp(codec, decompX, decompY, decompZ, decompTS)
+
+
+def set_qm(codec, Qm):
+    lib().bwc_set_qm(codec, Qm)
+
+
+def set_tiles(codec, tilesX, tilesY, tilesZ, tilesTS, instr):
+    lib().bwc_set_tiles(codec, tilesX, tilesY, tilesZ, tilesTS, int(instr))
+
+
+def set_precincts(codec, pX, pY, pZ, pTS):
+    lib().bwc_set_precincts(codec, pX, pY, pZ, pTS)
+
+
+def set_error_resilience(codec):
+    lib().bwc_set_error_resilience(codec)
+
+
+def create_compression(codec, stream, rate_control):
+    buf = C.create_string_buffer(rate_control.encode() if isinstance(rate_control, str) else rate_control)
+    return lib().bwc_create_compression(codec, stream, buf)
+
+
+def compress(codec, stream):
+    return lib().bwc_compress(codec, stream)
+
+
+def create_decompression(codec, stream, layer):
+    return lib().bwc_create_decompression(codec, stream, layer)
+
+
+def decompress(codec, stream):
+    return lib().bwc_decompress(codec, stream)
+
+
+# ---- array-level helpers -----------------------------------------------------
+def _shape4(shape):
+    s = [int(x) for x in shape] + [1] * (4 - len(shape))
+    return s[:4]
+
+
+class Codec:
+    """One configured compression job: owns the bwc_codec and its stream.
+
+    ``shape`` is (nX, nY, nZ, nTS) with x fastest; settings follow the C
+    setters and are applied in the order decomp, precincts, codeblocks, qm,
+    tiles, error resilience.
+    """
+
+    def __init__(self, shape, npar=1, prec=8, rate="2", decomp=None, precincts=None, cb=None, qm=None,
+                 tiles=None, erres=False, fresh_gain_table=True):
+        L = lib()
+        if fresh_gain_table:
+            L.bwc_amd_reset_gain_lut()
+        n = _shape4(shape)
+        self.shape = n
+        self.npar = npar
+        self.prec = prec
+        self.nbytes = int(np.prod(n)) * npar * prec
+        self.h = L.bwc_alloc_coder(n[0], n[1], n[2], n[3], npar, prec)
+        if not self.h:
+            raise RuntimeError("bwc_alloc_coder failed")
+        if decomp is not None:
+            L.bwc_set_decomp(self.h, *[int(x) for x in decomp])
+        if precincts is not None:
+            L.bwc_set_precincts(self.h, *[int(x) for x in precincts])
+        if cb is not None:
+            L.bwc_set_codeblocks(self.h, *[int(x) for x in cb])
+        if qm is not None:
+            L.bwc_set_qm(self.h, int(qm))
+        if tiles is not None:
+            L.bwc_set_tiles(self.h, *[int(x) for x in tiles], 0)
+        if erres:
+            L.bwc_set_error_resilience(self.h)
+        self.out = np.zeros(self.nbytes * 2 + (1 << 20), np.uint8)
+        self.stream = L.bwc_init_stream(None, self.out.ctypes.data_as(C.c_void_p), 0)
+        self._rate = C.create_string_buffer(rate.encode())
+        if L.bwc_create_compression(self.h, self.stream, self._rate):
+            raise RuntimeError("bwc_create_compression failed: " + last_error())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bwc_free_codec(self.h)
+            self.h = None
+        if getattr(self, "stream", None):
+            C.CDLL(None).free(C.c_void_p(self.stream))
+            self.stream = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def compress_ptr(self, ptr: int) -> bytes:
+        """Compress from a host or HIP device address (e.g. tensor.data_ptr())."""
+        st = _Stream.from_address(self.stream)
+        st.inp = ptr
+        n = lib().bwc_compress(self.h, self.stream)
+        if n == 0:
+            raise RuntimeError("bwc_compress failed: " + last_error())
+        return self.out[:n].tobytes()
+
+    def compress(self, data: np.ndarray) -> bytes:
+        data = np.ascontiguousarray(data)
+        if data.nbytes != self.nbytes:
+            raise ValueError("input size does not match the codec")
+        self._keep = data
+        return self.compress_ptr(data.ctypes.data)
+
+    def timings(self):
+        ms = (C.c_double * 10)()
+        la = (C.c_int * 4)()
+        lib().bwc_amd_last_timings(self.h, ms, la)
+        return list(ms), list(la)
+
+    # ---- single stages (bwc_amd.h) ----
+    def ncblk(self, tile=0, param=0):
+        return lib().bwc_amd_ncblk(self.h, tile, param)
+
+    def param_dims(self, tile=0):
+        d = (C.c_uint64 * 4)()
+        if lib().bwc_amd_param_dims(self.h, tile, d):
+            raise RuntimeError(last_error())
+        return tuple(int(x) for x in d)
+
+    def geometry(self, tile=0, param=0):
+        n = self.ncblk(tile, param)
+        box = np.zeros((n, 8), np.uint32)
+        sb = np.zeros((n, 3), np.float64)
+        meta = np.zeros((n, 4), np.uint32)
+        lib().bwc_amd_cblk_info(self.h, tile, param, _p(box, C.c_uint32), _p(sb, C.c_double), _p(meta, C.c_uint32))
+        return box, sb, meta
+
+    def stage_dwt(self, data, tile=0, param=0):
+        data = np.ascontiguousarray(data)
+        out = np.zeros(int(np.prod(self.param_dims(tile))), np.float64)
+        mm = np.zeros(4, np.float64)
+        if lib().bwc_amd_stage_dwt(self.h, tile, param, data.ctypes.data_as(C.c_void_p), _p(out, C.c_double),
+                                   _p(mm, C.c_double)):
+            raise RuntimeError("stage_dwt failed: " + last_error())
+        return out, mm
+
+    def stage_idwt(self, coeffs, tile=0, param=0):
+        c = np.ascontiguousarray(coeffs, dtype=np.float64).copy()
+        if lib().bwc_amd_stage_idwt(self.h, tile, param, _p(c, C.c_double)):
+            raise RuntimeError("stage_idwt failed: " + last_error())
+        return c
+
+    def stage_t1(self, coeffs, tile=0, param=0):
+        coeffs = np.ascontiguousarray(coeffs, dtype=np.float64)
+        n = self.ncblk(tile, param)
+        K = np.zeros(n, np.uint8)
+        Km = np.zeros(n, np.uint8)
+        Z = np.zeros(n, np.uint8)
+        Lr = np.zeros((n, 190), np.uint32)
+        S = np.zeros((n, 191), np.uint16)
+        cap = coeffs.size * 10 + 4096
+        data = np.zeros(cap, np.uint8)
+        off = np.zeros(n + 1, np.uint64)
+        sl = np.zeros(2, np.uint16)
+        rc = lib().bwc_amd_stage_t1(self.h, tile, param, _p(coeffs, C.c_double), _p(K, C.c_uint8),
+                                    _p(Km, C.c_uint8), _p(Z, C.c_uint8), _p(Lr, C.c_uint32), _p(S, C.c_uint16),
+                                    _p(data, C.c_uint8), cap, _p(off, C.c_uint64), _p(sl, C.c_uint16))
+        if rc:
+            raise RuntimeError("stage_t1 failed: " + last_error())
+        return dict(K=K, Kmsbs=Km, Z=Z, L=Lr, S=S, data=data[: int(off[-1])], off=off, slopes=tuple(int(x) for x in sl))
+
+    def begin_records(self):
+        lib().bwc_amd_begin_records(self.h)
+
+    def set_records(self, tile, param, K, Kmsbs, Z, L, S, data, off, mm):
+        K, Kmsbs, Z = [np.ascontiguousarray(a, np.uint8) for a in (K, Kmsbs, Z)]
+        L = np.ascontiguousarray(L, np.uint32)
+        S = np.ascontiguousarray(S, np.uint16)
+        data = np.ascontiguousarray(np.frombuffer(bytes(data), np.uint8) if not isinstance(data, np.ndarray)
+                                    else data, np.uint8)
+        if data.size == 0:
+            data = np.zeros(1, np.uint8)
+        off = np.ascontiguousarray(off, np.uint64)
+        mm = np.ascontiguousarray(mm, np.float64)
+        self._rec_keep = (K, Kmsbs, Z, L, S, data, off, mm)
+        if lib().bwc_amd_set_records(self.h, tile, param, _p(K, C.c_uint8), _p(Kmsbs, C.c_uint8), _p(Z, C.c_uint8),
+                                     _p(L, C.c_uint32), _p(S, C.c_uint16), _p(data, C.c_uint8), _p(off, C.c_uint64),
+                                     _p(mm, C.c_double)):
+            raise RuntimeError("set_records failed")
+
+    def encode_records(self) -> bytes:
+        n = lib().bwc_amd_encode_records(self.h, self.stream)
+        if n == 0:
+            raise RuntimeError("encode_records failed")
+        return self.out[:n].tobytes()
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class _Stream(C.Structure):
+    _fields_ = [("aux", C.c_void_p), ("com", C.c_void_p), ("inp", C.c_void_p), ("out", C.c_void_p),
+                ("mode", C.c_int)]
+
+
+def compress_array(data, shape, npar=1, rate="2", **kw) -> bytes:
+    """One-shot compression of a numpy array (float32 or float64)."""
+    prec = 4 if np.asarray(data).dtype == np.float32 else 8
+    c = Codec(shape, npar=npar, prec=prec, rate=rate, **kw)
+    try:
+        return c.compress(data)
+    finally:
+        c.close()

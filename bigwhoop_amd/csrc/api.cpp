@@ -1,0 +1,651 @@
+// C API of libbwc (include/bwc.h, include/bwc_amd.h).
+//
+// Configuration and setters keep the reference's validation, warnings and
+// codestream control bits (libbwc.c:1828-2828); bwc_compress runs the hot
+// stages on one GPU (gpu.hip) and tier-2 / codestream assembly on the host
+// (tier2.cpp).
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "codec.hpp"
+#include "gpu.hpp"
+
+using namespace bwc;
+
+namespace bwc {
+int t2_rate_control(bwc_codec *codec, Job *job, uint64_t tile);
+int t2_packets(bwc_codec *codec, Job *job, uint64_t tile);
+}
+
+namespace {
+
+thread_local char g_api_err[512];
+
+void set_err(const char *m) {
+  std::snprintf(g_api_err, sizeof g_api_err, "%s", m);
+  std::fprintf(stderr, "bwc: error: %s\n", m);
+}
+
+bwc_codec *alloc_codec() {
+  auto *c = static_cast<bwc_codec *>(std::calloc(1, sizeof(bwc_codec)));
+  if (!c) return nullptr;
+  c->info.codec_prec = bwc_precision_double;
+  c->tile = new Job();
+  return c;
+}
+
+void refresh_qt(bwc_codec *c) {
+  double lut[3][10];
+  std::memcpy(lut, gain_lut(), sizeof lut);
+  const double delta = 1 / (std::pow(2, 2 + kPrecBits) * std::sqrt(energy_gain(c, lut, 0, c->control.nDecomp)));
+  compute_qt(&c->control, delta);
+}
+
+uint8_t levels_of(uint64_t n) { return (uint8_t)(std::log((double)n) / std::log(2)); }
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+void reset_for_encode(bwc_codec *c, Job *job) {
+  for (auto &tl : job->plan.tiles) {
+    tl.slope_min = 0xFFFF;
+    tl.slope_max = 0;
+    for (auto &par : tl.par)
+      for (auto &res : par.res) {
+        for (auto &pk : res.pk) pk = Packet();
+        for (auto &sb : res.sb)
+          for (auto &pr : sb.prec) {
+            std::fill(pr.incl.value.begin(), pr.incl.value.end(), 0xFFFF);
+            std::fill(pr.incl.threshold.begin(), pr.incl.threshold.end(), 0);
+            std::fill(pr.msbs.value.begin(), pr.msbs.value.end(), 0xFFFF);
+            std::fill(pr.msbs.threshold.begin(), pr.msbs.threshold.end(), 0);
+          }
+      }
+  }
+  job->rec.assign(job->plan.tiles.size() * c->info.nPar, CblkRecords());
+}
+
+}  // namespace
+
+extern "C" {
+
+bwc_stream *bwc_init_stream(void *const inpbuf, void *const outbuf, bwc_mode const mode) {
+  auto *s = static_cast<bwc_stream *>(std::calloc(1, sizeof(bwc_stream)));
+  if (!s) return nullptr;
+  s->inp = inpbuf;
+  s->out = outbuf;
+  s->mode = mode;
+  return s;
+}
+
+uchar bwc_set_com(bwc_stream *const data, char const *const com, uint16 const size) {
+  data->codestream.com = static_cast<bwc_span *>(std::calloc(1, sizeof(bwc_span)));
+  if (!data->codestream.com) return 1;
+  data->codestream.com->memory = static_cast<uchar *>(std::calloc(size ? size : 1, 1));
+  if (!data->codestream.com->memory) return 1;
+  std::memcpy(data->codestream.com->memory, com, size);
+  data->codestream.com->access = data->codestream.com->memory;
+  data->codestream.com->size = size;
+  return 0;
+}
+
+uchar bwc_set_aux(bwc_stream *const data, char const *const aux, uint32 const size) {
+  data->codestream.aux = static_cast<bwc_span *>(std::calloc(1, sizeof(bwc_span)));
+  if (!data->codestream.aux) return 1;
+  data->codestream.aux->memory = static_cast<uchar *>(std::calloc(size ? size : 1, 1));
+  if (!data->codestream.aux->memory) return 1;
+  std::memcpy(data->codestream.aux->memory, aux, size);
+  data->codestream.aux->access = data->codestream.aux->memory;
+  data->codestream.aux->size = size;
+  return 0;
+}
+
+// configure_codec (libbwc.c:1828-1982)
+bwc_codec *configure_codec(bwc_codec *const codec, uint64 const nX, uint64 const nY, uint64 const nZ, uint64 const nTS,
+                           uint8 const nPar, bwc_precision const prec) {
+  bwc_gl_ctrl &c = codec->control;
+  bwc_gl_inf &i = codec->info;
+  i.nX = nX, i.nY = nY, i.nZ = nZ, i.nTS = nTS, i.nPar = nPar, i.data_prec = prec;
+  c.tileSizeX = nX, c.tileSizeY = nY, c.tileSizeZ = nZ, c.tileSizeTS = nTS;
+  c.nTiles = 1;
+  c.precSizeX = (nX >> 1) ? 15 : 0, c.precSizeY = (nY >> 1) ? 15 : 0;
+  c.precSizeZ = (nZ >> 1) ? 15 : 0, c.precSizeTS = (nTS >> 1) ? 15 : 0;
+  c.cbX = (nX >> 1) ? 5 : 0, c.cbY = (nY >> 1) ? 5 : 0, c.cbZ = (nZ >> 1) ? 5 : 0, c.cbTS = (nTS >> 1) ? 5 : 0;
+  c.KernelX = c.KernelY = c.KernelZ = c.KernelTS = bwc_dwt_9_7;
+  c.progression = bwc_prog_LRCP;
+  c.quantization_style = bwc_qt_derived;
+  c.nLayers = 0;
+  c.Qm = kPrecBits - 1;
+  c.bitrate = nullptr;
+  c.guard_bits = 2;
+  const uint8_t lx = levels_of(nX), ly = levels_of(nY), lz = levels_of(nZ), lt = levels_of(nTS);
+  c.decompX = (nX >> 1) ? (4 < lx ? 4 : lx) : 0;
+  c.decompY = (nY >> 1) ? (4 < ly ? 4 : ly) : 0;
+  c.decompZ = (nZ >> 1) ? (4 < lz ? 4 : lz) : 0;
+  c.decompTS = (nTS >> 1) ? (4 < lt ? 4 : lt) : 0;
+  c.nDecomp = std::max(c.decompX, std::max(c.decompY, std::max(c.decompZ, c.decompTS)));
+  gain_lut_accumulate();
+  refresh_qt(codec);
+  return codec;
+}
+
+bwc_codec *bwc_alloc_coder(uint64 const nX, uint64 const nY, uint64 const nZ, uint64 const nTS, uint8 const nPar,
+                           bwc_precision const prec) {
+  bwc_codec *c = alloc_codec();
+  if (!c) return nullptr;
+  configure_codec(c, nX, nY, nZ, nTS, nPar, prec);
+  c->mode = comp;
+  return c;
+}
+
+bwc_codec *bwc_alloc_decoder(void) {
+  bwc_codec *c = alloc_codec();
+  if (!c) return nullptr;
+  c->mode = decomp;
+  return c;
+}
+
+void bwc_free_codec(bwc_codec *const codec) {
+  if (!codec) return;
+  delete job_of(codec);
+  std::free(codec->control.bitrate);
+  std::free(codec);
+}
+
+void bwc_set_error_resilience(bwc_codec *const codec) {
+  codec->control.error_resilience ^= 0x01;
+  codec->control.CSsgc ^= (0x01 << 0);
+}
+
+void set_quant_style(bwc_codec *const codec, bwc_quant_st const style) {
+  codec->control.quantization_style = style;
+  codec->control.CSsgc |= (0x01 << 1);
+}
+
+void set_quant_step_size(bwc_codec *const codec, double const delta) {
+  if ((delta <= 0) || (delta >= 2)) {
+    warn("invalid quantization step size; 0 < step size < 2");
+    return;
+  }
+  compute_qt(&codec->control, delta);
+  codec->control.CSsgc |= (0x01 << 2);
+}
+
+void set_progression(bwc_codec *const codec, bwc_prog_ord const progression) {
+  codec->control.progression = progression;
+  codec->control.CSsgc |= (0x01 << 3);
+}
+
+void set_kernels(bwc_codec *const codec, bwc_dwt_filter const KernelX, bwc_dwt_filter const KernelY,
+                 bwc_dwt_filter const KernelZ, bwc_dwt_filter const KernelTS) {
+  bwc_gl_ctrl &c = codec->control;
+  c.KernelX = KernelX ? KernelX : bwc_dwt_9_7;
+  c.KernelY = KernelY ? KernelY : bwc_dwt_9_7;
+  c.KernelZ = KernelZ ? KernelZ : bwc_dwt_9_7;
+  c.KernelTS = KernelTS ? KernelTS : bwc_dwt_haar;
+  c.CSsgc |= (0x01 << 4);
+}
+
+// bwc_set_decomp (libbwc.c:2293-2421): accumulates the gain table again
+void bwc_set_decomp(bwc_codec *const codec, uint8 const dX, uint8 const dY, uint8 const dZ, uint8 const dTS) {
+  bwc_gl_ctrl &c = codec->control;
+  bwc_gl_inf &i = codec->info;
+  const uint8_t lx = levels_of(i.nX), ly = levels_of(i.nY), lz = levels_of(i.nZ), lt = levels_of(i.nTS);
+  c.decompX = (i.nX >> 1) ? (dX < lx ? dX : lx) : 0;
+  c.decompY = (i.nY >> 1) ? (dY < ly ? dY : ly) : 0;
+  c.decompZ = (i.nZ >> 1) ? (dZ < lz ? dZ : lz) : 0;
+  c.decompTS = (i.nTS >> 1) ? (dTS < lt ? dTS : lt) : 0;
+  c.nDecomp = std::max(c.decompX, std::max(c.decompY, std::max(c.decompZ, c.decompTS)));
+  if (c.decompX > 63 || c.decompY > 63 || c.decompZ > 63 || c.decompTS > 31) {
+    warn("invalid decomposition level value; maximum is 63 (spatial) and 31 (temporal)");
+    c.decompX = (i.nX >> 1) ? 4 : 0;
+    c.decompY = (i.nY >> 1) ? 4 : 0;
+    c.decompZ = (i.nZ >> 1) ? 4 : 0;
+    c.decompTS = (i.nTS >> 1) ? 4 : 0;
+    return;
+  }
+  gain_lut_accumulate();
+  refresh_qt(codec);
+  c.CSsgc |= (0x01 << 5);
+}
+
+void bwc_set_precincts(bwc_codec *const codec, uint8 const pX, uint8 const pY, uint8 const pZ, uint8 const pTS) {
+  bwc_gl_ctrl &c = codec->control;
+  bwc_gl_inf &i = codec->info;
+  c.precSizeX = (i.nX >> 1) ? (pX ? pX : 15) : 0;
+  c.precSizeY = (i.nY >> 1) ? (pY ? pY : 15) : 0;
+  c.precSizeZ = (i.nZ >> 1) ? (pZ ? pZ : 15) : 0;
+  c.precSizeTS = (i.nTS >> 1) ? (pTS ? pTS : 15) : 0;
+  if ((c.precSizeX < 1 && (i.nX >> 1)) || (c.precSizeY < 1 && (i.nY >> 1)) || (c.precSizeZ < 1 && (i.nZ >> 1)) ||
+      (c.precSizeTS < 1 && (i.nTS >> 1)) || c.precSizeX > 15 || c.precSizeY > 15 || c.precSizeZ > 15 ||
+      c.precSizeTS > 15) {
+    warn("invalid precinct size; valid sizes are 2^1 to 2^15");
+    c.precSizeX = (i.nX >> 1) ? 15 : 0;
+    c.precSizeY = (i.nY >> 1) ? 15 : 0;
+    c.precSizeZ = (i.nZ >> 1) ? 15 : 0;
+    c.precSizeTS = (i.nTS >> 1) ? 15 : 0;
+    return;
+  }
+  c.CSsgc |= (0x01 << 6);
+}
+
+void bwc_set_codeblocks(bwc_codec *const codec, uint8 const cbX, uint8 const cbY, uint8 const cbZ, uint8 const cbTS) {
+  bwc_gl_ctrl &c = codec->control;
+  bwc_gl_inf &i = codec->info;
+  c.cbX = (i.nX >> 1) ? (cbX ? cbX : 5) : 0;
+  c.cbY = (i.nY >> 1) ? (cbY ? cbY : 5) : 0;
+  c.cbZ = (i.nZ >> 1) ? (cbZ ? cbZ : 5) : 0;
+  c.cbTS = (i.nTS >> 1) ? (cbTS ? cbTS : 5) : 0;
+  const int s = c.cbX + c.cbY + c.cbZ + c.cbTS;
+  if (c.cbX > 10 || c.cbY > 10 || c.cbZ > 10 || c.cbTS > 10 || s < 4 || s > 20) {
+    warn("invalid codeblock size; at most 2^20 values with at most 2^10 per axis, at least 2^4");
+    c.cbX = (i.nX >> 1) ? 5 : 0;
+    c.cbY = (i.nY >> 1) ? 5 : 0;
+    c.cbZ = (i.nZ >> 1) ? 5 : 0;
+    c.cbTS = (i.nTS >> 1) ? 5 : 0;
+    return;
+  }
+  c.CSsgc |= (0x01 << 7);
+}
+
+void bwc_set_qm(bwc_codec *const codec, uint8 const Qm) {
+  if ((int8_t)(kPrecBits - Qm) < 1) {
+    warn("invalid Q number format range; larger than the permitted 62 bits");
+    return;
+  }
+  codec->control.Qm = Qm;
+  codec->control.CSsgc |= (0x01 << 8);
+}
+
+// bwc_set_tiles (libbwc.c:2648-2828)
+void bwc_set_tiles(bwc_codec *const codec, uint64 const tX, uint64 const tY, uint64 const tZ, uint64 const tTS,
+                   bwc_tile_instr const instr) {
+  bwc_gl_ctrl &c = codec->control;
+  bwc_gl_inf &i = codec->info;
+  if (instr == bwc_tile_sizeof) {
+    if ((c.tileSizeX < 16 && c.tileSizeX > i.nX) || (c.tileSizeY < 16 && c.tileSizeY > i.nY) ||
+        (c.tileSizeZ < 16 && c.tileSizeZ > i.nZ) || (c.tileSizeTS < 16 && c.tileSizeTS > i.nTS)) {
+      warn("invalid tile dimensions; 16 <= tile size <= grid points");
+      return;
+    }
+    c.tileSizeX = (i.nX >> 1) ? (tX ? tX : i.nX) : i.nX;
+    c.tileSizeY = (i.nY >> 1) ? (tY ? tY : i.nY) : i.nY;
+    c.tileSizeZ = (i.nZ >> 1) ? (tZ ? tZ : i.nZ) : i.nZ;
+    c.tileSizeTS = (i.nTS >> 1) ? (tTS ? tTS : i.nTS) : i.nTS;
+    const uint64_t nx = (uint64_t)std::ceil(((float)i.nX / c.tileSizeX));
+    const uint64_t ny = (uint64_t)std::ceil(((float)i.nY / c.tileSizeY));
+    const uint64_t nz = (uint64_t)std::ceil(((float)i.nZ / c.tileSizeZ));
+    const uint64_t nt = (uint64_t)std::ceil(((float)i.nTS / c.tileSizeTS));
+    c.nTiles = nx * ny * nz * nt;
+    if (((double)nx * ny * nz * nt) > (double)0xFFFFFFFFFFFFFFFFull) {
+      warn("invalid tile dimensions; the number of tiles must stay below 2^64");
+      c.tileSizeX = i.nX, c.tileSizeY = i.nY, c.tileSizeZ = i.nZ, c.tileSizeTS = i.nTS;
+      return;
+    }
+  } else {
+    if (((double)tX * tY * tZ * tTS) > (double)0xFFFFFFFFFFFFFFFFull) {
+      warn("invalid number of tiles; must stay below 2^64");
+      return;
+    }
+    c.tileSizeX = (i.nX >> 1) ? (tX ? (uint64_t)std::ceil(((float)i.nX / tX)) : i.nX) : i.nX;
+    c.tileSizeY = (i.nY >> 1) ? (tY ? (uint64_t)std::ceil(((float)i.nY / tY)) : i.nY) : i.nY;
+    c.tileSizeZ = (i.nZ >> 1) ? (tZ ? (uint64_t)std::ceil(((float)i.nZ / tZ)) : i.nZ) : i.nZ;
+    c.tileSizeTS = (i.nTS >> 1) ? (tTS ? (uint64_t)std::ceil(((float)i.nTS / tTS)) : i.nTS) : i.nTS;
+    if ((c.tileSizeX < 16 && c.tileSizeX > i.nX) || (c.tileSizeY < 16 && c.tileSizeY > i.nY) ||
+        (c.tileSizeZ < 16 && c.tileSizeZ > i.nZ) || (c.tileSizeTS < 16 && c.tileSizeTS > i.nTS)) {
+      warn("invalid number of tiles; tile sizes must satisfy 16 <= size <= grid points");
+      c.tileSizeX = i.nX, c.tileSizeY = i.nY, c.tileSizeZ = i.nZ, c.tileSizeTS = i.nTS;
+      return;
+    }
+    const uint64_t nx = (uint64_t)std::ceil(((float)i.nX / c.tileSizeX));
+    const uint64_t ny = (uint64_t)std::ceil(((float)i.nY / c.tileSizeY));
+    const uint64_t nz = (uint64_t)std::ceil(((float)i.nZ / c.tileSizeZ));
+    const uint64_t nt = (uint64_t)std::ceil(((float)i.nTS / c.tileSizeTS));
+    c.nTiles = nx * ny * nz * nt;
+  }
+  c.CSsgc |= (0x01 << 9);
+}
+
+uchar create_codec(bwc_codec *const codec) {
+  if (!codec->tile) codec->tile = new Job();
+  Job *job = job_of(codec);
+  // the gain table is read when the geometry is built, whatever was
+  // accumulated into it by then (libbwc.c:450 via create_codec)
+  std::memcpy(job->lut, gain_lut(), sizeof job->lut);
+  job->plan = Plan();
+  if (build_plan(codec, job)) return 1;
+  job->created = true;
+  return 0;
+}
+
+// bwc_create_compression (libbwc.c:2970-3131)
+uchar bwc_create_compression(bwc_codec *const codec, bwc_stream *const stream, char *const rate_control) {
+  bwc_gl_ctrl &c = codec->control;
+  std::vector<float> br;
+  if (rate_control == nullptr) {
+    br.push_back(kPrecBits + 1);
+  } else {
+    char *save = nullptr;
+    char *tok = strtok_r(rate_control, "-, ", &save);
+    if (!tok) br.push_back(kPrecBits + 1);
+    for (; tok; tok = strtok_r(nullptr, "-, ", &save)) {
+      const float f = std::strtof(tok, nullptr);
+      if (f <= 0 || f > 64) {
+        std::fprintf(stderr, "bwc: error: invalid bitrate value %f; valid range 0 < bitrate <= 64\n", f);
+        return 1;
+      }
+      // sorted insert, dropping values within 0.01 of an existing one
+      int l = (int)br.size();
+      br.push_back(0.0f);
+      for (; l >= 0; --l) {
+        if (l && f < br[l - 1]) {
+          br[l] = br[l - 1];
+        } else if (l && std::fabs(f - br[l - 1]) < 0.01f) {
+          for (; l < (int)br.size() - 1; ++l) br[l] = br[l + 1];
+          br.pop_back();
+          break;
+        } else {
+          br[l] = f;
+          break;
+        }
+      }
+    }
+  }
+  std::free(c.bitrate);
+  c.bitrate = static_cast<float *>(std::malloc(br.size() * sizeof(float)));
+  std::memcpy(c.bitrate, br.data(), br.size() * sizeof(float));
+  c.nLayers = (uint8_t)br.size();
+  if (create_codec(codec)) return 1;
+  c.headerSize = header_size(codec, stream);
+  if (c.headerSize >= 0xFFFFFFFF) {
+    set_err("main header exceeds the maximum size of 4294967295 bytes");
+    return 1;
+  }
+  return 0;
+}
+
+// bwc_compress (libbwc.c:3145-3438)
+size_t bwc_compress(bwc_codec *const codec, bwc_stream *const stream) {
+  Job *job = job_of(codec);
+  if (!job || !job->created) {
+    set_err("bwc_compress called before bwc_create_compression");
+    return 0;
+  }
+  gpu::Workspace *ws = gpu::workspace_of(job);
+  if (!ws) {
+    set_err(gpu::last_error());
+    return 0;
+  }
+  bwc_gl_ctrl &ctl = codec->control;
+  const double t_start = now_ms();
+  reset_for_encode(codec, job);
+  const bool on_dev = gpu::is_device_pointer(stream->inp) != 0;
+  uint64_t maxsz = 0;
+  for (auto &tl : job->plan.tiles) maxsz = std::max<uint64_t>(maxsz, tl.box.volume());
+  if (gpu::ensure_work(ws, maxsz)) {
+    set_err(gpu::last_error());
+    return 0;
+  }
+  double ms[10] = {0};
+  int dwt_launch = 0;
+  for (uint64_t ti = 0; ti < job->plan.tiles.size(); ++ti) {
+    Tile &tl = job->plan.tiles[ti];
+    for (int p = 0; p < codec->info.nPar; ++p) {
+      Param &par = tl.par[p];
+      gpu::record(ws, 0);
+      if (gpu::fill_minmax(ws, codec, tl, p, stream->inp, on_dev, ws->d_work)) goto fail;
+      gpu::record(ws, 1);
+      if (gpu::normalize(ws, ws->d_work, par.box.volume(), ctl.Qm)) goto fail;
+      gpu::record(ws, 2);
+      if (gpu::dwt_forward(ws, codec, par, ws->d_work)) goto fail;
+      gpu::record(ws, 3);
+      for (int a = 0; a < 4; ++a) (void)a;
+      {
+        double mm[4];
+        if (gpu::read_norm(ws, mm)) goto fail;
+        par.pmin = mm[0];
+        par.pmax = mm[1];
+        par.alpha = mm[2];
+        par.beta = mm[3];
+      }
+      ms[0] += gpu::elapsed(ws, 0, 1);
+      ms[1] += gpu::elapsed(ws, 1, 2);
+      ms[2] += gpu::elapsed(ws, 2, 3);
+      dwt_launch += 0;
+      gpu::record(ws, 4);
+      if (gpu::t1_encode(ws, codec, par, ws->d_work, job->rec[ti * codec->info.nPar + p], &tl.slope_min,
+                         &tl.slope_max))
+        goto fail;
+      gpu::record(ws, 5);
+      gpu::sync(ws);
+      ms[4] += gpu::elapsed(ws, 4, 5);
+    }
+    {
+      const double t0 = now_ms();
+      t2_rate_control(codec, job, ti);
+      ms[6] += now_ms() - t0;
+      const double t1 = now_ms();
+      const int nl = ctl.nLayers;
+      for (int p = 0; p < codec->info.nPar; ++p) {
+        CblkRecords &R = job->rec[ti * codec->info.nPar + p];
+        std::vector<uint32_t> len(R.n);
+        for (uint64_t i = 0; i < R.n; ++i) {
+          const int16_t cut = R.cp[i * (nl + 1) + nl];
+          len[i] = cut > 0 ? R.L[i * kMaxPasses + cut - 1] : 0;
+        }
+        if (gpu::gather(ws, R, len)) goto fail;
+      }
+      ms[7] += now_ms() - t1;
+      const double t2 = now_ms();
+      t2_packets(codec, job, ti);
+      ms[8] += now_ms() - t2;
+    }
+  }
+  {
+    const double t3 = now_ms();
+    const size_t n = assemble_codestream(codec, job, stream, static_cast<uint8_t *>(stream->out));
+    ms[8] += now_ms() - t3;
+    ms[9] = now_ms() - t_start;
+    std::memcpy(job->ms, ms, sizeof ms);
+    job->launches[0] = dwt_launch;
+    if (!n) set_err("codestream assembly failed");
+    return n;
+  }
+fail:
+  set_err(gpu::last_error());
+  return 0;
+}
+
+uchar bwc_create_decompression(bwc_codec *const codec, bwc_stream *const stream, uint8 const layer) {
+  (void)codec, (void)stream, (void)layer;
+  set_err("decompression is not available in this build");
+  return 1;
+}
+
+uchar bwc_decompress(bwc_codec *const codec, bwc_stream *const stream) {
+  (void)codec, (void)stream;
+  set_err("decompression is not available in this build");
+  return 1;
+}
+
+bwc_header *bwc_open_header(void *const inpbuf) {
+  (void)inpbuf;
+  set_err("header parsing is not available in this build");
+  return nullptr;
+}
+
+void bwc_close_header(bwc_header *const header) {
+  if (!header) return;
+  std::free(header->aux.memory);
+  std::free(header->com.memory);
+  std::free(header->control.bitrate);
+  std::free(header);
+}
+
+// ---------------------------------------------------------------------------
+// bwc_amd.h
+// ---------------------------------------------------------------------------
+int bwc_amd_version(void) { return gpu::device_count() > 0 ? 1 : 0; }
+
+void bwc_amd_reset_gain_lut(void) { reset_gain_lut(); }
+int bwc_amd_device_count(void) { return gpu::device_count(); }
+const char *bwc_amd_last_error(void) { return g_api_err[0] ? g_api_err : gpu::last_error(); }
+
+uint64_t bwc_amd_ncblk(bwc_codec *codec, uint64_t tile, uint8_t param) {
+  Job *job = job_of(codec);
+  if (!job || !job->created || tile >= job->plan.tiles.size() || param >= codec->info.nPar) return 0;
+  return job->plan.tiles[tile].par[param].ncb;
+}
+
+int bwc_amd_cblk_info(bwc_codec *codec, uint64_t tile, uint8_t param, uint32_t *box, double *sb, uint32_t *meta) {
+  Job *job = job_of(codec);
+  if (!job || !job->created || tile >= job->plan.tiles.size() || param >= codec->info.nPar) return 1;
+  const Param &par = job->plan.tiles[tile].par[param];
+  for (uint64_t i = 0; i < par.ncb; ++i) {
+    for (int k = 0; k < 8; ++k) box[8 * i + k] = par.cb_box[8 * i + k];
+    const Subband &s = par.res[par.cb_sbref[i] >> 8].sb[par.cb_sbref[i] & 0xFF];
+    sb[3 * i] = s.gain;
+    sb[3 * i + 1] = s.step;
+    sb[3 * i + 2] = s.step_eff;
+    meta[4 * i] = s.hb;
+    meta[4 * i + 1] = s.Kmax;
+    meta[4 * i + 2] = s.qt_exp;
+    meta[4 * i + 3] = par.cb_idx[i];
+  }
+  return 0;
+}
+
+int bwc_amd_param_dims(bwc_codec *codec, uint64_t tile, uint64_t dims[4]) {
+  Job *job = job_of(codec);
+  if (!job || !job->created || tile >= job->plan.tiles.size()) return 1;
+  for (int a = 0; a < 4; ++a) dims[a] = job->plan.tiles[tile].box.len(a);
+  return 0;
+}
+
+int bwc_amd_stage_dwt(bwc_codec *codec, uint64_t tile, uint8_t param, const void *input, double *out,
+                      double mm[4]) {
+  Job *job = job_of(codec);
+  if (!job || !job->created || tile >= job->plan.tiles.size() || param >= codec->info.nPar) return 1;
+  gpu::Workspace *ws = gpu::workspace_of(job);
+  if (!ws) return 1;
+  Tile &tl = job->plan.tiles[tile];
+  Param &par = tl.par[param];
+  const uint64_t n = par.box.volume();
+  if (gpu::ensure_work(ws, n)) return 1;
+  if (gpu::fill_minmax(ws, codec, tl, param, input, gpu::is_device_pointer(input) != 0, ws->d_work)) return 1;
+  if (gpu::normalize(ws, ws->d_work, n, codec->control.Qm)) return 1;
+  if (gpu::dwt_forward(ws, codec, par, ws->d_work)) return 1;
+  if (gpu::read_norm(ws, mm)) return 1;
+  return gpu::copy_d2h(ws, out, ws->d_work, n * sizeof(double));
+}
+
+int bwc_amd_stage_idwt(bwc_codec *codec, uint64_t tile, uint8_t param, double *coeffs) {
+  Job *job = job_of(codec);
+  if (!job || !job->created || tile >= job->plan.tiles.size() || param >= codec->info.nPar) return 1;
+  gpu::Workspace *ws = gpu::workspace_of(job);
+  if (!ws) return 1;
+  Param &par = job->plan.tiles[tile].par[param];
+  const uint64_t n = par.box.volume();
+  if (gpu::ensure_work(ws, n)) return 1;
+  if (gpu::copy_h2d(ws, ws->d_work, coeffs, n * sizeof(double))) return 1;
+  if (gpu::dwt_inverse(ws, codec, par, ws->d_work)) return 1;
+  return gpu::copy_d2h(ws, coeffs, ws->d_work, n * sizeof(double));
+}
+
+int bwc_amd_stage_t1(bwc_codec *codec, uint64_t tile, uint8_t param, const double *coeffs, uint8_t *K,
+                     uint8_t *Kmsbs, uint8_t *Z, uint32_t *L, uint16_t *S, uint8_t *data, uint64_t data_cap,
+                     uint64_t *data_off, uint16_t slopes[2]) {
+  Job *job = job_of(codec);
+  if (!job || !job->created || tile >= job->plan.tiles.size() || param >= codec->info.nPar) return 1;
+  gpu::Workspace *ws = gpu::workspace_of(job);
+  if (!ws) return 1;
+  Param &par = job->plan.tiles[tile].par[param];
+  const uint64_t n = par.box.volume();
+  if (gpu::ensure_work(ws, n)) return 1;
+  if (gpu::copy_h2d(ws, ws->d_work, coeffs, n * sizeof(double))) return 1;
+  CblkRecords R;
+  uint16_t smin = 0xFFFF, smax = 0;
+  if (gpu::t1_encode(ws, codec, par, ws->d_work, R, &smin, &smax)) return 1;
+  std::vector<uint32_t> len(R.n);
+  uint64_t tot = 0;
+  for (uint64_t i = 0; i < R.n; ++i) {
+    len[i] = R.Z[i] ? R.L[i * kMaxPasses + R.Z[i] - 1] : 0;
+    tot += len[i];
+  }
+  if (tot > data_cap) {
+    set_err("data buffer too small");
+    return 2;
+  }
+  if (gpu::gather(ws, R, len)) return 1;
+  std::memcpy(K, R.K.data(), R.n);
+  std::memcpy(Kmsbs, R.Kmsbs.data(), R.n);
+  std::memcpy(Z, R.Z.data(), R.n);
+  std::memcpy(L, R.L.data(), R.n * kMaxPasses * sizeof(uint32_t));
+  std::memcpy(S, R.S.data(), R.n * (kMaxPasses + 1) * sizeof(uint16_t));
+  if (tot) std::memcpy(data, R.data.data(), tot);
+  for (uint64_t i = 0; i < R.n; ++i) data_off[i] = R.data_off[i];
+  data_off[R.n] = tot;
+  slopes[0] = smin;
+  slopes[1] = smax;
+  return 0;
+}
+
+int bwc_amd_set_records(bwc_codec *codec, uint64_t tile, uint8_t param, const uint8_t *K, const uint8_t *Kmsbs,
+                        const uint8_t *Z, const uint32_t *L, const uint16_t *S, const uint8_t *data,
+                        const uint64_t *data_off, const double mm[4]) {
+  Job *job = job_of(codec);
+  if (!job || !job->created || tile >= job->plan.tiles.size() || param >= codec->info.nPar) return 1;
+  if (job->rec.size() != job->plan.tiles.size() * codec->info.nPar) reset_for_encode(codec, job);
+  Tile &tl = job->plan.tiles[tile];
+  Param &par = tl.par[param];
+  CblkRecords &R = job->rec[tile * codec->info.nPar + param];
+  R.resize(par.ncb, codec->control.nLayers);
+  std::memcpy(R.K.data(), K, par.ncb);
+  std::memcpy(R.Kmsbs.data(), Kmsbs, par.ncb);
+  std::memcpy(R.Z.data(), Z, par.ncb);
+  std::memcpy(R.L.data(), L, par.ncb * kMaxPasses * sizeof(uint32_t));
+  std::memcpy(R.S.data(), S, par.ncb * (kMaxPasses + 1) * sizeof(uint16_t));
+  R.data.assign(data, data + data_off[par.ncb]);
+  for (uint64_t i = 0; i < par.ncb; ++i) {
+    R.data_off[i] = data_off[i];
+    if (R.K[i] == 0) continue;
+    for (int z = 0; z <= R.Z[i]; ++z) {
+      const uint16_t s = R.S[i * (kMaxPasses + 1) + z];
+      if (s) {
+        tl.slope_min = std::min(tl.slope_min, s);
+        tl.slope_max = std::max(tl.slope_max, s);
+      }
+    }
+  }
+  par.pmin = mm[0], par.pmax = mm[1], par.alpha = mm[2], par.beta = mm[3];
+  return 0;
+}
+
+size_t bwc_amd_encode_records(bwc_codec *codec, bwc_stream *stream) {
+  Job *job = job_of(codec);
+  if (!job || !job->created || job->rec.size() != job->plan.tiles.size() * codec->info.nPar) return 0;
+  for (uint64_t ti = 0; ti < job->plan.tiles.size(); ++ti) {
+    t2_rate_control(codec, job, ti);
+    t2_packets(codec, job, ti);
+  }
+  return assemble_codestream(codec, job, stream, static_cast<uint8_t *>(stream->out));
+}
+
+void bwc_amd_begin_records(bwc_codec *codec) {
+  Job *job = job_of(codec);
+  if (job && job->created) reset_for_encode(codec, job);
+}
+
+int bwc_amd_last_timings(bwc_codec *codec, double ms[10], int launches[4]) {
+  Job *job = job_of(codec);
+  if (!job) return 1;
+  std::memcpy(ms, job->ms, sizeof job->ms);
+  std::memcpy(launches, job->launches, sizeof job->launches);
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,157 @@
+// Internal data model of libbwc (MI355X build).
+//
+// The public bwc_codec (include/bwc.h) keeps the reference layout; its opaque
+// `tile` pointer holds a Job: the codec plan (tiles -> parameters ->
+// resolutions -> subbands -> precincts -> codeblocks, flattened into arrays
+// so device kernels and tier-2 share one codeblock index) plus the encoder
+// state between bwc_create_compression and bwc_compress.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/bwc.h"
+
+namespace bwc {
+
+constexpr int kMaxPasses = 190;     // (64 * 3) - 2 fractional bitplanes, FP64 build
+constexpr int kPrecBits = 63;       // PREC_BIT of the FP64 build
+constexpr int kPrecBytes = 8;
+
+struct Box {
+  uint64_t lo[4] = {0, 0, 0, 0};
+  uint64_t hi[4] = {0, 0, 0, 0};
+  uint64_t len(int a) const { return hi[a] - lo[a]; }
+  uint64_t volume() const { return len(0) * len(1) * len(2) * len(3); }
+};
+
+// One tag tree (JPEG 2000 style, 4-D leaves) stored as flat node arrays.
+struct TagTree {
+  std::vector<uint16_t> value, threshold;
+  std::vector<int64_t> parent;   // -1 at the root
+  void build(uint64_t lx, uint64_t ly, uint64_t lz, uint64_t lt);
+  void set(uint64_t leaf, uint16_t v);
+  void reset_thresholds();
+};
+
+struct Precinct {
+  uint32_t box[8];           // precinct bounds in subband coordinates (x0..t0, x1..t1)
+  uint16_t ncb[4];
+  uint64_t ncbs = 0;
+  uint64_t first_cb = 0;     // index of its first codeblock in Param::cb_* arrays
+  TagTree incl, msbs;
+};
+
+struct Subband {
+  Box box;
+  double gain = 1, step = 1, step_eff = 1;
+  uint8_t hb = 0;
+  uint16_t Kmax = 0;
+  uint8_t qt_exp = 0;
+  uint16_t qt_mant = 0;
+  std::vector<Precinct> prec;   // one per precinct of the resolution
+};
+
+struct Packet {
+  std::vector<uint8_t> header;
+  uint64_t body_size = 0;
+  uint8_t e = 0;
+  uint32_t size = 0;
+  bool built = false;
+};
+
+struct Resolution {
+  Box box;
+  uint16_t np[4] = {0, 0, 0, 0};
+  uint64_t nprec = 0;
+  std::vector<Subband> sb;
+  std::vector<Packet> pk;       // [layer * nprec + p]
+};
+
+struct Param {
+  Box box;
+  double pmin = 0, pmax = 0, alpha = 0, beta = 0;
+  std::vector<Resolution> res;
+  // codeblocks in the reference's access order (subband, precinct, raster)
+  uint64_t ncb = 0;
+  std::vector<uint32_t> cb_box;      // 8 per codeblock, tile-local buffer coordinates
+  std::vector<uint32_t> cb_sbref;    // (resolution << 8) | subband index
+  std::vector<uint32_t> cb_prec;     // precinct index within the resolution
+  std::vector<uint32_t> cb_idx;      // index within its precinct
+};
+
+struct Tile {
+  Box box;
+  uint32_t index = 0;
+  uint32_t header_size = 18;
+  uint64_t body_size = 0;
+  uint64_t npackets = 0;
+  uint16_t slope_min = 0xFFFF, slope_max = 0;
+  std::vector<Param> par;
+  std::vector<Packet *> seq;
+};
+
+// Per-codeblock tier-1 output consumed by tier-2 (host copy).
+struct CblkRecords {
+  uint64_t n = 0;
+  std::vector<uint8_t> K, Kmsbs, Z;
+  std::vector<uint32_t> L;      // kMaxPasses per codeblock, bytes
+  std::vector<uint16_t> S;      // kMaxPasses + 1 per codeblock
+  // tier-2 state
+  std::vector<uint16_t> beta, beta_est;
+  std::vector<int16_t> cp;      // (nLayers + 1) per codeblock, cp[0] == 0 is layer -1
+  // included bytes, packed after rate control: data_off[i] into `data`
+  std::vector<uint64_t> data_off;
+  std::vector<uint8_t> data;
+  void resize(uint64_t ncb, int nlayers);
+};
+
+struct Plan {
+  bwc_codec *codec = nullptr;
+  std::vector<Tile> tiles;
+};
+
+// Whole library-side state hanging off bwc_codec::tile.
+struct Job {
+  Plan plan;
+  double lut[3][10];                 // gain LUT snapshot used for this codec
+  std::vector<float> bitrate;        // owned copy; codec->control.bitrate points here
+  bool created = false;
+  // per (tile, param) records, in tile-major order
+  std::vector<CblkRecords> rec;
+  // decoder-side
+  std::vector<uint8_t> cs_copy;
+  // device workspace (gpu::Workspace), created on first use
+  void *ws = nullptr;
+  void (*ws_free)(void *) = nullptr;
+  // stage timings of the last compression (bwc_amd_last_timings)
+  double ms[10] = {0};
+  int launches[4] = {0};
+  ~Job() {
+    if (ws && ws_free) ws_free(ws);
+  }
+};
+
+// ---- configuration / geometry (geometry.cpp) ----
+void gain_lut_accumulate();                    // one initialize_gain_lut call (process global)
+const double (*gain_lut())[10];
+void reset_gain_lut();                         // back to the state of a fresh process
+double energy_gain(const bwc_codec *c, const double lut[3][10], uint8_t hb, uint16_t level);
+void compute_qt(bwc_gl_ctrl *ctl, double delta);
+int build_plan(bwc_codec *codec, Job *job);
+Job *job_of(bwc_codec *c);
+
+// ---- tier-2 + codestream (tier2.cpp) ----
+int t2_encode_tile(bwc_codec *codec, Job *job, uint64_t tile);
+size_t assemble_codestream(bwc_codec *codec, Job *job, const bwc_stream *st, uint8_t *out);
+uint64_t header_size(const bwc_codec *codec, const bwc_stream *st);
+
+// ---- decode (decode.cpp) ----
+int parse_codestream(bwc_codec *codec, Job *job, const uint8_t *cs, uint8_t layer);
+int decompress(bwc_codec *codec, Job *job, void *out);
+
+void warn(const char *msg);
+
+}  // namespace bwc

@@ -1,0 +1,1246 @@
+// gfx950 kernels of the compression path and their host-side launchers.
+//
+//   fill_minmax_kernel    tile copy to f64 + min/max      (libbwc.c:622-812)
+//   normalize_kernel      (x-a)*b, exponent += Qm          (libbwc.c:993-1078)
+//   dwt97_axis_kernel     one lifting pass over many lines (dwt.c:133-404, :645-721, :1216-1724)
+//   quantize_kernel       |c|/delta -> stripe columns, K   (tier1.c:526-729)
+//   t1_encode_kernel      passes + MQ + lengths + slopes   (tier1.c:962-1511, :2100-2588; mq.c)
+//   gather_kernel         pack included codeblock prefixes
+//
+// Arithmetic that decides quantisation indices is kept in IEEE binary64 with
+// the reference's operand order; the library is compiled with
+// -ffp-contract=off so no FMA is formed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+
+#include "gpu.hpp"
+
+namespace bwc {
+namespace gpu {
+
+static thread_local char g_err[256];
+const char *last_error() { return g_err; }
+
+#define HIP_OK(x)                                                                                      \
+  do {                                                                                                 \
+    hipError_t e_ = (x);                                                                               \
+    if (e_ != hipSuccess) {                                                                            \
+      std::snprintf(g_err, sizeof g_err, "%s:%d %s: %s", __FILE__, __LINE__, #x, hipGetErrorString(e_)); \
+      std::fprintf(stderr, "bwc: HIP error: %s\n", g_err);                                             \
+      return 1;                                                                                        \
+    }                                                                                                  \
+  } while (0)
+
+#define CHK(x)       \
+  do {               \
+    if ((x)) return 1; \
+  } while (0)
+
+Workspace::~Workspace() {
+  if (device < 0) return;
+  (void)hipSetDevice(device);
+  void *ptrs[] = {d_input, d_work,  d_red,  d_norm, d_cbbox, d_cbsb, d_cbmeta, d_qoff,   d_doff, d_q,
+                  d_state, d_data,  d_K,    d_Kmsbs, d_Z,    d_L,    d_S,      d_snap,   d_mse,  d_flags,
+                  d_packed, d_gather};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (h_pinned) (void)hipHostFree(h_pinned);
+  for (auto &e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+static int ensure(void **p, size_t *cap, size_t bytes) {
+  if (*p && *cap >= bytes) return 0;
+  if (*p) HIP_OK(hipFree(*p));
+  *p = nullptr;
+  size_t nb = std::max<size_t>(bytes, 256);
+  HIP_OK(hipMalloc(p, nb));
+  *cap = nb;
+  return 0;
+}
+
+// ===========================================================================
+// fill + min/max, normalize
+// ===========================================================================
+__global__ void fill_minmax_kernel(const void *__restrict__ in, int prec, uint64_t nX, uint64_t nY, uint64_t nZ,
+                                   uint64_t poff, uint64_t x0, uint64_t y0, uint64_t z0, uint64_t t0, uint64_t w,
+                                   uint64_t h, uint64_t d, uint64_t n, double *__restrict__ out,
+                                   double *__restrict__ red) {
+  double mx = -1.7976931348623157e+308, mn = 1.7976931348623157e+308;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t x = i % w, r = i / w;
+    const uint64_t y = r % h, r2 = r / h;
+    const uint64_t z = r2 % d, t = r2 / d;
+    const uint64_t src = poff + x0 + x + nX * (y0 + y + nY * (z0 + z + nZ * (t0 + t)));
+    const double v = prec == 8 ? ((const double *)in)[src] : (double)((const float *)in)[src];
+    out[i] = v;
+    mx = (mx < v) ? v : mx;
+    mn = (mn > v) ? v : mn;
+  }
+  __shared__ double smx[256], smn[256];
+  smx[threadIdx.x] = mx;
+  smn[threadIdx.x] = mn;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      smx[threadIdx.x] = (smx[threadIdx.x] < smx[threadIdx.x + s]) ? smx[threadIdx.x + s] : smx[threadIdx.x];
+      smn[threadIdx.x] = (smn[threadIdx.x] > smn[threadIdx.x + s]) ? smn[threadIdx.x + s] : smn[threadIdx.x];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    red[2 * blockIdx.x] = smn[0];
+    red[2 * blockIdx.x + 1] = smx[0];
+  }
+}
+
+// reduce the partials; alpha = (max+min)/2, beta = 2/(max-min) (libbwc.c:810-811)
+__global__ void minmax_final_kernel(const double *red, int nblk, double *norm) {
+  __shared__ double smx[256], smn[256];
+  double mx = -1.7976931348623157e+308, mn = 1.7976931348623157e+308;
+  for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
+    mn = (mn > red[2 * i]) ? red[2 * i] : mn;
+    mx = (mx < red[2 * i + 1]) ? red[2 * i + 1] : mx;
+  }
+  smx[threadIdx.x] = mx;
+  smn[threadIdx.x] = mn;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      smx[threadIdx.x] = (smx[threadIdx.x] < smx[threadIdx.x + s]) ? smx[threadIdx.x + s] : smx[threadIdx.x];
+      smn[threadIdx.x] = (smn[threadIdx.x] > smn[threadIdx.x + s]) ? smn[threadIdx.x + s] : smn[threadIdx.x];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    norm[0] = smn[0];
+    norm[1] = smx[0];
+    norm[2] = (smx[0] + smn[0]) / 2.0f;
+    norm[3] = 2.0f / (smx[0] - smn[0]);
+  }
+}
+
+__global__ void normalize_kernel(double *__restrict__ data, uint64_t n, const double *__restrict__ norm,
+                                 uint64_t shift) {
+  const double alpha = norm[2], beta = norm[3];
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const double v = (data[i] - alpha) * beta;
+    uint64_t raw = __double_as_longlong(v);
+    raw += ((raw & 0x7FF0000000000000ull) == 0 && (raw & 0x000FFFFFFFFFFFFFull) == 0) ? 0 : shift;
+    data[i] = __longlong_as_double((long long)raw);
+  }
+}
+
+int fill_minmax(Workspace *ws, const bwc_codec *c, const Tile &tl, int param, const void *input, bool on_dev,
+                double *work) {
+  const Param &par = tl.par[param];
+  const uint64_t n = par.box.volume();
+  const uint64_t nX = c->info.nX, nY = c->info.nY, nZ = c->info.nZ, nTS = c->info.nTS;
+  const int prec = c->info.data_prec == bwc_precision_single ? 4 : 8;
+  const uint64_t poff = nX * nY * nZ * nTS * (uint64_t)param;
+  const void *src = input;
+  uint64_t srcoff = poff;
+  uint64_t sx = nX, sy = nY, sz = nZ;
+  uint64_t ox = tl.box.lo[0], oy = tl.box.lo[1], oz = tl.box.lo[2], ot = tl.box.lo[3];
+  if (!on_dev) {
+    // copy just this tile's parameter slab (whole rows of the global field)
+    const uint64_t slab = nX * nY * nZ * nTS;
+    CHK(ensure(&ws->d_input, &ws->input_cap, slab * prec));
+    HIP_OK(hipMemcpyAsync(ws->d_input, (const char *)input + poff * prec, slab * prec, hipMemcpyHostToDevice,
+                          ws->stream));
+    src = ws->d_input;
+    srcoff = 0;
+  }
+  (void)sx, (void)sy, (void)sz;
+  const int threads = 256;
+  const int blocks = (int)std::min<uint64_t>((n + threads - 1) / threads, 4096);
+  size_t redcap = 0;
+  if (!ws->d_red) HIP_OK(hipMalloc((void **)&ws->d_red, 2 * 4096 * sizeof(double)));
+  if (!ws->d_norm) HIP_OK(hipMalloc((void **)&ws->d_norm, 4 * sizeof(double)));
+  (void)redcap;
+  hipLaunchKernelGGL(fill_minmax_kernel, dim3(blocks), dim3(threads), 0, ws->stream, src, prec, nX, nY, nZ, srcoff, ox,
+                     oy, oz, ot, par.box.len(0), par.box.len(1), par.box.len(2), n, work, ws->d_red);
+  hipLaunchKernelGGL(minmax_final_kernel, dim3(1), dim3(256), 0, ws->stream, ws->d_red, blocks, ws->d_norm);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+int normalize(Workspace *ws, double *work, uint64_t n, uint8_t Qm) {
+  const int threads = 256;
+  const int blocks = (int)std::min<uint64_t>((n + threads - 1) / threads, 8192);
+  hipLaunchKernelGGL(normalize_kernel, dim3(blocks), dim3(threads), 0, ws->stream, work, n, ws->d_norm,
+                     (uint64_t)Qm << 52);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+// ===========================================================================
+// DWT: one axis pass of one level over all lines of the current LL box.
+// A workgroup stages G adjacent lines in LDS (line-major, E-sample whole-point
+// symmetric extension on both ends), runs the lifting steps over exactly the
+// reference's index ranges, and writes lowpass | highpass back in place.
+// Lines are read either along the line (contiguous X pass) or across lines
+// (strided Y/Z/T passes, adjacent lines adjacent in memory) for coalescing.
+// ===========================================================================
+#define DWT_E 8   // extension slots per side (reference uses 6 for 9/7, 4 for 5/3)
+
+__device__ __forceinline__ int64_t mirror_index(int64_t j, int64_t len) {
+  // whole-point symmetric reflection about 0 and len-1, repeated for short
+  // lines (dwt.c:286-342)
+  if (len == 1) return 0;
+  while (j < 0 || j > len - 1) j = (j < 0) ? -j : 2 * (len - 1) - j;
+  return j;
+}
+
+template <int KERN, bool FWD>
+__global__ void dwt_axis_kernel(double *__restrict__ data, int64_t len, int par0, uint64_t sa, uint64_t n0,
+                                uint64_t s0, uint64_t n1, uint64_t s1, uint64_t s2, int G, int contiguous) {
+  extern __shared__ double lds[];
+  const int64_t pitch = len + 2 * DWT_E + 1;
+  const uint64_t g0 = (uint64_t)blockIdx.x * G;
+  const uint64_t i1 = blockIdx.y % n1, i2 = blockIdx.y / n1;
+  const int ng = (int)std::min<uint64_t>((uint64_t)G, n0 - g0);
+  double *base = data + g0 * s0 + i1 * s1 + i2 * s2;
+  const int64_t tot = (int64_t)ng * len;
+  const int nt = blockDim.x;
+  const int64_t half = par0 ? (len >> 1) : ((len + 1) >> 1);
+
+  // ---- load ----
+  for (int64_t e = threadIdx.x; e < tot; e += nt) {
+    int64_t g, p;
+    if (contiguous) {
+      g = e / len;
+      p = e % len;
+    } else {
+      g = e % ng;
+      p = e / ng;
+    }
+    double v;
+    if (FWD) {
+      v = base[g * s0 + p * sa];
+    } else {
+      // inverse: interleave lowpass [0,half) and highpass [half,len)
+      // (fill_inverse_buffer, dwt.c:169-272): sample p comes from
+      // low[(p - par0)/2] when p - par0 is even, else high[(p + par0 - 1)/2]
+      const int64_t q = p - par0;
+      const int64_t src = (q >= 0 && (q & 1) == 0) ? (q >> 1) : (half + ((p + par0 - 1) >> 1));
+      v = base[g * s0 + src * sa];
+    }
+    lds[g * pitch + DWT_E + p] = v;
+  }
+  __syncthreads();
+  // ---- symmetric extension ----
+  for (int64_t e = threadIdx.x; e < (int64_t)ng * 2 * DWT_E; e += nt) {
+    const int64_t g = e / (2 * DWT_E), k = e % (2 * DWT_E);
+    const int64_t p = (k < DWT_E) ? (k - DWT_E) : (len + (k - DWT_E));
+    lds[g * pitch + DWT_E + p] = lds[g * pitch + DWT_E + mirror_index(p, len)];
+  }
+  __syncthreads();
+  // working index i addresses line position i + par0 (dwt.c:1393-1396)
+  const int64_t w0 = DWT_E + par0;
+  if (KERN == 0) {
+    const double A = -1.586134342059924f, B = -0.052980118572961f, Gm = 0.882911075530934f,
+                 D = 0.360523644801462f, KH = 1.230174104914001f, KL = 0.812893066115961f;
+    if (FWD) {
+      // four lifting steps with their extended ranges (dwt.c:371-403)
+      const int64_t lo[4] = {-3, -2, -1, 0}, hi[4] = {len + 3, len + 2, len + 1, len};
+      for (int st = 0; st < 4; ++st) {
+        const int64_t cnt = (hi[st] - lo[st] + 1) / 2;
+        for (int64_t e = threadIdx.x; e < (int64_t)ng * cnt; e += nt) {
+          const int64_t g = e / cnt, m = e % cnt;
+          double *w = lds + g * pitch + w0;
+          const int64_t i = lo[st] + 2 * m;
+          if (st == 0) w[i] += A * (w[i - 1] + w[i + 1]);
+          else if (st == 1) w[i] += B * (w[i - 1] + w[i + 1]);
+          else if (st == 2) { w[i] += Gm * (w[i - 1] + w[i + 1]); w[i] = KH * w[i]; }
+          else { w[i] += D * (w[i - 1] + w[i + 1]); w[i] = KL * w[i]; }
+        }
+        __syncthreads();
+      }
+    } else {
+      // inverse steps (dwt.c:428-465)
+      const int64_t lo[4] = {-4, -5, -4, -3}, hi[4] = {len + 4, len + 3, len + 2, len + 1};
+      for (int st = 0; st < 4; ++st) {
+        const int64_t cnt = (hi[st] - lo[st] + 1) / 2;
+        for (int64_t e = threadIdx.x; e < (int64_t)ng * cnt; e += nt) {
+          const int64_t g = e / cnt, m = e % cnt;
+          double *w = lds + g * pitch + w0;
+          const int64_t i = lo[st] + 2 * m;
+          if (st == 0) { w[i] = KH * w[i]; w[i] -= D * (w[i - 1] + w[i + 1]); }
+          else if (st == 1) { w[i] = KL * w[i]; w[i] -= Gm * (w[i - 1] + w[i + 1]); }
+          else if (st == 2) w[i] -= B * (w[i - 1] + w[i + 1]);
+          else w[i] -= A * (w[i - 1] + w[i + 1]);
+        }
+        __syncthreads();
+      }
+    }
+  } else {
+    if (FWD) {
+      const int64_t lo[2] = {-1, 0}, hi[2] = {len + 1, len};
+      for (int st = 0; st < 2; ++st) {
+        const int64_t cnt = (hi[st] - lo[st] + 1) / 2;
+        for (int64_t e = threadIdx.x; e < (int64_t)ng * cnt; e += nt) {
+          const int64_t g = e / cnt, m = e % cnt;
+          double *w = lds + g * pitch + w0;
+          const int64_t i = lo[st] + 2 * m;
+          if (st == 0) w[i] -= (w[i - 1] + w[i + 1]) / 2;
+          else w[i] += (w[i - 1] + w[i + 1] + 2) / 4;
+        }
+        __syncthreads();
+      }
+    } else {
+      const int64_t lo[2] = {-2, -1}, hi[2] = {len + 1, len};
+      for (int st = 0; st < 2; ++st) {
+        const int64_t cnt = (hi[st] - lo[st] + 1) / 2;
+        for (int64_t e = threadIdx.x; e < (int64_t)ng * cnt; e += nt) {
+          const int64_t g = e / cnt, m = e % cnt;
+          double *w = lds + g * pitch + w0;
+          const int64_t i = lo[st] + 2 * m;
+          if (st == 0) w[i] -= (w[i - 1] + w[i + 1] + 2) / 4;
+          else w[i] += (w[i - 1] + w[i + 1]) / 2;
+        }
+        __syncthreads();
+      }
+    }
+  }
+  // ---- store ----
+  for (int64_t e = threadIdx.x; e < tot; e += nt) {
+    int64_t g, o;
+    if (contiguous) {
+      g = e / len;
+      o = e % len;
+    } else {
+      g = e % ng;
+      o = e / ng;
+    }
+    double v;
+    if (FWD) {
+      // lowpass o < half from position 2o + par0, highpass from 2(o-half) + 1 - par0
+      // (buffer_flush_forward, dwt.c:645-721)
+      const int64_t p = (o < half) ? (2 * o + par0) : (2 * (o - half) + 1 - par0);
+      v = lds[g * pitch + DWT_E + p];
+    } else {
+      v = lds[g * pitch + DWT_E + o];
+    }
+    base[g * s0 + o * sa] = v;
+  }
+}
+
+static int dwt_run(Workspace *ws, const bwc_codec *c, const Param &par, double *work, bool fwd) {
+  const bwc_gl_ctrl &ctl = c->control;
+  const uint64_t W = par.box.len(0), H = par.box.len(1), D = par.box.len(2);
+  const uint64_t inc[4] = {1, W, W * H, W * H * D};
+  const uint8_t dec[4] = {ctl.decompX, ctl.decompY, ctl.decompZ, ctl.decompTS};
+  const int kern[4] = {ctl.KernelX, ctl.KernelY, ctl.KernelZ, ctl.KernelTS};
+  const int nd = ctl.nDecomp;
+  for (int step = 0; step < nd; ++step) {
+    const int level = fwd ? step : nd - 1 - step;
+    const Box &rb = par.res[nd - level].box;
+    uint64_t sz[4];
+    for (int a = 0; a < 4; ++a) sz[a] = rb.len(a);
+    for (int oi = 0; oi < 4; ++oi) {
+      const int a = fwd ? oi : 3 - oi;
+      if (!(level < dec[a] && sz[a] > 1)) continue;
+      if (kern[a] == bwc_dwt_haar) {
+        std::snprintf(g_err, sizeof g_err, "Haar kernel has no boundary extension in the codec");
+        return 1;
+      }
+      // the three other axes; axis o[0] is the fastest in memory
+      int o[3], k = 0;
+      for (int b = 0; b < 4; ++b)
+        if (b != a) o[k++] = b;
+      const int64_t len = (int64_t)sz[a];
+      const int par0 = (int)((fwd ? rb.lo[a] : rb.lo[a]) & 1);
+      const bool contiguous = (a == 0);
+      // lines per workgroup: ~24 KiB of LDS for contiguous lines; at least
+      // 16 adjacent lines for strided axes so each row read spans 128 bytes
+      const int64_t pitch = len + 2 * DWT_E + 1;
+      int G = contiguous ? (int)std::max<int64_t>(1, std::min<int64_t>(64, 3072 / pitch))
+                         : (int)std::max<int64_t>(16, std::min<int64_t>(64, 8192 / pitch));
+      while (G > 1 && (size_t)G * pitch * sizeof(double) > 96 * 1024) --G;
+      if ((uint64_t)G > sz[o[0]]) G = (int)sz[o[0]];
+      const size_t lds = (size_t)G * (len + 2 * DWT_E + 1) * sizeof(double);
+      if (lds > 160 * 1024) {
+        std::snprintf(g_err, sizeof g_err, "DWT line of %lld samples exceeds the LDS line buffer", (long long)len);
+        return 1;
+      }
+      const uint64_t gx = (sz[o[0]] + G - 1) / G;
+      const uint64_t gy = sz[o[1]] * sz[o[2]];
+      const int threads = 256;
+      auto launch = [&](auto kfn) {
+        hipLaunchKernelGGL(kfn, dim3((uint32_t)gx, (uint32_t)gy), dim3(threads), lds, ws->stream, work, len, par0,
+                           inc[a], sz[o[0]], inc[o[0]], sz[o[1]], inc[o[1]], inc[o[2]], G, contiguous ? 1 : 0);
+      };
+      if (kern[a] == bwc_dwt_9_7) {
+        if (fwd) launch(dwt_axis_kernel<0, true>);
+        else launch(dwt_axis_kernel<0, false>);
+      } else {
+        if (fwd) launch(dwt_axis_kernel<1, true>);
+        else launch(dwt_axis_kernel<1, false>);
+      }
+      HIP_OK(hipGetLastError());
+    }
+  }
+  return 0;
+}
+
+int dwt_forward(Workspace *ws, const bwc_codec *c, const Param &par, double *work) {
+  return dwt_run(ws, c, par, work, true);
+}
+int dwt_inverse(Workspace *ws, const bwc_codec *c, const Param &par, double *work) {
+  return dwt_run(ws, c, par, work, false);
+}
+
+// ===========================================================================
+// quantisation into stripe columns (tier1.c:526-729)
+// ===========================================================================
+// (uint64_t) of a non-negative double exactly as x86-64 code compiled by gcc
+// computes it: truncation in range, 2^63 for NaN, 0 at or above 2^64.
+__device__ __forceinline__ uint64_t x86_to_u64(double v) {
+  const double two63 = 9223372036854775808.0;
+  if (v < two63) return (uint64_t)(int64_t)v;
+  if (v != v) return 0x8000000000000000ull;
+  const double r = v - two63;
+  if (r < two63) return ((uint64_t)(int64_t)r) ^ 0x8000000000000000ull;
+  return 0;
+}
+
+__global__ void quantize_kernel(const double *__restrict__ work, uint64_t W, uint64_t H, uint64_t D,
+                                const uint32_t *__restrict__ cbbox, const double *__restrict__ cbsb,
+                                const uint64_t *__restrict__ qoff, uint64_t *__restrict__ q,
+                                uint16_t *__restrict__ state, uint8_t *__restrict__ Kout, uint64_t ncb) {
+  __shared__ int smax[256];
+  for (uint64_t cb = blockIdx.x; cb < ncb; cb += gridDim.x) {
+    const uint32_t *b = cbbox + 8 * cb;
+    const uint64_t cw = b[4] - b[0], ch = b[5] - b[1], cd = b[6] - b[2], ct = b[7] - b[3];
+    const uint64_t ns = (ch + 3) / 4;
+    const uint64_t ncol = cw * ns * cd * ct;
+    const double step = cbsb[3 * cb];
+    const uint64_t qo = qoff[cb];
+    int kb = 0;
+    for (uint64_t k = threadIdx.x; k < ncol; k += blockDim.x) {
+      const uint64_t x = k % cw, j = (k / cw) % ns, sl = k / (cw * ns);
+      const uint64_t z = sl % cd, t = sl / cd;
+      uint16_t xi = 0;
+      for (int r = 0; r < 4; ++r) {
+        const uint64_t row = 4 * j + r;
+        uint64_t qq = 0;
+        if (row < ch) {
+          const uint64_t idx = (b[0] + x) + W * ((b[1] + row) + H * ((b[2] + z) + D * (b[3] + t)));
+          uint64_t raw = (uint64_t)__double_as_longlong(work[idx]);
+          xi |= (uint16_t)((raw >> 63) << (3 - r));
+          raw &= 0x7FFFFFFFFFFFFFFFull;
+          qq = x86_to_u64(__longlong_as_double((long long)raw) / step);
+          const int nb = qq ? 64 - __clzll((long long)qq) : 0;
+          kb = nb > kb ? nb : kb;
+        }
+        q[qo + 4 * k + (3 - r)] = qq;
+      }
+      state[qo / 4 + k] = (uint16_t)(xi << 12);
+    }
+    smax[threadIdx.x] = kb;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) smax[threadIdx.x] = max(smax[threadIdx.x], smax[threadIdx.x + s]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) Kout[cb] = (uint8_t)smax[0];
+    __syncthreads();
+  }
+}
+
+// ===========================================================================
+// tier-1: one lane per codeblock (reference-order passes + MQ coder)
+// ===========================================================================
+// MQ probability states (ITU-T T.800 Table C.2): qe | nmps << 16 | nlps << 22 | switch << 28
+__constant__ uint32_t c_mq[47];
+__constant__ int32_t c_tm[64];
+__constant__ uint8_t c_sig[3][45];
+__constant__ uint8_t c_sig2xi[16];
+__constant__ uint8_t c_xi2cont[11][2];
+__constant__ uint16_t c_frac2log[256];
+
+struct MqState {
+  int64_t L;
+  uint32_t C;
+  uint16_t A;
+  int8_t t;
+  uint8_t T;
+};
+
+struct Coder {
+  MqState s;
+  uint8_t *buf;
+  uint64_t cap;
+  uint32_t ovf;
+  uint8_t *ctx;   // LDS, stride 64: state index | mps << 7
+  int lane;
+  __device__ __forceinline__ uint8_t &cx(int k) { return ctx[k * 64 + lane]; }
+
+  __device__ __forceinline__ void put(int64_t L, uint8_t v) {
+    if (L >= 0) {
+      if ((uint64_t)L < cap) buf[L] = v;
+      else ovf = 1;
+    }
+  }
+  // transfer_byte (mq.c:85-127)
+  __device__ __forceinline__ void byte_out() {
+    if (s.T == 0xFF) {
+      put(s.L, s.T);
+      s.L++;
+      s.T = (uint8_t)(s.C >> 20);
+      s.C &= 0xFFFFF;
+      s.t = 7;
+    } else {
+      if (s.C & 0x8000000) {
+        s.T++;
+        s.C &= 0x7FFFFFF;
+      }
+      put(s.L, s.T);
+      s.L++;
+      if (s.T == 0xFF) {
+        s.T = (uint8_t)(s.C >> 20);
+        s.C &= 0xFFFFF;
+        s.t = 7;
+      } else {
+        s.T = (uint8_t)(s.C >> 19);
+        s.C &= 0x7FFFF;
+        s.t = 8;
+      }
+    }
+  }
+  // mq_bit_encode (mq.c:477-556)
+  __device__ __forceinline__ void encode(uint32_t sym, int k) {
+    uint8_t &st = cx(k);
+    const uint32_t e = c_mq[st & 0x3F];
+    const uint16_t p = (uint16_t)(e & 0xFFFF);
+    const uint32_t mps = st >> 7;
+    s.A -= p;
+    if (sym == mps) {
+      if (s.A >= 0x8000) {
+        s.C += p;
+        return;
+      }
+      if (s.A < p) s.A = p;
+      else s.C += p;
+      st = (uint8_t)(((e >> 16) & 0x3F) | (mps << 7));
+    } else {
+      if (s.A < p) s.C += p;
+      else s.A = p;
+      const uint32_t nm = ((e >> 28) & 1) ? (mps ^ 1) : mps;
+      st = (uint8_t)(((e >> 22) & 0x3F) | (nm << 7));
+    }
+    do {
+      s.A <<= 1;
+      s.C <<= 1;
+      s.t--;
+      if (s.t == 0) byte_out();
+    } while (s.A < 0x8000);
+  }
+};
+
+struct Blk {
+  uint64_t *q;       // 4 per stripe column (l = 3 is the stripe's top row)
+  uint16_t *st;      // state per stripe column
+  uint64_t cw, ns, nslice, ch;
+  __device__ __forceinline__ uint32_t sig(int64_t k) const { return k < 0 ? 0u : (st[k] & 0xF); }
+  __device__ __forceinline__ uint32_t xi(int64_t k) const { return k < 0 ? 0u : ((st[k] >> 12) & 0xF); }
+  __device__ __forceinline__ uint32_t nib(int64_t k, int b) const {
+    const uint64_t *p = q + 4 * k;
+    return (uint32_t)(((p[0] >> b) & 1) | (((p[1] >> b) & 1) << 1) | (((p[2] >> b) & 1) << 2) |
+                      (((p[3] >> b) & 1) << 3));
+  }
+};
+
+// neighbour links of stripe column k (negative = insignificant dummy)
+struct Nb {
+  int64_t l, r, u, d, ul, ur, dl, dr;
+};
+__device__ __forceinline__ Nb links(const Blk &B, int64_t k) {
+  const uint64_t x = (uint64_t)k % B.cw, j = ((uint64_t)k / B.cw) % B.ns;
+  Nb n;
+  n.l = x > 0 ? k - 1 : -1;
+  n.r = x + 1 < B.cw ? k + 1 : -1;
+  n.u = j > 0 ? k - (int64_t)B.cw : -1;
+  n.d = j + 1 < B.ns ? k + (int64_t)B.cw : -1;
+  n.ul = n.u >= 0 && x > 0 ? n.u - 1 : -1;
+  n.ur = n.u >= 0 && x + 1 < B.cw ? n.u + 1 : -1;
+  n.dl = n.d >= 0 && x > 0 ? n.d - 1 : -1;
+  n.dr = n.d >= 0 && x + 1 < B.cw ? n.d + 1 : -1;
+  return n;
+}
+
+__device__ __forceinline__ uint32_t sig_index(const Blk &B, int64_t k, const Nb &n, uint32_t bm, int l, bool top,
+                                              bool bot) {
+  const uint32_t s = B.sig(k), L = B.sig(n.l), R = B.sig(n.r);
+  uint32_t kh = (L & bm) + (R & bm);
+  uint32_t kv = (bm & (s >> 1)) + (bm & (s << 1));
+  uint32_t kd = (bm & (L >> 1)) + (bm & (L << 1)) + (bm & (R >> 1)) + (bm & (R << 1));
+  if (top) {
+    kv += (0x01 & B.sig(n.u)) << 3;
+    kd += ((0x01 & B.sig(n.ul)) + (0x01 & B.sig(n.ur))) << 3;
+  } else if (bot) {
+    kv += (0x08 & B.sig(n.d)) >> 3;
+    kd += ((0x08 & B.sig(n.dl)) + (0x08 & B.sig(n.dr))) >> 3;
+  }
+  return (15 * kh + 5 * kv + kd) >> l;
+}
+
+__device__ __forceinline__ void code_sign(Coder &C, const Blk &B, int64_t k, const Nb &n, uint32_t bm, int l, bool top,
+                                          bool bot) {
+  const uint32_t s = B.sig(k), x = B.xi(k);
+  const uint32_t L = B.sig(n.l), R = B.sig(n.r), xL = B.xi(n.l), xR = B.xi(n.r);
+  uint32_t xh = ((bm & L) | ((bm & xL) << 2) | ((bm & R) << 1) | ((bm & xR) << 3)) >> l;
+  uint32_t xv = ((bm & (s >> 1)) | ((bm & (x >> 1)) << 2) | ((bm & (s << 1)) << 1) | ((bm & (x << 1)) << 3)) >> l;
+  if (top) xv |= (0x01 & B.sig(n.u)) | ((0x01 & B.xi(n.u)) << 2);
+  else if (bot) xv |= ((0x08 & B.sig(n.d)) >> 2) | (0x08 & B.xi(n.d));
+  xh = c_sig2xi[xh];
+  xv = c_sig2xi[xv];
+  const int idx = (int)((xh << 2) | xv);
+  C.encode(((bm & x) >> l) ^ c_xi2cont[idx][1], c_xi2cont[idx][0]);
+}
+
+__device__ int64_t pass_spp(Coder &C, Blk &B, int b, const uint8_t *s2c) {
+  int64_t mse = 0;
+  const uint32_t rest = 4 - (uint32_t)(B.ch & 3);
+  const uint32_t smask = (uint32_t)(~((0x0F >> (B.ch & 3)) | 0xF0)) & 0xFF;
+  const uint32_t dsh = b < 5 ? 0 : b - 5, dcor = b < 5 ? 5 - b : 0;
+  const uint64_t nfull = B.ch >> 2;
+  int64_t k = 0;
+  for (uint64_t sl = 0; sl < B.nslice; ++sl) {
+    for (uint64_t rowblk = 0; rowblk < B.ns; ++rowblk) {
+      const bool full = rowblk < nfull;
+      for (uint64_t x = 0; x < B.cw; ++x, ++k) {
+        const uint32_t s0 = B.sig(k);
+        if (full ? (s0 ^ 0xF) : ((s0 ^ 0xF) & smask)) {
+          const Nb n = links(B, k);
+          const uint32_t bits = B.nib(k, b);
+          const int lmin = full ? 0 : (int)rest;
+          for (int l = 3; l >= lmin; --l) {
+            const uint32_t bm = 1u << l;
+            if (!(B.sig(k) & bm)) {
+              const uint32_t ks = sig_index(B, k, n, bm, l, l == 3, full && l == 0);
+              if (ks) {
+                C.encode((bits >> l) & 1, s2c[ks]);
+                if (bits & bm) {
+                  code_sign(C, B, k, n, bm, l, l == 3, full && l == 0);
+                  mse += 49152 + 6144 * (int32_t)(((B.q[4 * k + l] >> dsh) << dcor) & 0x1F);
+                  B.st[k] |= (uint16_t)bm;
+                }
+                B.st[k] |= (uint16_t)(bm << 4);
+              }
+            } else {
+              B.st[k] &= (uint16_t)~(bm << 4);
+            }
+          }
+        } else {
+          B.st[k] &= (uint16_t)~0xF0;
+        }
+      }
+    }
+  }
+  return mse;
+}
+
+__device__ int64_t pass_mrp(Coder &C, Blk &B, int b) {
+  int64_t mse = 0;
+  const uint32_t rest = 4 - (uint32_t)(B.ch & 3);
+  const uint32_t smask = (uint32_t)(~((0x0F >> (B.ch & 3)) | 0xF0)) & 0xFF;
+  const uint32_t dsh = b < 6 ? 0 : b - 6, dcor = b < 6 ? 6 - b : 0;
+  const uint64_t nfull = B.ch >> 2;
+  int64_t k = 0;
+  for (uint64_t sl = 0; sl < B.nslice; ++sl) {
+    for (uint64_t rowblk = 0; rowblk < B.ns; ++rowblk) {
+      const bool full = rowblk < nfull;
+      for (uint64_t x = 0; x < B.cw; ++x, ++k) {
+        const uint16_t stv = B.st[k];
+        uint32_t sg = (stv & 0xF) & ~((stv >> 4) & 0xF);
+        if (!full) sg &= smask;
+        if (!sg) continue;
+        const Nb n = links(B, k);
+        const uint32_t bits = B.nib(k, b);
+        const uint32_t s = stv & 0xF, L = B.sig(n.l), R = B.sig(n.r);
+        const int lmin = full ? 0 : (int)rest;
+        for (int l = 3; l >= lmin; --l) {
+          const uint32_t bm = 1u << l;
+          if (!(sg & bm)) continue;
+          int km = 15;
+          if ((B.st[k] >> 8) & bm) {
+            km += 2;
+          } else if ((bm & (s >> 1)) || (bm & (s << 1)) || (L & bm) || (R & bm) || (bm & (L >> 1)) ||
+                     (bm & (L << 1)) || (bm & (R >> 1)) || (bm & (R << 1))) {
+            km += 1;
+          } else if (l == 3 && ((0x01 & B.sig(n.u)) || (0x01 & B.sig(n.ul)) || (0x01 & B.sig(n.ur)))) {
+            km += 1;
+          } else if (full && l == 0 && ((0x08 & B.sig(n.d)) || (0x08 & B.sig(n.dl)) || (0x08 & B.sig(n.dr)))) {
+            km += 1;
+          }
+          C.encode((bits >> l) & 1, km);
+          mse += c_tm[((B.q[4 * k + l] >> dsh) << dcor) & 0x3F];
+          B.st[k] |= (uint16_t)(bm << 8);
+        }
+      }
+    }
+  }
+  return mse;
+}
+
+__device__ int64_t pass_cup(Coder &C, Blk &B, int b, const uint8_t *s2c) {
+  int64_t mse = 0;
+  const uint32_t rest = 4 - (uint32_t)(B.ch & 3);
+  const uint32_t dsh = b < 5 ? 0 : b - 5, dcor = b < 5 ? 5 - b : 0;
+  const uint64_t nfull = B.ch >> 2;
+  int64_t k = 0;
+  for (uint64_t sl = 0; sl < B.nslice; ++sl) {
+    for (uint64_t rowblk = 0; rowblk < B.ns; ++rowblk) {
+      const bool full = rowblk < nfull;
+      for (uint64_t x = 0; x < B.cw; ++x, ++k) {
+        const Nb n = links(B, k);
+        const uint32_t bits = B.nib(k, b);
+        int r = -1;
+        if (full) {
+          if (!(0x08 & B.sig(n.dl)) && !(0x08 & B.sig(n.d)) && !(0x08 & B.sig(n.dr)) && !(0x01 & B.sig(n.u)) &&
+              !(0x01 & B.sig(n.ul)) && !B.sig(n.l) && !(0x01 & B.sig(n.ur)) && !B.sig(n.r) && !B.sig(k)) {
+            if (bits) {
+              uint32_t sm = bits | (bits >> 1);
+              sm |= sm >> 2;
+              r = sm == 1 ? 3 : (sm == 3 ? 2 : (sm == 7 ? 1 : 0));
+              C.encode(1, 9);
+              C.encode(((uint32_t)r >> 1) & 1, 18);
+              C.encode((uint32_t)r & 1, 18);
+            } else {
+              r = 4;
+              C.encode(0, 9);
+            }
+          }
+        }
+        const int lmin = full ? 0 : (int)rest;
+        for (int l = 3; l >= lmin; --l) {
+          const uint32_t bm = 1u << l;
+          const uint16_t stv = B.st[k];
+          if (!(stv & bm) && !((stv >> 4) & bm)) {
+            if (r >= 0) --r;
+            else C.encode((bits >> l) & 1, s2c[sig_index(B, k, n, bm, l, l == 3, full && l == 0)]);
+            if (bits & bm) {
+              code_sign(C, B, k, n, bm, l, l == 3, full && l == 0);
+              mse += 49152 + 6144 * (int32_t)(((B.q[4 * k + l] >> dsh) << dcor) & 0x1F);
+              B.st[k] |= (uint16_t)bm;
+            }
+          }
+        }
+      }
+    }
+  }
+  return mse;
+}
+
+// minimal truncation length of one pass snapshot (mq.c:586-675, FP64 build)
+__device__ __forceinline__ uint32_t byte_at(const uint8_t *buf, uint64_t cap, int64_t i) {
+  return (i >= 0 && (uint64_t)i < cap) ? buf[i] : 0u;
+}
+
+__device__ int64_t trunc_len(MqState s, const uint8_t *buf, uint64_t cap) {
+  int64_t Cr = ((int64_t)s.T << 27) + ((int64_t)s.C << s.t);
+  int64_t CAr = ((int64_t)s.A << s.t) + Cr;
+  int sh = 8;
+  // bounded: each round consumes one byte; the terminated codeword ends
+  // within the buffer, so the loop exits long before 2^20 rounds
+  for (int guard = 0; guard < (1 << 20) && (((Cr >> 27) > 0xff) || ((CAr >> 27) <= 0xff)); ++guard) {
+    const int64_t v = byte_at(buf, cap, s.L);
+    s.L++;
+    Cr -= v << (35 - sh);
+    CAr -= v << (35 - sh);
+    Cr <<= sh;
+    CAr <<= sh;
+    sh = (v == 0xFF) ? 7 : 8;
+  }
+  if (s.L > 0 && byte_at(buf, cap, s.L - 1) == 0xFF) s.L--;
+  while (s.L > 1 && byte_at(buf, cap, s.L - 2) == 0xFF && byte_at(buf, cap, s.L - 1) == 0x7F) s.L -= 2;
+  return s.L;
+}
+
+__device__ uint16_t slope2log(double lambda) {
+  const uint64_t raw = (uint64_t)__double_as_longlong(lambda);
+  const uint64_t ex = ((raw & 0x7FF0000000000000ull) >> 44) - 0x41F00;
+  const uint64_t ma = (raw & 0x000FFFFFFFFFFFFFull) >> 44;
+  const uint64_t v = (ex + c_frac2log[ma]) + 0x10000;
+  if (v > 0xFFFF) return 0xFFFF;
+  if (v < 2) return 2;
+  return (uint16_t)v;
+}
+
+__global__ void __launch_bounds__(64) t1_encode_kernel(
+    uint64_t ncb, const uint32_t *__restrict__ cbbox, const double *__restrict__ cbsb,
+    const uint32_t *__restrict__ cbmeta, const uint64_t *__restrict__ qoff, const uint64_t *__restrict__ doff,
+    uint64_t *__restrict__ q, uint16_t *__restrict__ state, uint8_t *__restrict__ data, uint8_t *__restrict__ Kp,
+    uint8_t *__restrict__ Kmsbsp, uint8_t *__restrict__ Zp, uint32_t *__restrict__ Lp, uint16_t *__restrict__ Sp,
+    uint64_t *__restrict__ snap, double *__restrict__ msep, uint32_t *__restrict__ flags, int erres) {
+  __shared__ uint8_t ctx[19 * 64];
+  const uint64_t cb = (uint64_t)blockIdx.x * 64 + threadIdx.x;
+  if (cb >= ncb) return;
+  const int lane = threadIdx.x;
+  const uint32_t *bx = cbbox + 8 * cb;
+  const uint64_t cw = bx[4] - bx[0], ch = bx[5] - bx[1], cd = bx[6] - bx[2], ct = bx[7] - bx[3];
+  const int K = Kp[cb];
+  uint16_t *S = Sp + cb * 191;
+  uint32_t *L = Lp + cb * 190;
+  if (K == 0) {
+    S[0] = 0xFFFF;
+    Zp[cb] = 0;
+    Kmsbsp[cb] = 0;
+    return;
+  }
+  const uint32_t meta = cbmeta[cb];
+  const uint32_t hb = meta & 0xFF, Kmax = meta >> 8;
+  const int sidx = (hb == 1 || hb == 5 || hb == 9 || hb == 13) ? 1 : ((hb & 3) == 3 ? 2 : 0);
+  const uint8_t *s2c = c_sig[sidx];
+  Blk B;
+  B.q = q + qoff[cb];
+  B.st = state + qoff[cb] / 4;
+  B.cw = cw;
+  B.ns = (ch + 3) / 4;
+  B.nslice = cd * ct;
+  B.ch = ch;
+  Coder C;
+  C.buf = data + doff[cb];
+  C.cap = doff[cb + 1] - doff[cb];
+  C.ovf = 0;
+  C.ctx = ctx;
+  C.lane = lane;
+  for (int i = 0; i < 19; ++i) C.cx(i) = 0;
+  C.cx(0) = 4;
+  C.cx(9) = 3;
+  C.cx(18) = 46;
+  C.s.A = 0x8000;
+  C.s.t = 12;
+  C.s.L = -1;
+  C.s.C = 0;
+  C.s.T = 0;
+  uint64_t *sn = snap + cb * 2 * 190;
+  double *mse = msep + cb * 190;
+  double scale = cbsb[3 * cb + 2];
+  scale *= (cbsb[3 * cb + 1] * cbsb[3 * cb + 1]);
+  scale *= ldexp(1.0, 2 * K);
+  scale /= 0x10000;
+  int j = 0, i = 2;
+  for (int k = K; k-- > 0; i = 0) {
+    for (; i < 3; ++i, ++j) {
+      // pass j starts from the live state; its end state is snapshotted below
+      const int64_t dm = (i == 0) ? pass_spp(C, B, k, s2c) : (i == 1 ? pass_mrp(C, B, k) : pass_cup(C, B, k, s2c));
+      mse[j] = (double)dm * scale;
+      if (!(i == 2 && erres)) {
+        sn[2 * j] = (uint64_t)C.s.L;
+        sn[2 * j + 1] = (uint64_t)C.s.C | ((uint64_t)C.s.A << 32) | ((uint64_t)(uint8_t)C.s.t << 48) |
+                        ((uint64_t)C.s.T << 56);
+      }
+    }
+    if (erres) {
+      C.encode(1, 18);
+      C.encode(0, 18);
+      C.encode(1, 18);
+      C.encode(0, 18);
+      sn[2 * (j - 1)] = (uint64_t)C.s.L;
+      sn[2 * (j - 1) + 1] = (uint64_t)C.s.C | ((uint64_t)C.s.A << 32) | ((uint64_t)(uint8_t)C.s.t << 48) |
+                            ((uint64_t)C.s.T << 56);
+    }
+    scale *= 0.25;
+  }
+  const int Z = j;
+  // termination of the last state (mq.c:706-741); the live state is restored
+  {
+    MqState keep = C.s;
+    int n = 12 - C.s.t;
+    C.s.C <<= C.s.t;
+    while (n > 0) {
+      C.byte_out();
+      n -= C.s.t;
+      C.s.C <<= C.s.t;
+    }
+    C.byte_out();
+    C.s = keep;
+  }
+  if (C.ovf) {
+    atomicOr(&flags[0], 1u);
+    Zp[cb] = 0;
+    return;
+  }
+  for (int z = 0; z < Z; ++z) {
+    MqState s;
+    s.L = (int64_t)sn[2 * z];
+    const uint64_t w = sn[2 * z + 1];
+    s.C = (uint32_t)w;
+    s.A = (uint16_t)(w >> 32);
+    s.t = (int8_t)(uint8_t)(w >> 48);
+    s.T = (uint8_t)(w >> 56);
+    L[z] = (uint32_t)trunc_len(s, C.buf, C.cap);
+  }
+  Kmsbsp[cb] = (uint8_t)(Kmax - K);
+  Zp[cb] = (uint8_t)Z;
+  // convex hull and log slopes (tier1.c:2161-2267); lambda kept in the mse
+  // scratch after its value has been consumed
+  uint8_t hull[192];
+  double lam[191];
+  for (int z = 0; z <= Z; ++z) lam[z] = 0;
+  int h = 0;
+  hull[0] = 0;
+  int hlast = 0;
+  lam[0] = 18446744073709551616.0;
+  for (int ii = 0; ii < Z; ++ii) {
+    double dD = 0;
+    for (int jj = hlast; jj <= ii; ++jj) dD += mse[jj];
+    uint64_t dL = hlast > 0 ? (uint64_t)(L[ii] - L[hlast - 1]) : (uint64_t)L[ii];
+    if (dD > 0) {
+      while (h > 0 && dD >= lam[hlast] * (double)dL) {
+        lam[hlast] = 0;
+        --h;
+        hlast = hull[h];
+        dD = 0;
+        for (int jj = hlast; jj <= ii; ++jj) dD += mse[jj];
+        dL = hlast > 0 ? (uint64_t)(L[ii] - L[hlast - 1]) : (uint64_t)L[ii];
+      }
+      ++h;
+      hlast = ii + 1;
+      hull[h] = (uint8_t)hlast;
+      if (dL == 0) lam[hlast] = dD < 0 ? -1.0 : (dD == 0 ? 0.0 : 18446744073709551616.0);
+      else lam[hlast] = dD / (double)dL;
+    } else {
+      lam[ii + 1] = 0;
+    }
+  }
+  uint32_t smin = 0xFFFF, smax = 0;
+  for (int z = 0; z <= Z; ++z) {
+    uint16_t v;
+    if (lam[z] < 0) v = 1;
+    else if (lam[z] == 0) v = (z == Z) ? 1 : 0;
+    else v = slope2log(lam[z]);
+    S[z] = v;
+    if (v) {
+      smin = v < smin ? v : smin;
+      smax = v > smax ? v : smax;
+    }
+  }
+  atomicMin(&flags[1], smin);
+  atomicMax(&flags[2], smax);
+}
+
+__global__ void init_flags_kernel(uint32_t *flags) {
+  if (threadIdx.x == 0) {
+    flags[0] = 0;
+    flags[1] = 0xFFFF;
+    flags[2] = 0;
+    flags[3] = 0;
+  }
+}
+
+// copy each codeblock's included prefix into one packed buffer
+__global__ void gather_kernel(const uint8_t *__restrict__ data, const uint64_t *__restrict__ doff,
+                              const uint64_t *__restrict__ g, uint64_t ncb, uint8_t *__restrict__ out) {
+  for (uint64_t cb = blockIdx.x; cb < ncb; cb += gridDim.x) {
+    const uint64_t len = g[2 * cb + 1], dst = g[2 * cb];
+    const uint8_t *src = data + doff[cb];
+    for (uint64_t i = threadIdx.x; i < len; i += blockDim.x) out[dst + i] = src[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static const uint16_t kQe[47] = {0x5601, 0x3401, 0x1801, 0x0AC1, 0x0521, 0x0221, 0x5601, 0x5401, 0x4801, 0x3801,
+                                 0x3001, 0x2401, 0x1C01, 0x1601, 0x5601, 0x5401, 0x5101, 0x4801, 0x3801, 0x3401,
+                                 0x3001, 0x2801, 0x2401, 0x2201, 0x1C01, 0x1801, 0x1601, 0x1401, 0x1201, 0x1101,
+                                 0x0AC1, 0x09C1, 0x08A1, 0x0521, 0x0441, 0x02A1, 0x0221, 0x0141, 0x0111, 0x0085,
+                                 0x0049, 0x0025, 0x0015, 0x0009, 0x0005, 0x0001, 0x5601};
+static const uint8_t kNmps[47] = {1,  2,  3,  4,  5,  38, 7,  8,  9,  10, 11, 12, 13, 29, 15, 16,
+                                  17, 18, 19, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29, 30, 31, 32,
+                                  33, 34, 35, 36, 37, 38, 39, 40, 41, 42, 43, 44, 45, 45, 46};
+static const uint8_t kNlps[47] = {1,  6,  9,  12, 29, 33, 6,  14, 14, 14, 17, 18, 20, 21, 14, 14,
+                                  15, 16, 17, 18, 19, 19, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29,
+                                  30, 31, 32, 33, 34, 35, 36, 37, 38, 39, 40, 41, 42, 43, 46};
+// significance context maps by 15*kh + 5*kv + kd for L(L/LH), H(HL) and HH
+// subbands, sign-context tables and the distortion / log tables
+// (tier1.c:139-275)
+static const uint8_t kSig[3][45] = {
+    {0, 1, 2, 2, 2, 3, 3, 3, 3, 3, 4, 4, 4, 4, 4, 5, 6, 6, 6, 6, 7, 7, 7,
+     7, 7, 7, 7, 7, 7, 7, 8, 8, 8, 8, 8, 8, 8, 8, 8, 8, 8, 8, 8, 8, 8},
+    {0, 1, 2, 2, 2, 5, 6, 6, 6, 6, 8, 8, 8, 8, 8, 3, 3, 3, 3, 3, 7, 7, 7,
+     7, 7, 8, 8, 8, 8, 8, 4, 4, 4, 4, 4, 7, 7, 7, 7, 7, 8, 8, 8, 8, 8},
+    {0, 3, 6, 8, 8, 1, 4, 7, 8, 8, 2, 5, 7, 8, 8, 1, 4, 7, 8, 8, 2, 5, 7,
+     8, 8, 2, 5, 7, 8, 8, 2, 5, 7, 8, 8, 2, 5, 7, 8, 8, 2, 5, 7, 8, 8}};
+static const uint8_t kSig2xi[16] = {1, 2, 2, 2, 1, 0, 2, 1, 1, 2, 0, 1, 1, 0, 0, 0};
+static const uint8_t kXi2cont[11][2] = {{14, 1}, {13, 1}, {12, 1}, {0, 0}, {11, 1}, {10, 0},
+                                        {11, 0}, {0, 0},  {12, 0}, {13, 0}, {14, 0}};
+static const int32_t kTm[64] = {49152,  47104,  45056,  43008,  40960,  38912,  36864,  34816,  32768,  30720,
+                                28672,  26624,  24576,  22528,  20480,  18432,  16384,  14336,  12288,  10240,
+                                8192,   6144,   4096,   2048,   0,      -2048,  -4096,  -6144,  -8192,  -10240,
+                                -12288, -14336, -16384, -14336, -12288, -10240, -8192,  -6144,  -4096,  -2048,
+                                0,      2048,   4096,   6144,   8192,   10240,  12288,  14336,  16384,  18432,
+                                20480,  22528,  24576,  26624,  28672,  30720,  32768,  34816,  36864,  38912,
+                                40960,  43008,  45056,  47104};
+static const uint16_t kFrac2log[256] = {
+    0,   1,   2,   4,   5,   7,   8,   9,   11,  12,  14,  15,  16,  18,  19,  21,  22,  23,  25,  26,  27,  29,
+    30,  31,  33,  34,  35,  37,  38,  39,  40,  42,  43,  44,  46,  47,  48,  49,  51,  52,  53,  54,  56,  57,
+    58,  59,  61,  62,  63,  64,  65,  67,  68,  69,  70,  71,  73,  74,  75,  76,  77,  78,  80,  81,  82,  83,
+    84,  85,  87,  88,  89,  90,  91,  92,  93,  94,  96,  97,  98,  99,  100, 101, 102, 103, 104, 105, 106, 108,
+    109, 110, 111, 112, 113, 114, 115, 116, 117, 118, 119, 120, 121, 122, 123, 124, 125, 126, 127, 128, 129, 131,
+    132, 133, 134, 135, 136, 137, 138, 139, 140, 140, 141, 142, 143, 144, 145, 146, 147, 148, 149, 150, 151, 152,
+    153, 154, 155, 156, 157, 158, 159, 160, 161, 162, 162, 163, 164, 165, 166, 167, 168, 169, 170, 171, 172, 173,
+    173, 174, 175, 176, 177, 178, 179, 180, 181, 181, 182, 183, 184, 185, 186, 187, 188, 188, 189, 190, 191, 192,
+    193, 194, 194, 195, 196, 197, 198, 199, 200, 200, 201, 202, 203, 204, 205, 205, 206, 207, 208, 209, 209, 210,
+    211, 212, 213, 214, 214, 215, 216, 217, 218, 218, 219, 220, 221, 222, 222, 223, 224, 225, 225, 226, 227, 228,
+    229, 229, 230, 231, 232, 232, 233, 234, 235, 235, 236, 237, 238, 239, 239, 240, 241, 242, 242, 243, 244, 245,
+    245, 246, 247, 247, 248, 249, 250, 250, 251, 252, 253, 253, 254, 255};
+static const uint8_t kSwitch0 = 0, kSwitch6 = 6, kSwitch14 = 14;
+
+static int upload_tables() {
+  uint32_t mq[47];
+  for (int i = 0; i < 47; ++i) {
+    const uint32_t sw = (i == kSwitch0 || i == kSwitch6 || i == kSwitch14) ? 1u : 0u;
+    mq[i] = kQe[i] | ((uint32_t)kNmps[i] << 16) | ((uint32_t)kNlps[i] << 22) | (sw << 28);
+  }
+  HIP_OK(hipMemcpyToSymbol(HIP_SYMBOL(c_mq), mq, sizeof mq));
+  HIP_OK(hipMemcpyToSymbol(HIP_SYMBOL(c_tm), kTm, sizeof kTm));
+  HIP_OK(hipMemcpyToSymbol(HIP_SYMBOL(c_sig), kSig, sizeof kSig));
+  HIP_OK(hipMemcpyToSymbol(HIP_SYMBOL(c_sig2xi), kSig2xi, sizeof kSig2xi));
+  HIP_OK(hipMemcpyToSymbol(HIP_SYMBOL(c_xi2cont), kXi2cont, sizeof kXi2cont));
+  HIP_OK(hipMemcpyToSymbol(HIP_SYMBOL(c_frac2log), kFrac2log, sizeof kFrac2log));
+  return 0;
+}
+
+Workspace *workspace_of(Job *job);
+
+int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double *work, CblkRecords &rec,
+              uint16_t *slope_min, uint16_t *slope_max) {
+  const uint64_t ncb = par.ncb;
+  rec.resize(ncb, c->control.nLayers);
+  if (!ncb) return 0;
+  // per-codeblock tables
+  std::vector<double> sb(3 * ncb);
+  std::vector<uint32_t> meta(ncb);
+  std::vector<uint64_t> qoff(ncb + 1), doff(ncb + 1);
+  uint64_t qo = 0, dof = 0;
+  for (uint64_t i = 0; i < ncb; ++i) {
+    const uint32_t *b = &par.cb_box[8 * i];
+    const Subband &s = par.res[par.cb_sbref[i] >> 8].sb[par.cb_sbref[i] & 0xFF];
+    sb[3 * i] = s.step_eff;
+    sb[3 * i + 1] = s.step;
+    sb[3 * i + 2] = s.gain;
+    meta[i] = (uint32_t)s.hb | ((uint32_t)s.Kmax << 8);
+    const uint64_t cw = b[4] - b[0], ch = b[5] - b[1], cd = b[6] - b[2], ct = b[7] - b[3];
+    const uint64_t ncol = cw * ((ch + 3) / 4) * cd * ct;
+    qoff[i] = qo;
+    qo += 4 * ncol;
+    doff[i] = dof;
+    dof += (cw * ch * cd * ct) * 10 + 256;
+  }
+  qoff[ncb] = qo;
+  doff[ncb] = dof;
+  size_t cap;
+  if (ncb > ws->cap_cb) {
+    void **tabs[] = {(void **)&ws->d_cbbox, (void **)&ws->d_cbsb, (void **)&ws->d_cbmeta, (void **)&ws->d_qoff,
+                     (void **)&ws->d_doff,  (void **)&ws->d_K,    (void **)&ws->d_Kmsbs,  (void **)&ws->d_Z,
+                     (void **)&ws->d_L,     (void **)&ws->d_S,    (void **)&ws->d_snap,   (void **)&ws->d_mse,
+                     (void **)&ws->d_gather};
+    for (void **p : tabs)
+      if (*p) {
+        HIP_OK(hipFree(*p));
+        *p = nullptr;
+      }
+    const uint64_t n = ncb + 1;
+    HIP_OK(hipMalloc((void **)&ws->d_cbbox, 8 * n * sizeof(uint32_t)));
+    HIP_OK(hipMalloc((void **)&ws->d_cbsb, 3 * n * sizeof(double)));
+    HIP_OK(hipMalloc((void **)&ws->d_cbmeta, n * sizeof(uint32_t)));
+    HIP_OK(hipMalloc((void **)&ws->d_qoff, n * sizeof(uint64_t)));
+    HIP_OK(hipMalloc((void **)&ws->d_doff, n * sizeof(uint64_t)));
+    HIP_OK(hipMalloc((void **)&ws->d_K, n));
+    HIP_OK(hipMalloc((void **)&ws->d_Kmsbs, n));
+    HIP_OK(hipMalloc((void **)&ws->d_Z, n));
+    HIP_OK(hipMalloc((void **)&ws->d_L, n * kMaxPasses * sizeof(uint32_t)));
+    HIP_OK(hipMalloc((void **)&ws->d_S, n * (kMaxPasses + 1) * sizeof(uint16_t)));
+    HIP_OK(hipMalloc((void **)&ws->d_snap, n * 2 * kMaxPasses * sizeof(uint64_t)));
+    HIP_OK(hipMalloc((void **)&ws->d_mse, n * kMaxPasses * sizeof(double)));
+    HIP_OK(hipMalloc((void **)&ws->d_gather, 2 * n * sizeof(uint64_t)));
+    ws->cap_cb = ncb;
+  }
+  cap = ws->cap_q;
+  CHK(ensure((void **)&ws->d_q, &cap, qo * sizeof(uint64_t)));
+  ws->cap_q = cap;
+  {
+    static size_t state_cap = 0;
+    (void)state_cap;
+  }
+  size_t scap = 0;
+  if (ws->d_state) {
+    HIP_OK(hipFree(ws->d_state));
+    ws->d_state = nullptr;
+  }
+  CHK(ensure((void **)&ws->d_state, &scap, (qo / 4 + 1) * sizeof(uint16_t)));
+  cap = ws->cap_data;
+  CHK(ensure((void **)&ws->d_data, &cap, dof));
+  ws->cap_data = cap;
+  if (!ws->d_flags) HIP_OK(hipMalloc((void **)&ws->d_flags, 16));
+  HIP_OK(hipMemcpyAsync(ws->d_cbbox, par.cb_box.data(), 8 * ncb * sizeof(uint32_t), hipMemcpyHostToDevice,
+                        ws->stream));
+  HIP_OK(hipMemcpyAsync(ws->d_cbsb, sb.data(), 3 * ncb * sizeof(double), hipMemcpyHostToDevice, ws->stream));
+  HIP_OK(hipMemcpyAsync(ws->d_cbmeta, meta.data(), ncb * sizeof(uint32_t), hipMemcpyHostToDevice, ws->stream));
+  HIP_OK(hipMemcpyAsync(ws->d_qoff, qoff.data(), (ncb + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, ws->stream));
+  HIP_OK(hipMemcpyAsync(ws->d_doff, doff.data(), (ncb + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, ws->stream));
+  hipLaunchKernelGGL(init_flags_kernel, dim3(1), dim3(64), 0, ws->stream, ws->d_flags);
+  const uint64_t W = par.box.len(0), H = par.box.len(1), D = par.box.len(2);
+  hipLaunchKernelGGL(quantize_kernel, dim3((uint32_t)std::min<uint64_t>(ncb, 65535)), dim3(256), 0, ws->stream, work,
+                     W, H, D, ws->d_cbbox, ws->d_cbsb, ws->d_qoff, ws->d_q, ws->d_state, ws->d_K, ncb);
+  HIP_OK(hipGetLastError());
+  hipLaunchKernelGGL(t1_encode_kernel, dim3((uint32_t)((ncb + 63) / 64)), dim3(64), 0, ws->stream, ncb, ws->d_cbbox,
+                     ws->d_cbsb, ws->d_cbmeta, ws->d_qoff, ws->d_doff, ws->d_q, ws->d_state, ws->d_data, ws->d_K,
+                     ws->d_Kmsbs, ws->d_Z, ws->d_L, ws->d_S, ws->d_snap, ws->d_mse, ws->d_flags,
+                     (int)c->control.error_resilience);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(rec.K.data(), ws->d_K, ncb, hipMemcpyDeviceToHost, ws->stream));
+  HIP_OK(hipMemcpyAsync(rec.Kmsbs.data(), ws->d_Kmsbs, ncb, hipMemcpyDeviceToHost, ws->stream));
+  HIP_OK(hipMemcpyAsync(rec.Z.data(), ws->d_Z, ncb, hipMemcpyDeviceToHost, ws->stream));
+  HIP_OK(hipMemcpyAsync(rec.L.data(), ws->d_L, ncb * kMaxPasses * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                        ws->stream));
+  HIP_OK(hipMemcpyAsync(rec.S.data(), ws->d_S, ncb * (kMaxPasses + 1) * sizeof(uint16_t), hipMemcpyDeviceToHost,
+                        ws->stream));
+  uint32_t hfl[4];
+  HIP_OK(hipMemcpyAsync(hfl, ws->d_flags, sizeof hfl, hipMemcpyDeviceToHost, ws->stream));
+  HIP_OK(hipStreamSynchronize(ws->stream));
+  if (hfl[0]) {
+    std::snprintf(g_err, sizeof g_err, "tier-1 output exceeded the per-codeblock buffer");
+    return 1;
+  }
+  if (hfl[1] < *slope_min) *slope_min = (uint16_t)hfl[1];
+  if (hfl[2] > *slope_max) *slope_max = (uint16_t)hfl[2];
+  return 0;
+}
+
+int gather(Workspace *ws, CblkRecords &rec, const std::vector<uint32_t> &len) {
+  const uint64_t ncb = rec.n;
+  std::vector<uint64_t> g(2 * ncb);
+  uint64_t tot = 0;
+  for (uint64_t i = 0; i < ncb; ++i) {
+    rec.data_off[i] = tot;
+    g[2 * i] = tot;
+    g[2 * i + 1] = len[i];
+    tot += len[i];
+  }
+  rec.data.assign(tot ? tot : 1, 0);
+  if (!ncb || !tot) return 0;
+  size_t cap = ws->cap_packed;
+  CHK(ensure((void **)&ws->d_packed, &cap, tot));
+  ws->cap_packed = cap;
+  HIP_OK(hipMemcpyAsync(ws->d_gather, g.data(), 2 * ncb * sizeof(uint64_t), hipMemcpyHostToDevice, ws->stream));
+  hipLaunchKernelGGL(gather_kernel, dim3((uint32_t)std::min<uint64_t>(ncb, 65535)), dim3(256), 0, ws->stream,
+                     ws->d_data, ws->d_doff, ws->d_gather, ncb, ws->d_packed);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(rec.data.data(), ws->d_packed, tot, hipMemcpyDeviceToHost, ws->stream));
+  HIP_OK(hipStreamSynchronize(ws->stream));
+  return 0;
+}
+
+int sync(Workspace *ws) {
+  HIP_OK(hipStreamSynchronize(ws->stream));
+  return 0;
+}
+
+int init_device(Workspace *ws) {
+  if (ws->device >= 0) return 0;
+  int dev = 0, n = 0;
+  HIP_OK(hipGetDeviceCount(&n));
+  if (n <= 0) {
+    std::snprintf(g_err, sizeof g_err, "no HIP device");
+    return 1;
+  }
+  HIP_OK(hipGetDevice(&dev));
+  ws->device = dev;
+  HIP_OK(hipStreamCreateWithFlags(&ws->stream, hipStreamNonBlocking));
+  for (auto &e : ws->ev) HIP_OK(hipEventCreate(&e));
+  static bool tables = false;
+  if (!tables) {
+    if (upload_tables()) return 1;
+    const int lds_max = 160 * 1024;
+    HIP_OK(hipFuncSetAttribute((const void *)dwt_axis_kernel<0, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
+    HIP_OK(hipFuncSetAttribute((const void *)dwt_axis_kernel<0, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
+    HIP_OK(hipFuncSetAttribute((const void *)dwt_axis_kernel<1, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
+    HIP_OK(hipFuncSetAttribute((const void *)dwt_axis_kernel<1, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
+    tables = true;
+  }
+  return 0;
+}
+
+static void ws_delete(void *p) { delete static_cast<Workspace *>(p); }
+
+Workspace *workspace_of(Job *job) {
+  if (!job->ws) {
+    auto *ws = new Workspace();
+    if (init_device(ws)) {
+      delete ws;
+      return nullptr;
+    }
+    job->ws = ws;
+    job->ws_free = ws_delete;
+  }
+  return static_cast<Workspace *>(job->ws);
+}
+
+int ensure_work(Workspace *ws, uint64_t n) {
+  size_t cap = ws->work_cap;
+  if (ensure((void **)&ws->d_work, &cap, n * sizeof(double))) return 1;
+  ws->work_cap = cap;
+  return 0;
+}
+
+int record(Workspace *ws, int i) {
+  HIP_OK(hipEventRecord(ws->ev[i], ws->stream));
+  return 0;
+}
+
+double elapsed(Workspace *ws, int a, int b) {
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, ws->ev[a], ws->ev[b]) != hipSuccess) return -1;
+  return ms;
+}
+
+int read_norm(Workspace *ws, double mm[4]) {
+  if (!ws->h_pinned) HIP_OK(hipHostMalloc((void **)&ws->h_pinned, 64, hipHostMallocDefault));
+  HIP_OK(hipMemcpyAsync(ws->h_pinned, ws->d_norm, 4 * sizeof(double), hipMemcpyDeviceToHost, ws->stream));
+  HIP_OK(hipStreamSynchronize(ws->stream));
+  std::memcpy(mm, ws->h_pinned, 4 * sizeof(double));
+  return 0;
+}
+
+int is_device_pointer(const void *p) {
+  hipPointerAttribute_t attr;
+  if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return attr.type == hipMemoryTypeDevice ? 1 : 0;
+}
+
+int copy_h2d(Workspace *ws, void *dst, const void *src, size_t n) {
+  HIP_OK(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, ws->stream));
+  HIP_OK(hipStreamSynchronize(ws->stream));
+  return 0;
+}
+
+int copy_d2h(Workspace *ws, void *dst, const void *src, size_t n) {
+  HIP_OK(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, ws->stream));
+  HIP_OK(hipStreamSynchronize(ws->stream));
+  return 0;
+}
+
+int device_count() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+}  // namespace gpu
+}  // namespace bwc

@@ -1,0 +1,74 @@
+// Host-side interface of the gfx950 kernels (gpu.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "codec.hpp"
+
+namespace bwc {
+namespace gpu {
+
+// Device buffers owned by one codec; grown on demand, freed with the codec.
+struct Workspace {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[8] = {};
+  // field staging (host input copied here) and the tile working buffer (f64)
+  void *d_input = nullptr;
+  size_t input_cap = 0;
+  double *d_work = nullptr;
+  size_t work_cap = 0;
+  // reductions
+  double *d_red = nullptr;        // partial min/max
+  double *d_norm = nullptr;       // {min, max, alpha, beta}
+  // codeblock tables + tier-1 outputs for one (tile, param)
+  uint32_t *d_cbbox = nullptr;    // 8 per codeblock
+  double *d_cbsb = nullptr;       // step_eff, step, gain per codeblock
+  uint32_t *d_cbmeta = nullptr;   // hb | Kmax << 8 per codeblock
+  uint64_t *d_qoff = nullptr;     // offset of each codeblock's stripe columns
+  uint64_t *d_doff = nullptr;     // offset of each codeblock's byte buffer
+  uint64_t *d_q = nullptr;        // quantised magnitudes, 4 per stripe column
+  uint16_t *d_state = nullptr;    // sigma | pi << 4 | delta << 8 | xi << 12
+  uint8_t *d_data = nullptr;      // MQ bytes
+  uint8_t *d_K = nullptr, *d_Kmsbs = nullptr, *d_Z = nullptr;
+  uint32_t *d_L = nullptr;
+  uint16_t *d_S = nullptr;
+  uint64_t *d_snap = nullptr;     // 2 words per pass per codeblock
+  double *d_mse = nullptr;
+  uint32_t *d_flags = nullptr;    // [0] overflow, [1] slope_min, [2] slope_max
+  uint8_t *d_packed = nullptr;
+  uint64_t *d_gather = nullptr;
+  size_t cap_cb = 0, cap_q = 0, cap_data = 0, cap_packed = 0, cap_gather = 0;
+  uint64_t *h_pinned = nullptr;   // small pinned staging for flags / min-max
+  ~Workspace();
+};
+
+Workspace *workspace_of(Job *job);
+
+// Stage entry points (all asynchronous on ws->stream unless stated).
+int fill_minmax(Workspace *ws, const bwc_codec *c, const Tile &tl, int param, const void *input, bool input_on_device,
+                double *work);
+int normalize(Workspace *ws, double *work, uint64_t n, uint8_t Qm);
+int dwt_forward(Workspace *ws, const bwc_codec *c, const Param &par, double *work);
+int dwt_inverse(Workspace *ws, const bwc_codec *c, const Param &par, double *work);
+// quantise + tier-1 for all codeblocks of (tile, param); results land in rec
+int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double *work, CblkRecords &rec,
+              uint16_t *slope_min, uint16_t *slope_max);
+// gather each codeblock's prefix [0, len[i]) into rec.data / rec.data_off
+int gather(Workspace *ws, CblkRecords &rec, const std::vector<uint32_t> &len);
+int sync(Workspace *ws);
+const char *last_error();
+int ensure_work(Workspace *ws, uint64_t n);
+int record(Workspace *ws, int i);
+double elapsed(Workspace *ws, int a, int b);
+int read_norm(Workspace *ws, double mm[4]);
+int is_device_pointer(const void *p);
+int copy_h2d(Workspace *ws, void *dst, const void *src, size_t n);
+int copy_d2h(Workspace *ws, void *dst, const void *src, size_t n);
+int device_count();
+
+}  // namespace gpu
+}  // namespace bwc

@@ -1,0 +1,463 @@
+// Tier-2 on the host: PCRD rate control over the slope threshold, packet
+// headers (tag trees, Δz codes, β/Lblock lengths with 0xFF bit stuffing) and
+// the codestream writer.
+//
+// Reference: tier2.c:100-153 (Δz), :240-312 (lengths), :433-711 (packets with
+// estimate modes 2 = probe, 1 = final estimate, 0 = real header), :750-849,
+// :895-1254 (quality layers and the bisection); tagtree.c:310-373;
+// bitstream.c:277-363, :561-612; codestream.c:139-570, :1329-1413.
+#include <cmath>
+#include <cstring>
+
+#include "codec.hpp"
+
+namespace bwc {
+
+namespace {
+
+// MSB-first bit writer with JPEG 2000 style stuffing: after an emitted 0xFF
+// byte the next byte carries only 7 bits.
+struct BitWriter {
+  std::vector<uint8_t> out;
+  uint8_t T = 0;
+  int t = 8;
+  void bit(uint64_t b) {
+    --t;
+    if (b) T |= (uint8_t)(1u << t);
+    if (!t) {
+      out.push_back(T);
+      t = (T == 0xFF) ? 7 : 8;
+      T = 0;
+    }
+  }
+  void flush() {
+    if (t != 8) out.push_back(T);
+  }
+};
+
+// Counting writer for the estimation modes: same bit-stuffing arithmetic,
+// no buffer.
+struct BitCounter {
+  uint64_t n = 0;
+  uint8_t T = 0;
+  int t = 8;
+  void bit(uint64_t b) {
+    --t;
+    if (b) T |= (uint8_t)(1u << t);
+    if (!t) {
+      ++n;
+      t = (T == 0xFF) ? 7 : 8;
+      T = 0;
+    }
+  }
+  void flush() {
+    if (t != 8) ++n;
+  }
+};
+
+template <class W>
+void tagtree_encode(TagTree &tt, W &w, uint32_t threshold, uint64_t leaf, int est) {
+  int64_t branch[64];
+  int nb = 0;
+  int64_t n = (int64_t)leaf;
+  while (tt.parent[n] >= 0) {
+    branch[nb++] = n;
+    n = tt.parent[n];
+  }
+  uint32_t tmin = 0;
+  for (;;) {
+    uint32_t th = tt.threshold[n];
+    if (th < tmin) th = tmin;
+    const uint16_t v = tt.value[n];
+    while (v >= th && th < threshold) {
+      ++th;
+      w.bit(v >= th ? 0 : 1);
+    }
+    tmin = std::min<uint32_t>(v, th);
+    if (!est) tt.threshold[n] = (uint16_t)th;
+    if (n == (int64_t)leaf) break;
+    n = branch[--nb];
+  }
+}
+
+template <class W>
+void delta_z_code(W &w, int dz) {
+  if (dz == 1) {
+    w.bit(0);
+  } else if (dz == 2) {
+    w.bit(1), w.bit(0);
+  } else if (dz < 6) {
+    dz -= 3;
+    w.bit(1), w.bit(1), w.bit(dz & 2), w.bit(dz & 1);
+  } else if (dz < 37) {
+    dz -= 6;
+    for (int i = 0; i < 4; ++i) w.bit(1);
+    for (int m = 0x10; m; m >>= 1) w.bit(dz & m);
+  } else if (dz < 293) {
+    dz -= 37;
+    for (int i = 0; i < 9; ++i) w.bit(1);
+    for (int m = 0x80; m; m >>= 1) w.bit(dz & m);
+  }
+}
+
+struct T2 {
+  bwc_codec *c;
+  Job *job;
+  Tile *tl;
+  uint64_t tile;
+  int nl;
+  CblkRecords &rec(int j) { return job->rec[tile * c->info.nPar + j]; }
+
+  template <class W>
+  void length_code(W &w, CblkRecords &R, uint64_t i, int q, int est) {
+    int16_t *cp = &R.cp[i * (nl + 1) + 1];
+    const int16_t zc = (int16_t)(cp[q] - 1), zp = (int16_t)(cp[q - 1] - 1);
+    uint16_t beta = est ? R.beta_est[i] : R.beta[i];
+    uint16_t maxb = beta;
+    const uint32_t *L = &R.L[i * kMaxPasses];
+    const uint32_t len = L[zc] - ((zp >= 0) ? L[zp] : 0);
+    const uint16_t cpb = (uint16_t)std::log2(zc - zp);
+    int32_t lb = (uint16_t)std::log2(len) + 1 - cpb;
+    if (maxb < lb) maxb = (uint16_t)lb;
+    while (beta < maxb) {
+      w.bit(1);
+      ++beta;
+    }
+    w.bit(0);
+    for (int32_t m = 1 << (beta + cpb - 1); m > 0; m >>= 1) w.bit(len & (uint32_t)m);
+    if (est == 0) {
+      R.beta[i] = beta;
+      R.beta_est[i] = beta;
+    } else if (est == 1) {
+      R.beta_est[i] = beta;
+    }
+  }
+
+  // header bytes of packet (param j, resolution r, precinct p, layer q)
+  template <class W>
+  void packet_header(W &w, int j, Resolution &res, uint64_t p, int q, int est, uint64_t *body) {
+    CblkRecords &R = rec(j);
+    w.bit(1);
+    for (auto &sb : res.sb) {
+      Precinct &pr = sb.prec[p];
+      for (uint64_t k = 0; k < pr.ncbs; ++k) {
+        const uint64_t i = pr.first_cb + k;
+        int16_t *cp = &R.cp[i * (nl + 1) + 1];
+        const int dz = cp[q] - cp[q - 1];
+        if (cp[q - 1] == 0) {
+          tagtree_encode(pr.incl, w, (uint32_t)q + 1, k, est);
+          if (cp[q])
+            for (uint32_t t = 1; t <= R.Kmsbs[i] + 1u; ++t) tagtree_encode(pr.msbs, w, t, k, est);
+        } else {
+          w.bit((uint64_t)dz);
+        }
+        if (dz) {
+          delta_z_code(w, dz);
+          length_code(w, R, i, q, est);
+          if (est <= 1 && body) {
+            const uint32_t *L = &R.L[i * kMaxPasses];
+            const uint32_t start = (cp[q - 1] == 0) ? 0 : L[cp[q - 1] - 1];
+            const uint32_t end = (cp[q] == 0) ? 0 : L[cp[q] - 1];
+            *body += end - start;
+          }
+        }
+      }
+    }
+    w.flush();
+  }
+
+  // create_packet for est 1/2: returns the header estimate in bytes
+  int64_t estimate_packet(int j, Resolution &res, uint64_t p, int q, int est) {
+    Packet &pk = res.pk[q * res.nprec + p];
+    if (!pk.e) {
+      if (est == 2) pk.e = 0;
+      return 1;
+    }
+    BitCounter w;
+    uint64_t body = 0;
+    packet_header(w, j, res, p, q, est, est == 1 ? &body : nullptr);
+    if (est == 1) pk.body_size += body;
+    if (est == 2) pk.e = 0;
+    return (int64_t)w.n;
+  }
+
+  void real_packet(int j, Resolution &res, uint64_t p, int q) {
+    Packet &pk = res.pk[q * res.nprec + p];
+    pk.built = true;
+    if (!pk.e) {
+      pk.header.assign(1, 0);
+      pk.size = 1;
+      return;
+    }
+    BitWriter w;
+    packet_header(w, j, res, p, q, 0, nullptr);
+    pk.header = std::move(w.out);
+    pk.size = (uint32_t)(pk.body_size + pk.header.size());
+  }
+
+  int64_t quality_layer(uint16_t threshold, int q, int est) {
+    int64_t ql = 0;
+    for (int j = 0; j < c->info.nPar; ++j) {
+      Param &par = tl->par[j];
+      CblkRecords &R = rec(j);
+      for (auto &res : par.res) {
+        for (auto &sb : res.sb)
+          for (uint64_t p = 0; p < res.nprec; ++p) {
+            Precinct &pr = sb.prec[p];
+            for (uint64_t k = 0; k < pr.ncbs; ++k) {
+              const uint64_t i = pr.first_cb + k;
+              int16_t *cp = &R.cp[i * (nl + 1) + 1];
+              const uint16_t *S = &R.S[i * (kMaxPasses + 1)];
+              const uint32_t *L = &R.L[i * kMaxPasses];
+              int16_t z;
+              for (z = R.Z[i]; z >= 0 && S[z] < threshold; --z) {
+              }
+              if (z < cp[q - 1]) {
+                cp[q] = cp[q - 1];
+                if (cp[q] == 1) ql += L[0];
+              } else {
+                cp[q] = z;
+                if (z > 0) {
+                  ql += L[z - 1];
+                  if (z > 1 && cp[q - 1] > 0) ql -= L[cp[q - 1] - 1];
+                }
+              }
+              if (cp[q] > cp[q - 1]) res.pk[q * res.nprec + p].e = 1;
+              if (!est) {
+                if (q == 0) pr.msbs.set(k, R.Kmsbs[i]);
+                if (q + 1 == nl) {
+                  uint8_t l = 0;
+                  while (l < nl && cp[l] == 0) ++l;
+                  pr.incl.set(k, l);
+                }
+              }
+            }
+          }
+        for (uint64_t p = 0; p < res.nprec; ++p) ql += estimate_packet(j, res, p, q, est + 1);
+      }
+    }
+    return ql;
+  }
+
+  void rate_control() {
+    int64_t est_tile = 0, est_ql = 0;
+    const uint64_t tsz = tl->box.volume() * (uint64_t)c->info.nPar;
+    const uint64_t hdr = (uint64_t)std::ceil((float)((uint32_t)c->control.headerSize + tl->header_size) / nl);
+    for (int l = 0; l < nl; ++l) {
+      const int64_t target = (int64_t)std::ceil(c->control.bitrate[l] * tsz / 8.0f);
+      uint16_t slope;
+      if (target > 0.0f) {
+        uint16_t smin = tl->slope_min, smax = tl->slope_max, tmp = tl->slope_min;
+        slope = smin;
+        while (smin < smax) {
+          slope = (uint16_t)((smax + smin) >> 1);
+          est_ql = quality_layer(slope, l, 1) + est_tile + (int64_t)hdr;
+          if (est_ql > target) {
+            if (smin == slope) smax = slope;
+            smin = slope;
+          } else if (est_ql < target) {
+            if (smax == slope) smin = slope;
+            smax = slope;
+            tmp = slope;
+          } else {
+            smax = smin = slope;
+          }
+        }
+        if (est_ql > target) slope = std::max(tmp, tl->slope_min);
+      } else {
+        slope = 0;
+      }
+      est_ql = quality_layer(slope, l, 0);
+      est_tile += est_ql + (int64_t)hdr;
+    }
+  }
+
+  void packets() {
+    for (int j = 0; j < c->info.nPar; ++j)
+      for (auto &res : tl->par[j].res)
+        for (uint64_t p = 0; p < res.nprec; ++p)
+          for (int l = 0; l < nl; ++l) real_packet(j, res, p, l);
+  }
+};
+
+}  // namespace
+
+// Rate control for one tile: fills cp (cut points per layer) and the packet
+// contribution flags / body sizes. Needs tile slope bounds and records.
+int t2_rate_control(bwc_codec *codec, Job *job, uint64_t tile) {
+  T2 t{codec, job, &job->plan.tiles[tile], tile, codec->control.nLayers};
+  t.rate_control();
+  return 0;
+}
+
+// Packet headers; needs the codeblock data only at assembly time.
+int t2_packets(bwc_codec *codec, Job *job, uint64_t tile) {
+  T2 t{codec, job, &job->plan.tiles[tile], tile, codec->control.nLayers};
+  t.packets();
+  return 0;
+}
+
+int t2_encode_tile(bwc_codec *codec, Job *job, uint64_t tile) {
+  t2_rate_control(codec, job, tile);
+  return t2_packets(codec, job, tile);
+}
+
+uint64_t header_size(const bwc_codec *c, const bwc_stream *st) {
+  uint64_t h = 111 + (uint64_t)c->info.nPar * c->control.nTiles * 2 * kPrecBytes + (uint64_t)c->control.nLayers * 4;
+  if (st && st->codestream.aux) h += 6 + st->codestream.aux->size;
+  if (st && st->codestream.com) h += 6 + st->codestream.com->size;
+  return h;
+}
+
+namespace {
+struct ByteWriter {
+  uint8_t *m;
+  uint64_t L = 0, Lmax;
+  bool err = false;
+  void sym(uint64_t v, int n) {
+    if (err) return;
+    if (L + (uint64_t)n > Lmax) {
+      err = true;
+      return;
+    }
+    for (int i = n; i-- > 0;) m[L++] = (uint8_t)((v >> (8 * i)) & 0xFF);
+  }
+  void chunk(const uint8_t *p, uint64_t n) {
+    if (err) return;
+    if (L + n > Lmax) {
+      err = true;
+      return;
+    }
+    if (n) std::memcpy(m + L, p, n);
+    L += n;
+  }
+};
+}  // namespace
+
+// Packet sequencing and codestream assembly (codestream.c:139-570,
+// :1329-1413). Body bytes are copied straight from the packed codeblock data.
+size_t assemble_codestream(bwc_codec *codec, Job *job, const bwc_stream *st, uint8_t *out) {
+  bwc_gl_ctrl &ctl = codec->control;
+  bwc_gl_inf &inf = codec->info;
+  const int nl = ctl.nLayers;
+  const bool rlcp = ctl.progression == bwc_prog_RLCP;
+  ctl.codestreamSize = ctl.headerSize + 2;
+  for (auto &tl : job->plan.tiles) {
+    tl.seq.clear();
+    uint64_t size = 0;
+    const int nA = rlcp ? ctl.nDecomp + 1 : nl, nB = rlcp ? nl : ctl.nDecomp + 1;
+    for (int a = 0; a < nA; ++a)
+      for (int b = 0; b < nB; ++b)
+        for (int j = 0; j < inf.nPar; ++j) {
+          const int r = rlcp ? a : b, l = rlcp ? b : a;
+          Resolution &res = tl.par[j].res[r];
+          for (uint64_t p = 0; p < res.nprec; ++p) {
+            Packet &pk = res.pk[l * res.nprec + p];
+            tl.seq.push_back(&pk);
+            size += pk.size + (ctl.error_resilience ? 6 : 0);
+          }
+        }
+    tl.body_size = size;
+    ctl.codestreamSize += tl.header_size + tl.body_size;
+  }
+  ByteWriter w{out, 0, ctl.codestreamSize};
+  w.sym(0xFF50, 2);
+  w.sym(ctl.codestreamSize, 8);
+  w.sym(0xFF51, 2);
+  w.sym(31, 2);
+  w.sym(inf.nX, 8), w.sym(inf.nY, 8), w.sym(inf.nZ, 8), w.sym(inf.nTS, 8);
+  w.sym(inf.nPar, 1);
+  w.sym((uint8_t)inf.data_prec, 1);
+  w.sym((uint8_t)inf.codec_prec, 1);
+  w.sym(0xFF52, 2);
+  w.sym(50 + nl * 4, 2);
+  w.sym(ctl.CSsgc, 2);
+  w.sym(ctl.error_resilience, 1);
+  w.sym(ctl.quantization_style, 1);
+  w.sym(ctl.guard_bits, 1);
+  w.sym(ctl.qt_exponent, 1);
+  w.sym(ctl.qt_mantissa, 2);
+  w.sym(ctl.progression, 1);
+  w.sym((uint64_t)(ctl.KernelX << 6 | ctl.KernelY << 4 | ctl.KernelZ << 2 | ctl.KernelTS), 1);
+  w.sym(ctl.decompX, 1), w.sym(ctl.decompY, 1), w.sym(ctl.decompZ, 1), w.sym(ctl.decompTS, 1);
+  w.sym((uint64_t)(ctl.precSizeY << 4 | ctl.precSizeX), 1);
+  w.sym((uint64_t)(ctl.precSizeTS << 4 | ctl.precSizeZ), 1);
+  w.sym(ctl.cbX, 1), w.sym(ctl.cbY, 1), w.sym(ctl.cbZ, 1), w.sym(ctl.cbTS, 1);
+  w.sym(ctl.Qm, 1);
+  w.sym(ctl.tileSizeX, 8), w.sym(ctl.tileSizeY, 8), w.sym(ctl.tileSizeZ, 8), w.sym(ctl.tileSizeTS, 8);
+  w.sym(nl, 1);
+  for (int l = 0; l < nl; ++l) {
+    uint32_t bits;
+    std::memcpy(&bits, &ctl.bitrate[l], 4);
+    w.sym(bits, 4);
+  }
+  if (st && st->codestream.aux) {
+    w.sym(0xFF53, 2);
+    w.sym(st->codestream.aux->size + 4, 4);
+    w.chunk(st->codestream.aux->memory, st->codestream.aux->size);
+  }
+  if (st && st->codestream.com) {
+    w.sym(0xFF57, 2);
+    w.sym(st->codestream.com->size + 2, 2);
+    w.chunk(st->codestream.com->memory, st->codestream.com->size);
+  }
+  w.sym(0xFF58, 2);
+  w.sym((uint64_t)inf.nPar * ctl.nTiles * 2 * kPrecBytes, 2);
+  for (auto &tl : job->plan.tiles)
+    for (auto &par : tl.par) {
+      uint64_t r;
+      std::memcpy(&r, &par.pmin, 8);
+      w.sym(r, kPrecBytes);
+      std::memcpy(&r, &par.pmax, 8);
+      w.sym(r, kPrecBytes);
+      par.pmax = -1.7976931348623157e+308;
+      par.pmin = 1.7976931348623157e+308;
+    }
+  for (uint64_t ti = 0; ti < job->plan.tiles.size(); ++ti) {
+    Tile &tl = job->plan.tiles[ti];
+    w.sym(0xFF90, 2);
+    w.sym(14, 2);
+    w.sym(tl.index, 4);
+    w.sym(tl.body_size, 8);
+    w.sym(0xFF93, 2);
+    // body slices follow the packet order: for packet (j, r, p, l) every
+    // codeblock of every subband of precinct p contributes [L[cp[l-1]-1], L[cp[l]-1])
+    uint64_t k = 0;
+    const int nA = rlcp ? ctl.nDecomp + 1 : nl, nB = rlcp ? nl : ctl.nDecomp + 1;
+    for (int a = 0; a < nA; ++a)
+      for (int b = 0; b < nB; ++b)
+        for (int j = 0; j < inf.nPar; ++j) {
+          const int r = rlcp ? a : b, l = rlcp ? b : a;
+          Resolution &res = tl.par[j].res[r];
+          CblkRecords &R = job->rec[ti * inf.nPar + j];
+          for (uint64_t p = 0; p < res.nprec; ++p, ++k) {
+            Packet &pk = res.pk[l * res.nprec + p];
+            if (ctl.error_resilience) {
+              w.sym(0xFF91, 2);
+              w.sym(4, 2);
+              w.sym(k % 0x100000000ull, 4);
+            }
+            w.chunk(pk.header.data(), pk.header.size());
+            if (ctl.error_resilience) w.sym(0xFF92, 2);
+            if (pk.e) {
+              for (auto &sb : res.sb) {
+                Precinct &pr = sb.prec[p];
+                for (uint64_t c = 0; c < pr.ncbs; ++c) {
+                  const uint64_t i = pr.first_cb + c;
+                  const int16_t *cp = &R.cp[i * (nl + 1) + 1];
+                  if (cp[l] - cp[l - 1] <= 0) continue;
+                  const uint32_t *L = &R.L[i * kMaxPasses];
+                  const uint32_t start = (cp[l - 1] == 0) ? 0 : L[cp[l - 1] - 1];
+                  const uint32_t end = (cp[l] == 0) ? 0 : L[cp[l] - 1];
+                  w.chunk(R.data.data() + R.data_off[i] + start, end - start);
+                }
+              }
+            }
+          }
+        }
+  }
+  w.sym(0xFFFF, 2);
+  return w.err ? 0 : (size_t)w.L;
+}
+
+}  // namespace bwc

@@ -1,0 +1,86 @@
+/*
+ * bwc_amd.h — extra C-ABI entry points of the MI355X libbwc.so.
+ *
+ * None of these has a counterpart in the reference's public API: they expose
+ * single stages of bwc_compress (reference libbwc.c:3231-3383) on the GPU so
+ * that parity tests can compare each stage with the CPU oracle, and report the
+ * device-side stage timings of the last bwc_compress call for benchmarking.
+ * All pointers are plain host pointers unless stated; sizes are element
+ * counts. Return 0 on success, non-zero on error (bwc_amd_last_error()).
+ */
+#ifndef BWC_AMD_H
+#define BWC_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "bwc.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* library identification; 0 when no HIP device is usable */
+int bwc_amd_version(void);
+int bwc_amd_device_count(void);
+const char *bwc_amd_last_error(void);
+
+/* The energy-gain table is process-global and accumulates on every codec
+   configuration and bwc_set_decomp call, as in the reference (dwt.c:80,
+   :808-1066). This returns it to the state of a fresh process, so a test or
+   benchmark can run many compressions in one process with the gains a single
+   compression per process would see. */
+void bwc_amd_reset_gain_lut(void);
+
+/* geometry of a codec after bwc_create_compression */
+uint64_t bwc_amd_ncblk(bwc_codec *codec, uint64_t tile, uint8_t param);
+/* per codeblock in access order (libbwc.c:1651-1683): box[8] tile-local
+   x0 y0 z0 t0 x1 y1 z1 t1, sb[3] {dwt_gain, qt_step_size,
+   qt_effective_step_size}, meta[4] {highband_flag, Kmax, qt_exponent,
+   index in precinct} */
+int bwc_amd_cblk_info(bwc_codec *codec, uint64_t tile, uint8_t param, uint32_t *box, double *sb, uint32_t *meta);
+/* tile-local size of (tile, param): dims[4] */
+int bwc_amd_param_dims(bwc_codec *codec, uint64_t tile, uint64_t dims[4]);
+
+/* fill + normalize + forward DWT of one (tile, param) on the GPU
+   (libbwc.c:622-1078, dwt.c:1216-1724); out receives the tile-local float64
+   coefficients; mm receives {min, max, alpha, beta} */
+int bwc_amd_stage_dwt(bwc_codec *codec, uint64_t tile, uint8_t param, const void *input, double *out, double mm[4]);
+
+/* inverse DWT of tile-local float64 coefficients in place on the GPU
+   (dwt.c:1738-2246) */
+int bwc_amd_stage_idwt(bwc_codec *codec, uint64_t tile, uint8_t param, double *coeffs);
+
+/* tier-1 of all codeblocks of (tile, param) from tile-local coefficients
+   (tier1.c:2824-3095). Per codeblock in access order: K, Kmsbs, Z (uint8),
+   L (190 x uint32 bytes), S (191 x uint16); data receives each codeblock's
+   L[Z-1] MQ bytes back to back, data_off[ncb+1] their offsets; slopes[2] the
+   tile slope {min, max}. data_cap is the capacity of data in bytes. */
+int bwc_amd_stage_t1(bwc_codec *codec, uint64_t tile, uint8_t param, const double *coeffs, uint8_t *K,
+                     uint8_t *Kmsbs, uint8_t *Z, uint32_t *L, uint16_t *S, uint8_t *data, uint64_t data_cap,
+                     uint64_t *data_off, uint16_t slopes[2]);
+
+/* Host tier-2 on externally supplied tier-1 records (tier2.c:1297-1324,
+   codestream.c:1329-1413): bwc_amd_begin_records resets the per-call state,
+   bwc_amd_set_records installs one (tile, param) in the layout of
+   bwc_amd_stage_t1 (data holds each codeblock's bytes at data_off[i],
+   data_off[ncb] = total) plus {min, max, alpha, beta}, and
+   bwc_amd_encode_records runs rate control, packets and codestream assembly
+   into stream->out, returning the codestream size (0 on error). Used to test
+   tier-2 and the codestream writer without a GPU. */
+void bwc_amd_begin_records(bwc_codec *codec);
+int bwc_amd_set_records(bwc_codec *codec, uint64_t tile, uint8_t param, const uint8_t *K, const uint8_t *Kmsbs,
+                        const uint8_t *Z, const uint32_t *L, const uint16_t *S, const uint8_t *data,
+                        const uint64_t *data_off, const double mm[4]);
+size_t bwc_amd_encode_records(bwc_codec *codec, bwc_stream *stream);
+
+/* device-side stage times of the last bwc_compress on this codec, in ms:
+   [0] fill+minmax [1] normalize [2] DWT [3] quantize [4] tier-1
+   [5] records D2H [6] rate control (host) [7] gather+D2H [8] packets+assembly
+   (host) [9] total; launches[0] = number of DWT kernel launches */
+int bwc_amd_last_timings(bwc_codec *codec, double ms[10], int launches[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,84 @@
+"""Host-side parts of libbwc without a GPU: the C-ABI library loads and exports
+every declared symbol, its codec geometry equals the oracle's, and its tier-2
++ codestream writer, fed with the oracle's tier-1 records, reproduces the
+reference codestreams byte for byte (golden fixtures)."""
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from bigwhoop_amd import bwc
+from golden_cases import GOLDEN, cases, codec_kw, make_input, sha
+
+ROOT = Path(__file__).resolve().parents[1]
+CASES = cases()
+
+
+def declared_symbols():
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        txt = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        for m in re.finditer(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", txt):
+            n = m.group(1)
+            if n.startswith(("bwc_", "create_codec", "configure_codec", "set_")):
+                names.add(n)
+    return names
+
+
+def test_library_exports_declared_symbols():
+    import ctypes
+    L = ctypes.CDLL(str(bwc.LIB_PATH))
+    names = declared_symbols()
+    assert len(names) > 30
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def _oracle_records(o, data, t, p):
+    work, mm = o.fill_normalize(data, t, p)
+    o.dwt_forward(work, t, p)
+    o.t1_encode(work, t, p)
+    n = o.ncblk(t, p)
+    K = np.zeros(n, np.uint8)
+    Km = np.zeros(n, np.uint8)
+    Z = np.zeros(n, np.uint8)
+    L = np.zeros((n, 190), np.uint32)
+    S = np.zeros((n, 191), np.uint16)
+    chunks, off = [], [0]
+    for i in range(n):
+        r = o.cblk_record(i, t, p)
+        K[i], Km[i], Z[i] = r["K"], r["Kmsbs"], r["Z"]
+        L[i, : r["Z"]] = r["L"]
+        S[i, : r["Z"] + 1] = r["S"]
+        chunks.append(r["data"])
+        off.append(off[-1] + len(r["data"]))
+    return K, Km, Z, L, S, b"".join(chunks), np.array(off, np.uint64), mm
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c["name"] for c in CASES])
+def test_geometry_and_tier2_against_golden(case):
+    data = make_input(case)
+    shape = case["gen"]["shape"]
+    kw = codec_kw(case)
+    o = ol.Oracle(shape, npar=case["npar"], prec=case["prec"], rate=case["rate"], **kw)
+    c = bwc.Codec(shape, npar=case["npar"], prec=case["prec"], rate=case["rate"], **kw)
+    try:
+        assert o.ntiles == len([1 for _ in range(o.ntiles)])
+        c.begin_records()
+        for t in range(o.ntiles):
+            assert c.param_dims(t) == o.param_dims(t)
+            for p in range(case["npar"]):
+                ob, osb, om = o.geometry(t, p)
+                cbx, csb, cm = c.geometry(t, p)
+                np.testing.assert_array_equal(cbx.astype(np.uint64), ob)
+                np.testing.assert_array_equal(csb, osb)
+                np.testing.assert_array_equal(cm, om)
+                c.set_records(t, p, *_oracle_records(o, data, t, p))
+        cs = c.encode_records()
+        assert len(cs) == case["codestream_size"]
+        assert sha(cs) == case["codestream_sha256"]
+    finally:
+        c.close()
+        o.close()
