@@ -403,7 +403,6 @@ size_t bwc_compress(bwc_codec *const codec, bwc_stream *const stream) {
       gpu::record(ws, 2);
       if (gpu::dwt_forward(ws, codec, par, ws->d_work)) goto fail;
       gpu::record(ws, 3);
-      for (int a = 0; a < 4; ++a) (void)a;
       {
         double mm[4];
         if (gpu::read_norm(ws, mm)) goto fail;
@@ -415,10 +414,10 @@ size_t bwc_compress(bwc_codec *const codec, bwc_stream *const stream) {
       ms[0] += gpu::elapsed(ws, 0, 1);
       ms[1] += gpu::elapsed(ws, 1, 2);
       ms[2] += gpu::elapsed(ws, 2, 3);
-      dwt_launch += 0;
+      dwt_launch += ws->last_dwt_launches;
       gpu::record(ws, 4);
       if (gpu::t1_encode(ws, codec, par, ws->d_work, job->rec[ti * codec->info.nPar + p], &tl.slope_min,
-                         &tl.slope_max))
+                         &tl.slope_max, p))
         goto fail;
       gpu::record(ws, 5);
       gpu::sync(ws);
@@ -569,7 +568,7 @@ int bwc_amd_stage_t1(bwc_codec *codec, uint64_t tile, uint8_t param, const doubl
   if (gpu::copy_h2d(ws, ws->d_work, coeffs, n * sizeof(double))) return 1;
   CblkRecords R;
   uint16_t smin = 0xFFFF, smax = 0;
-  if (gpu::t1_encode(ws, codec, par, ws->d_work, R, &smin, &smax)) return 1;
+  if (gpu::t1_encode(ws, codec, par, ws->d_work, R, &smin, &smax, 0)) return 1;
   std::vector<uint32_t> len(R.n);
   uint64_t tot = 0;
   for (uint64_t i = 0; i < R.n; ++i) {

@@ -105,6 +105,10 @@ struct CblkRecords {
   // included bytes, packed after rate control: data_off[i] into `data`
   std::vector<uint64_t> data_off;
   std::vector<uint8_t> data;
+  // where tier-1 left the full MQ bytes on the device: workspace slot and
+  // per-codeblock offsets inside it
+  int slot = 0;
+  std::vector<uint64_t> src_off;
   void resize(uint64_t ncb, int nlayers);
 };
 

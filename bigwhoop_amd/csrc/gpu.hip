@@ -44,8 +44,10 @@ Workspace::~Workspace() {
   (void)hipSetDevice(device);
   void *ptrs[] = {d_input, d_work,  d_red,  d_norm, d_cbbox, d_cbsb, d_cbmeta, d_qoff,   d_doff, d_q,
                   d_state, d_data,  d_K,    d_Kmsbs, d_Z,    d_L,    d_S,      d_snap,   d_mse,  d_flags,
-                  d_packed, d_gather};
+                  d_packed, d_gather, d_bp, d_bpoff, d_sym, d_symoff, d_pcnt, d_pmse, d_list};
   for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  for (void *p : d_datap)
     if (p) (void)hipFree(p);
   if (h_pinned) (void)hipHostFree(h_pinned);
   for (auto &e : ev)
@@ -337,6 +339,7 @@ static int dwt_run(Workspace *ws, const bwc_codec *c, const Param &par, double *
   const uint8_t dec[4] = {ctl.decompX, ctl.decompY, ctl.decompZ, ctl.decompTS};
   const int kern[4] = {ctl.KernelX, ctl.KernelY, ctl.KernelZ, ctl.KernelTS};
   const int nd = ctl.nDecomp;
+  ws->last_dwt_launches = 0;
   for (int step = 0; step < nd; ++step) {
     const int level = fwd ? step : nd - 1 - step;
     const Box &rb = par.res[nd - level].box;
@@ -383,6 +386,7 @@ static int dwt_run(Workspace *ws, const bwc_codec *c, const Param &par, double *
         else launch(dwt_axis_kernel<1, false>);
       }
       HIP_OK(hipGetLastError());
+      ++ws->last_dwt_launches;
     }
   }
   return 0;
@@ -773,6 +777,9 @@ __device__ uint16_t slope2log(double lambda) {
   return (uint16_t)v;
 }
 
+__device__ void finish_block(MqState last, uint8_t *buf, uint64_t cap, const uint64_t *sn, const double *mse, int Z,
+                             uint32_t *L, uint16_t *S, uint32_t *flags);
+
 __global__ void __launch_bounds__(64) t1_encode_kernel(
     uint64_t ncb, const uint32_t *__restrict__ cbbox, const double *__restrict__ cbsb,
     const uint32_t *__restrict__ cbmeta, const uint64_t *__restrict__ qoff, const uint64_t *__restrict__ doff,
@@ -868,6 +875,17 @@ __global__ void __launch_bounds__(64) t1_encode_kernel(
     Zp[cb] = 0;
     return;
   }
+  Kmsbsp[cb] = (uint8_t)(Kmax - K);
+  Zp[cb] = (uint8_t)Z;
+  finish_block(C.s, C.buf, C.cap, sn, mse, Z, L, S, flags);
+}
+
+// Truncation lengths of every pass snapshot (mq.c:706-759), the convex hull
+// and the log slopes (tier1.c:2161-2267) of one finished codeblock; folds its
+// non-zero slopes into the tile bounds (tier1.c:3047-3061).
+__device__ void finish_block(MqState last, uint8_t *buf, uint64_t cap, const uint64_t *sn, const double *mse, int Z,
+                             uint32_t *L, uint16_t *S, uint32_t *flags) {
+  (void)last;
   for (int z = 0; z < Z; ++z) {
     MqState s;
     s.L = (int64_t)sn[2 * z];
@@ -876,12 +894,8 @@ __global__ void __launch_bounds__(64) t1_encode_kernel(
     s.A = (uint16_t)(w >> 32);
     s.t = (int8_t)(uint8_t)(w >> 48);
     s.T = (uint8_t)(w >> 56);
-    L[z] = (uint32_t)trunc_len(s, C.buf, C.cap);
+    L[z] = (uint32_t)trunc_len(s, buf, cap);
   }
-  Kmsbsp[cb] = (uint8_t)(Kmax - K);
-  Zp[cb] = (uint8_t)Z;
-  // convex hull and log slopes (tier1.c:2161-2267); lambda kept in the mse
-  // scratch after its value has been consumed
   uint8_t hull[192];
   double lam[191];
   for (int z = 0; z <= Z; ++z) lam[z] = 0;
@@ -927,6 +941,66 @@ __global__ void __launch_bounds__(64) t1_encode_kernel(
   atomicMax(&flags[2], smax);
 }
 
+// quantisation straight into bitplanes for the fast tier-1 path
+// (tier1.c:630-728): per codeblock, plane b holds one nibble per stripe
+// column (row 0 in bit 3), two columns per byte; plane 63 holds the signs.
+__global__ void bitplane_kernel(const double *__restrict__ work, uint64_t W, uint64_t H, uint64_t D,
+                                const uint32_t *__restrict__ cbbox, const double *__restrict__ cbsb,
+                                const uint64_t *__restrict__ bpoff, uint8_t *__restrict__ bp,
+                                uint8_t *__restrict__ Kout, uint64_t ncb) {
+  __shared__ int smax[256];
+  for (uint64_t cb = blockIdx.x; cb < ncb; cb += gridDim.x) {
+    const uint32_t *b = cbbox + 8 * cb;
+    const uint64_t cw = b[4] - b[0], ch = b[5] - b[1], cd = b[6] - b[2], ct = b[7] - b[3];
+    const uint64_t ns = (ch + 3) / 4;
+    const uint64_t ncol = cw * ns * cd * ct, ncolh = (ncol + 1) / 2;
+    const double step = cbsb[3 * cb];
+    uint8_t *planes = bp + bpoff[cb];
+    int kb = 0;
+    for (uint64_t pr = threadIdx.x; pr < ncolh; pr += blockDim.x) {
+      uint64_t q[8];
+      uint32_t sg = 0;
+      for (int h = 0; h < 2; ++h) {
+        const uint64_t k = 2 * pr + h;
+        for (int r = 0; r < 4; ++r) {
+          uint64_t qq = 0;
+          if (k < ncol) {
+            const uint64_t x = k % cw, j = (k / cw) % ns, sl = k / (cw * ns);
+            const uint64_t z = sl % cd, t = sl / cd, row = 4 * j + r;
+            if (row < ch) {
+              const uint64_t idx = (b[0] + x) + W * ((b[1] + row) + H * ((b[2] + z) + D * (b[3] + t)));
+              uint64_t raw = (uint64_t)__double_as_longlong(work[idx]);
+              sg |= (uint32_t)((raw >> 63) << (3 - r + 4 * h));
+              raw &= 0x7FFFFFFFFFFFFFFFull;
+              qq = x86_to_u64(__longlong_as_double((long long)raw) / step);
+              const int nb = qq ? 64 - __clzll((long long)qq) : 0;
+              kb = nb > kb ? nb : kb;
+            }
+          }
+          q[4 * h + r] = qq;
+        }
+      }
+      for (int pl = 0; pl < 63; ++pl) {
+        uint32_t v = 0;
+        for (int h = 0; h < 2; ++h)
+          for (int r = 0; r < 4; ++r) v |= (uint32_t)((q[4 * h + r] >> pl) & 1) << (3 - r + 4 * h);
+        planes[(uint64_t)pl * ncolh + pr] = (uint8_t)v;
+      }
+      planes[63 * ncolh + pr] = (uint8_t)sg;
+    }
+    smax[threadIdx.x] = kb;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) smax[threadIdx.x] = max(smax[threadIdx.x], smax[threadIdx.x + s]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) Kout[cb] = (uint8_t)smax[0];
+    __syncthreads();
+  }
+}
+
+#include "t1v2.inc"
+
 __global__ void init_flags_kernel(uint32_t *flags) {
   if (threadIdx.x == 0) {
     flags[0] = 0;
@@ -937,11 +1011,11 @@ __global__ void init_flags_kernel(uint32_t *flags) {
 }
 
 // copy each codeblock's included prefix into one packed buffer
-__global__ void gather_kernel(const uint8_t *__restrict__ data, const uint64_t *__restrict__ doff,
-                              const uint64_t *__restrict__ g, uint64_t ncb, uint8_t *__restrict__ out) {
+__global__ void gather_kernel(const uint8_t *__restrict__ data, const uint64_t *__restrict__ g, uint64_t ncb,
+                              uint8_t *__restrict__ out) {
   for (uint64_t cb = blockIdx.x; cb < ncb; cb += gridDim.x) {
-    const uint64_t len = g[2 * cb + 1], dst = g[2 * cb];
-    const uint8_t *src = data + doff[cb];
+    const uint64_t dst = g[3 * cb], len = g[3 * cb + 1];
+    const uint8_t *src = data + g[3 * cb + 2];
     for (uint64_t i = threadIdx.x; i < len; i += blockDim.x) out[dst + i] = src[i];
   }
 }
@@ -1013,7 +1087,7 @@ static int upload_tables() {
 Workspace *workspace_of(Job *job);
 
 int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double *work, CblkRecords &rec,
-              uint16_t *slope_min, uint16_t *slope_max) {
+              uint16_t *slope_min, uint16_t *slope_max, int slot) {
   const uint64_t ncb = par.ncb;
   rec.resize(ncb, c->control.nLayers);
   if (!ncb) return 0;
@@ -1042,8 +1116,7 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
   if (ncb > ws->cap_cb) {
     void **tabs[] = {(void **)&ws->d_cbbox, (void **)&ws->d_cbsb, (void **)&ws->d_cbmeta, (void **)&ws->d_qoff,
                      (void **)&ws->d_doff,  (void **)&ws->d_K,    (void **)&ws->d_Kmsbs,  (void **)&ws->d_Z,
-                     (void **)&ws->d_L,     (void **)&ws->d_S,    (void **)&ws->d_snap,   (void **)&ws->d_mse,
-                     (void **)&ws->d_gather};
+                     (void **)&ws->d_L,     (void **)&ws->d_S,    (void **)&ws->d_snap,   (void **)&ws->d_mse};
     for (void **p : tabs)
       if (*p) {
         HIP_OK(hipFree(*p));
@@ -1062,25 +1135,18 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
     HIP_OK(hipMalloc((void **)&ws->d_S, n * (kMaxPasses + 1) * sizeof(uint16_t)));
     HIP_OK(hipMalloc((void **)&ws->d_snap, n * 2 * kMaxPasses * sizeof(uint64_t)));
     HIP_OK(hipMalloc((void **)&ws->d_mse, n * kMaxPasses * sizeof(double)));
-    HIP_OK(hipMalloc((void **)&ws->d_gather, 2 * n * sizeof(uint64_t)));
     ws->cap_cb = ncb;
   }
-  cap = ws->cap_q;
-  CHK(ensure((void **)&ws->d_q, &cap, qo * sizeof(uint64_t)));
-  ws->cap_q = cap;
-  {
-    static size_t state_cap = 0;
-    (void)state_cap;
+  if (slot < 0 || slot > 255) {
+    std::snprintf(g_err, sizeof g_err, "parameter slot out of range");
+    return 1;
   }
-  size_t scap = 0;
-  if (ws->d_state) {
-    HIP_OK(hipFree(ws->d_state));
-    ws->d_state = nullptr;
-  }
-  CHK(ensure((void **)&ws->d_state, &scap, (qo / 4 + 1) * sizeof(uint16_t)));
-  cap = ws->cap_data;
-  CHK(ensure((void **)&ws->d_data, &cap, dof));
-  ws->cap_data = cap;
+  cap = ws->cap_datap[slot];
+  CHK(ensure((void **)&ws->d_datap[slot], &cap, dof));
+  ws->cap_datap[slot] = cap;
+  uint8_t *const d_data = ws->d_datap[slot];
+  rec.slot = slot;
+  rec.src_off = doff;
   if (!ws->d_flags) HIP_OK(hipMalloc((void **)&ws->d_flags, 16));
   HIP_OK(hipMemcpyAsync(ws->d_cbbox, par.cb_box.data(), 8 * ncb * sizeof(uint32_t), hipMemcpyHostToDevice,
                         ws->stream));
@@ -1090,14 +1156,107 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
   HIP_OK(hipMemcpyAsync(ws->d_doff, doff.data(), (ncb + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, ws->stream));
   hipLaunchKernelGGL(init_flags_kernel, dim3(1), dim3(64), 0, ws->stream, ws->d_flags);
   const uint64_t W = par.box.len(0), H = par.box.len(1), D = par.box.len(2);
-  hipLaunchKernelGGL(quantize_kernel, dim3((uint32_t)std::min<uint64_t>(ncb, 65535)), dim3(256), 0, ws->stream, work,
-                     W, H, D, ws->d_cbbox, ws->d_cbsb, ws->d_qoff, ws->d_q, ws->d_state, ws->d_K, ncb);
-  HIP_OK(hipGetLastError());
-  hipLaunchKernelGGL(t1_encode_kernel, dim3((uint32_t)((ncb + 63) / 64)), dim3(64), 0, ws->stream, ncb, ws->d_cbbox,
-                     ws->d_cbsb, ws->d_cbmeta, ws->d_qoff, ws->d_doff, ws->d_q, ws->d_state, ws->d_data, ws->d_K,
-                     ws->d_Kmsbs, ws->d_Z, ws->d_L, ws->d_S, ws->d_snap, ws->d_mse, ws->d_flags,
-                     (int)c->control.error_resilience);
-  HIP_OK(hipGetLastError());
+
+  // fast path: wavefront-per-codeblock symbol generation + few-lane MQ, when
+  // every codeblock row fits one wavefront and its state fits LDS
+  uint32_t maxcol = 0, obuf_cap = 0, maxcw = 0;
+  for (uint64_t i = 0; i < ncb; ++i) {
+    const uint32_t *b = &par.cb_box[8 * i];
+    const uint32_t cw = b[4] - b[0], ch = b[5] - b[1], ns = (ch + 3) / 4;
+    const uint32_t ncol = cw * ns * (b[6] - b[2]) * (b[7] - b[3]);
+    maxcol = std::max(maxcol, ncol);
+    maxcw = std::max(maxcw, cw);
+    obuf_cap = std::max(obuf_cap, cw * ch * 2 + cw * ns * 3 + 16);
+  }
+  uint32_t gw = 1;
+  while (gw < maxcw) gw <<= 1;
+  const uint32_t P = gw <= 64 ? 64 / gw : 1;
+  const size_t gen_lds = (size_t)maxcol * 2 + (size_t)((maxcol + 1) / 2) * 8 + (size_t)P * obuf_cap;
+  const char *force = std::getenv("BWC_AMD_T1");
+  const bool v2 = maxcw <= 64 && gen_lds <= 64 * 1024 && !(force && force[0] == '1');
+  const int erres = (int)c->control.error_resilience;
+  if (v2) {
+    std::vector<uint64_t> bpoff(ncb + 1);
+    uint64_t bo = 0;
+    for (uint64_t i = 0; i < ncb; ++i) {
+      bpoff[i] = bo;
+      bo += 64 * ((qoff[i + 1] - qoff[i]) / 4 + 1) / 2 + 64;
+    }
+    bpoff[ncb] = bo;
+    cap = ws->cap_bp;
+    CHK(ensure((void **)&ws->d_bp, &cap, bo));
+    ws->cap_bp = cap;
+    cap = ws->cap_bpoff;
+    CHK(ensure((void **)&ws->d_bpoff, &cap, (ncb + 1) * sizeof(uint64_t)));
+    ws->cap_bpoff = cap;
+    HIP_OK(hipMemcpyAsync(ws->d_bpoff, bpoff.data(), (ncb + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, ws->stream));
+    hipLaunchKernelGGL(bitplane_kernel, dim3((uint32_t)std::min<uint64_t>(ncb, 65535)), dim3(256), 0, ws->stream,
+                       work, W, H, D, ws->d_cbbox, ws->d_cbsb, ws->d_bpoff, ws->d_bp, ws->d_K, ncb);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(rec.K.data(), ws->d_K, ncb, hipMemcpyDeviceToHost, ws->stream));
+    HIP_OK(hipStreamSynchronize(ws->stream));
+    // symbol stream bounds from the actual K: per plane one significance or
+    // refinement symbol per value, one sign per value overall, three run
+    // symbols per stripe column, four segmentation symbols
+    std::vector<uint64_t> symoff(ncb + 1), list;
+    uint64_t so = 0;
+    for (uint64_t i = 0; i < ncb; ++i) {
+      symoff[i] = so;
+      const uint32_t *b = &par.cb_box[8 * i];
+      const uint64_t nv = (uint64_t)(b[4] - b[0]) * (b[5] - b[1]) * (b[6] - b[2]) * (b[7] - b[3]);
+      const uint64_t ncol = (qoff[i + 1] - qoff[i]) / 4;
+      const uint64_t K = rec.K[i];
+      if (K) list.push_back(i);
+      so += (nv * (K + 1) + 3 * ncol * K + 4 * K + 32 + 15) & ~15ull;
+    }
+    symoff[ncb] = so;
+    cap = ws->cap_sym;
+    CHK(ensure((void **)&ws->d_sym, &cap, so + 64));
+    ws->cap_sym = cap;
+    cap = ws->cap_symoff;
+    CHK(ensure((void **)&ws->d_symoff, &cap, (ncb + 1) * sizeof(uint64_t)));
+    ws->cap_symoff = cap;
+    cap = ws->cap_pc;
+    CHK(ensure((void **)&ws->d_pcnt, &cap, (ncb + 1) * kMaxPasses * sizeof(uint32_t)));
+    ws->cap_pc = cap;
+    cap = ws->cap_pm;
+    CHK(ensure((void **)&ws->d_pmse, &cap, (ncb + 1) * kMaxPasses * sizeof(int64_t)));
+    ws->cap_pm = cap;
+    cap = ws->cap_list;
+    CHK(ensure((void **)&ws->d_list, &cap, (list.size() + 1) * sizeof(uint64_t)));
+    ws->cap_list = cap;
+    HIP_OK(hipMemcpyAsync(ws->d_symoff, symoff.data(), (ncb + 1) * sizeof(uint64_t), hipMemcpyHostToDevice,
+                          ws->stream));
+    if (!list.empty()) {
+      HIP_OK(hipMemcpyAsync(ws->d_list, list.data(), list.size() * sizeof(uint64_t), hipMemcpyHostToDevice,
+                            ws->stream));
+      hipLaunchKernelGGL(t1_gen_kernel, dim3((uint32_t)list.size()), dim3(64), gen_lds, ws->stream, ws->d_list,
+                         ws->d_cbbox, ws->d_cbmeta, ws->d_K, ws->d_bpoff, ws->d_bp, ws->d_symoff, ws->d_sym,
+                         ws->d_pcnt, ws->d_pmse, ws->d_flags, erres, maxcol, obuf_cap);
+      HIP_OK(hipGetLastError());
+      constexpr int LPW = 16;
+      hipLaunchKernelGGL(t1_mq_kernel<LPW>, dim3((uint32_t)((list.size() + LPW - 1) / LPW)), dim3(64), 0, ws->stream,
+                         ws->d_list, (uint64_t)list.size(), ws->d_cbmeta, ws->d_cbsb, ws->d_K, ws->d_symoff, ws->d_sym,
+                         ws->d_pcnt, ws->d_pmse, ws->d_doff, d_data, ws->d_Kmsbs, ws->d_Z, ws->d_L, ws->d_S,
+                         ws->d_snap, ws->d_mse, ws->d_flags);
+      HIP_OK(hipGetLastError());
+    }
+  } else {
+    cap = ws->cap_q;
+    CHK(ensure((void **)&ws->d_q, &cap, qo * sizeof(uint64_t)));
+    ws->cap_q = cap;
+    cap = ws->cap_state;
+    CHK(ensure((void **)&ws->d_state, &cap, (qo / 4 + 1) * sizeof(uint16_t)));
+    ws->cap_state = cap;
+    hipLaunchKernelGGL(quantize_kernel, dim3((uint32_t)std::min<uint64_t>(ncb, 65535)), dim3(256), 0, ws->stream,
+                       work, W, H, D, ws->d_cbbox, ws->d_cbsb, ws->d_qoff, ws->d_q, ws->d_state, ws->d_K, ncb);
+    HIP_OK(hipGetLastError());
+    hipLaunchKernelGGL(t1_encode_kernel, dim3((uint32_t)((ncb + 63) / 64)), dim3(64), 0, ws->stream, ncb,
+                       ws->d_cbbox, ws->d_cbsb, ws->d_cbmeta, ws->d_qoff, ws->d_doff, ws->d_q, ws->d_state,
+                       d_data, ws->d_K, ws->d_Kmsbs, ws->d_Z, ws->d_L, ws->d_S, ws->d_snap, ws->d_mse,
+                       ws->d_flags, erres);
+    HIP_OK(hipGetLastError());
+  }
   HIP_OK(hipMemcpyAsync(rec.K.data(), ws->d_K, ncb, hipMemcpyDeviceToHost, ws->stream));
   HIP_OK(hipMemcpyAsync(rec.Kmsbs.data(), ws->d_Kmsbs, ncb, hipMemcpyDeviceToHost, ws->stream));
   HIP_OK(hipMemcpyAsync(rec.Z.data(), ws->d_Z, ncb, hipMemcpyDeviceToHost, ws->stream));
@@ -1109,9 +1268,16 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
   HIP_OK(hipMemcpyAsync(hfl, ws->d_flags, sizeof hfl, hipMemcpyDeviceToHost, ws->stream));
   HIP_OK(hipStreamSynchronize(ws->stream));
   if (hfl[0]) {
-    std::snprintf(g_err, sizeof g_err, "tier-1 output exceeded the per-codeblock buffer");
+    std::snprintf(g_err, sizeof g_err, "tier-1 output exceeded a per-codeblock buffer (flags %u)", hfl[0]);
     return 1;
   }
+  // codeblocks without significant bits are never launched on the fast path
+  for (uint64_t i = 0; i < ncb; ++i)
+    if (rec.K[i] == 0) {
+      rec.Z[i] = 0;
+      rec.Kmsbs[i] = 0;
+      rec.S[i * (kMaxPasses + 1)] = 0xFFFF;
+    }
   if (hfl[1] < *slope_min) *slope_min = (uint16_t)hfl[1];
   if (hfl[2] > *slope_max) *slope_max = (uint16_t)hfl[2];
   return 0;
@@ -1119,12 +1285,13 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
 
 int gather(Workspace *ws, CblkRecords &rec, const std::vector<uint32_t> &len) {
   const uint64_t ncb = rec.n;
-  std::vector<uint64_t> g(2 * ncb);
+  std::vector<uint64_t> g(3 * ncb);
   uint64_t tot = 0;
   for (uint64_t i = 0; i < ncb; ++i) {
     rec.data_off[i] = tot;
-    g[2 * i] = tot;
-    g[2 * i + 1] = len[i];
+    g[3 * i] = tot;
+    g[3 * i + 1] = len[i];
+    g[3 * i + 2] = rec.src_off[i];
     tot += len[i];
   }
   rec.data.assign(tot ? tot : 1, 0);
@@ -1132,9 +1299,12 @@ int gather(Workspace *ws, CblkRecords &rec, const std::vector<uint32_t> &len) {
   size_t cap = ws->cap_packed;
   CHK(ensure((void **)&ws->d_packed, &cap, tot));
   ws->cap_packed = cap;
-  HIP_OK(hipMemcpyAsync(ws->d_gather, g.data(), 2 * ncb * sizeof(uint64_t), hipMemcpyHostToDevice, ws->stream));
+  cap = ws->cap_gather;
+  CHK(ensure((void **)&ws->d_gather, &cap, 3 * ncb * sizeof(uint64_t)));
+  ws->cap_gather = cap;
+  HIP_OK(hipMemcpyAsync(ws->d_gather, g.data(), 3 * ncb * sizeof(uint64_t), hipMemcpyHostToDevice, ws->stream));
   hipLaunchKernelGGL(gather_kernel, dim3((uint32_t)std::min<uint64_t>(ncb, 65535)), dim3(256), 0, ws->stream,
-                     ws->d_data, ws->d_doff, ws->d_gather, ncb, ws->d_packed);
+                     ws->d_datap[rec.slot], ws->d_gather, ncb, ws->d_packed);
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpyAsync(rec.data.data(), ws->d_packed, tot, hipMemcpyDeviceToHost, ws->stream));
   HIP_OK(hipStreamSynchronize(ws->stream));

@@ -32,7 +32,9 @@ struct Workspace {
   uint64_t *d_doff = nullptr;     // offset of each codeblock's byte buffer
   uint64_t *d_q = nullptr;        // quantised magnitudes, 4 per stripe column
   uint16_t *d_state = nullptr;    // sigma | pi << 4 | delta << 8 | xi << 12
-  uint8_t *d_data = nullptr;      // MQ bytes
+  uint8_t *d_data = nullptr;      // unused; MQ bytes live in one slot per parameter:
+  uint8_t *d_datap[256] = {};     // slot p holds parameter p's codeblock byte buffers until
+  size_t cap_datap[256] = {};     // rate control has chosen what to gather
   uint8_t *d_K = nullptr, *d_Kmsbs = nullptr, *d_Z = nullptr;
   uint32_t *d_L = nullptr;
   uint16_t *d_S = nullptr;
@@ -41,8 +43,18 @@ struct Workspace {
   uint32_t *d_flags = nullptr;    // [0] overflow, [1] slope_min, [2] slope_max
   uint8_t *d_packed = nullptr;
   uint64_t *d_gather = nullptr;
-  size_t cap_cb = 0, cap_q = 0, cap_data = 0, cap_packed = 0, cap_gather = 0;
+  size_t cap_cb = 0, cap_q = 0, cap_data = 0, cap_packed = 0, cap_gather = 0, cap_state = 0;
+  // fast tier-1 path: bitplanes, symbol streams, per-pass counts/distortion
+  uint8_t *d_bp = nullptr;
+  uint64_t *d_bpoff = nullptr;
+  uint8_t *d_sym = nullptr;
+  uint64_t *d_symoff = nullptr;
+  uint32_t *d_pcnt = nullptr;
+  int64_t *d_pmse = nullptr;
+  uint64_t *d_list = nullptr;
+  size_t cap_bp = 0, cap_bpoff = 0, cap_sym = 0, cap_symoff = 0, cap_pc = 0, cap_pm = 0, cap_list = 0;
   uint64_t *h_pinned = nullptr;   // small pinned staging for flags / min-max
+  int last_dwt_launches = 0;      // kernel launches of the last DWT call
   ~Workspace();
 };
 
@@ -56,7 +68,7 @@ int dwt_forward(Workspace *ws, const bwc_codec *c, const Param &par, double *wor
 int dwt_inverse(Workspace *ws, const bwc_codec *c, const Param &par, double *work);
 // quantise + tier-1 for all codeblocks of (tile, param); results land in rec
 int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double *work, CblkRecords &rec,
-              uint16_t *slope_min, uint16_t *slope_max);
+              uint16_t *slope_min, uint16_t *slope_max, int slot);
 // gather each codeblock's prefix [0, len[i]) into rec.data / rec.data_off
 int gather(Workspace *ws, CblkRecords &rec, const std::vector<uint32_t> &len);
 int sync(Workspace *ws);

@@ -56,9 +56,12 @@ def test_dwt_bit_exact(case):
         o.close()
 
 
+@pytest.mark.parametrize("path", ["fast", "lane"])
 @pytest.mark.parametrize("case", STAGE, ids=[c["name"] for c in STAGE])
-def test_tier1_records_bit_exact(case):
-    _say("t1", case["name"])
+def test_tier1_records_bit_exact(case, path, monkeypatch):
+    # "fast": wavefront symbol generation + MQ kernel; "lane": one lane per codeblock
+    monkeypatch.setenv("BWC_AMD_T1", "1" if path == "lane" else "0")
+    _say("t1", path, case["name"])
     data = make_input(case)
     c = _codec(case)
     o = ol.Oracle(case["gen"]["shape"], npar=case["npar"], prec=case["prec"], rate=case["rate"], **codec_kw(case))
