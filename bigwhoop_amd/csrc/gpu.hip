@@ -1159,7 +1159,10 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
 
   // fast path: wavefront-per-codeblock symbol generation + few-lane MQ, when
   // every codeblock row fits one wavefront and its state fits LDS
-  uint32_t maxcol = 0, obuf_cap = 0, maxcw = 0;
+  // LDS: per codeblock the kernel packs P = 64 / pow2ceil(width) slices side
+  // by side, each with its own pass-output buffer, so size for the largest
+  // P times the largest per-slice bound
+  uint32_t maxcol = 0, obuf_cap = 0, maxcw = 0, P = 1;
   for (uint64_t i = 0; i < ncb; ++i) {
     const uint32_t *b = &par.cb_box[8 * i];
     const uint32_t cw = b[4] - b[0], ch = b[5] - b[1], ns = (ch + 3) / 4;
@@ -1167,10 +1170,11 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
     maxcol = std::max(maxcol, ncol);
     maxcw = std::max(maxcw, cw);
     obuf_cap = std::max(obuf_cap, cw * ch * 2 + cw * ns * 3 + 16);
+    uint32_t gw = 1;
+    while (gw < cw) gw <<= 1;
+    const uint32_t nsl = (b[6] - b[2]) * (b[7] - b[3]);
+    if (gw <= 64) P = std::max(P, std::min(64 / gw, nsl));
   }
-  uint32_t gw = 1;
-  while (gw < maxcw) gw <<= 1;
-  const uint32_t P = gw <= 64 ? 64 / gw : 1;
   const size_t gen_lds = (size_t)maxcol * 2 + (size_t)((maxcol + 1) / 2) * 8 + (size_t)P * obuf_cap;
   const char *force = std::getenv("BWC_AMD_T1");
   const bool v2 = maxcw <= 64 && gen_lds <= 64 * 1024 && !(force && force[0] == '1');

@@ -98,3 +98,23 @@ def test_codestream_matches_reference(case):
         assert c.compress(data) == cs
     finally:
         c.close()
+
+
+@pytest.mark.parametrize("n", [256])
+def test_headline_settings_match_oracle_at_scale(n):
+    """256^3 f64 k^-5/3 GRF at the headline settings (5 levels, cb 32x32x8, 1
+    layer at 2 bpv, and 4 layers 1,2,4,8): GPU codestream == CPU oracle's."""
+    from bigwhoop_amd.fields import grf
+
+    x = grf((n, n, n), seed=3)
+    for rate in ("2", "1,2,4,8"):
+        _say("scale", n, rate)
+        kw = dict(rate=rate, decomp=(5, 5, 5, 0), cb=(5, 5, 3, 0))
+        c = bwc.Codec((n, n, n), **kw)
+        try:
+            got = c.compress(x)
+        finally:
+            c.close()
+        ref = ol.oracle_compress(x, (n, n, n), **kw)
+        assert len(got) == len(ref)
+        assert got == ref
