@@ -467,6 +467,7 @@ __constant__ uint8_t c_sig[3][45];
 __constant__ uint8_t c_sig2xi[16];
 __constant__ uint8_t c_xi2cont[11][2];
 __constant__ uint16_t c_frac2log[256];
+__constant__ uint64_t c_sigpk[3][3];   // c_sig packed as 4-bit entries, 16 per word
 
 struct MqState {
   int64_t L;
@@ -1081,6 +1082,10 @@ static int upload_tables() {
   HIP_OK(hipMemcpyToSymbol(HIP_SYMBOL(c_sig2xi), kSig2xi, sizeof kSig2xi));
   HIP_OK(hipMemcpyToSymbol(HIP_SYMBOL(c_xi2cont), kXi2cont, sizeof kXi2cont));
   HIP_OK(hipMemcpyToSymbol(HIP_SYMBOL(c_frac2log), kFrac2log, sizeof kFrac2log));
+  uint64_t pk[3][3] = {};
+  for (int m = 0; m < 3; ++m)
+    for (int i = 0; i < 45; ++i) pk[m][i / 16] |= (uint64_t)kSig[m][i] << ((i % 16) * 4);
+  HIP_OK(hipMemcpyToSymbol(HIP_SYMBOL(c_sigpk), pk, sizeof pk));
   return 0;
 }
 
@@ -1238,11 +1243,25 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
                          ws->d_cbbox, ws->d_cbmeta, ws->d_K, ws->d_bpoff, ws->d_bp, ws->d_symoff, ws->d_sym,
                          ws->d_pcnt, ws->d_pmse, ws->d_flags, erres, maxcol, obuf_cap);
       HIP_OK(hipGetLastError());
-      constexpr int LPW = 16;
-      hipLaunchKernelGGL(t1_mq_kernel<LPW>, dim3((uint32_t)((list.size() + LPW - 1) / LPW)), dim3(64), 0, ws->stream,
-                         ws->d_list, (uint64_t)list.size(), ws->d_cbmeta, ws->d_cbsb, ws->d_K, ws->d_symoff, ws->d_sym,
-                         ws->d_pcnt, ws->d_pmse, ws->d_doff, d_data, ws->d_Kmsbs, ws->d_Z, ws->d_L, ws->d_S,
-                         ws->d_snap, ws->d_mse, ws->d_flags);
+      // codeblocks per wavefront of the MQ kernel (lanes beyond LPW idle):
+      // fewer lanes means less divergence between codeblocks of unequal
+      // length, more means fewer waves; BWC_AMD_MQ_LPW = 1, 2, 4, 8 or 16
+      const char *lpw_env = std::getenv("BWC_AMD_MQ_LPW");
+      const int lpw = lpw_env ? std::atoi(lpw_env) : 4;
+      const uint32_t nl = (uint32_t)list.size();
+#define MQ_LAUNCH(N)                                                                                              \
+  hipLaunchKernelGGL(t1_mq_kernel<N>, dim3((nl + N - 1) / N), dim3(64), 0, ws->stream, ws->d_list, (uint64_t)nl, \
+                     ws->d_cbmeta, ws->d_cbsb, ws->d_K, ws->d_symoff, ws->d_sym, ws->d_pcnt, ws->d_pmse,          \
+                     ws->d_doff, d_data, ws->d_Kmsbs, ws->d_Z, ws->d_L, ws->d_S, ws->d_snap, ws->d_mse,            \
+                     ws->d_flags)
+      switch (lpw) {
+        case 1: MQ_LAUNCH(1); break;
+        case 2: MQ_LAUNCH(2); break;
+        case 8: MQ_LAUNCH(8); break;
+        case 16: MQ_LAUNCH(16); break;
+        default: MQ_LAUNCH(4); break;
+      }
+#undef MQ_LAUNCH
       HIP_OK(hipGetLastError());
     }
   } else {
