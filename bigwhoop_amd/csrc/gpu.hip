@@ -1247,7 +1247,7 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
       // fewer lanes means less divergence between codeblocks of unequal
       // length, more means fewer waves; BWC_AMD_MQ_LPW = 1, 2, 4, 8 or 16
       const char *lpw_env = std::getenv("BWC_AMD_MQ_LPW");
-      const int lpw = lpw_env ? std::atoi(lpw_env) : 4;
+      const int lpw = lpw_env ? std::atoi(lpw_env) : 16;
       const uint32_t nl = (uint32_t)list.size();
 #define MQ_LAUNCH(N)                                                                                              \
   hipLaunchKernelGGL(t1_mq_kernel<N>, dim3((nl + N - 1) / N), dim3(64), 0, ws->stream, ws->d_list, (uint64_t)nl, \
@@ -1257,9 +1257,9 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
       switch (lpw) {
         case 1: MQ_LAUNCH(1); break;
         case 2: MQ_LAUNCH(2); break;
+        case 4: MQ_LAUNCH(4); break;
         case 8: MQ_LAUNCH(8); break;
-        case 16: MQ_LAUNCH(16); break;
-        default: MQ_LAUNCH(4); break;
+        default: MQ_LAUNCH(16); break;
       }
 #undef MQ_LAUNCH
       HIP_OK(hipGetLastError());

@@ -17,44 +17,90 @@ namespace {
 
 // MSB-first bit writer with JPEG 2000 style stuffing: after an emitted 0xFF
 // byte the next byte carries only 7 bits.
-struct BitWriter {
-  std::vector<uint8_t> out;
+//
+// put(v, n) appends the n low bits of v, most significant first; it handles
+// a byte at a time, which gives the same bytes as n calls of bit().
+template <class Sink>
+struct StuffedBits {
+  Sink sink;
   uint8_t T = 0;
   int t = 8;
+  void emit() {
+    sink(T);
+    t = (T == 0xFF) ? 7 : 8;
+    T = 0;
+  }
   void bit(uint64_t b) {
     --t;
     if (b) T |= (uint8_t)(1u << t);
-    if (!t) {
-      out.push_back(T);
-      t = (T == 0xFF) ? 7 : 8;
-      T = 0;
+    if (!t) emit();
+  }
+  void put(uint64_t v, int n) {
+    while (n > 0) {
+      const int k = n < t ? n : t;
+      const uint32_t chunk = (uint32_t)(v >> (n - k)) & ((1u << k) - 1u);
+      T |= (uint8_t)(chunk << (t - k));
+      t -= k;
+      n -= k;
+      if (!t) emit();
+    }
+  }
+  void ones(int n) {
+    while (n > 0) {
+      const int k = n < 32 ? n : 32;
+      put(0xFFFFFFFFull, k);
+      n -= k;
+    }
+  }
+  void zeros(int n) {
+    while (n > 0) {
+      const int k = n < 32 ? n : 32;
+      put(0, k);
+      n -= k;
     }
   }
   void flush() {
-    if (t != 8) out.push_back(T);
+    if (t != 8) sink(T);
   }
 };
 
+struct ByteSink {
+  std::vector<uint8_t> out;
+  void operator()(uint8_t b) { out.push_back(b); }
+};
+struct CountSink {
+  uint64_t n = 0;
+  void operator()(uint8_t) { ++n; }
+};
+
+using BitWriter = StuffedBits<ByteSink>;
 // Counting writer for the estimation modes: same bit-stuffing arithmetic,
 // no buffer.
-struct BitCounter {
-  uint64_t n = 0;
-  uint8_t T = 0;
-  int t = 8;
-  void bit(uint64_t b) {
-    --t;
-    if (b) T |= (uint8_t)(1u << t);
-    if (!t) {
-      ++n;
-      t = (T == 0xFF) ? 7 : 8;
-      T = 0;
+using BitCounter = StuffedBits<CountSink>;
+
+// Raw (unstuffed) bit string, used to cache the tag-tree bits of one
+// codeblock while the tree states stay fixed during a layer's probes.
+struct BitString {
+  std::vector<uint64_t> *words;
+  uint64_t nbits = 0;
+  void bit(uint64_t b) { put(b ? 1 : 0, 1); }
+  void put(uint64_t v, int n) {
+    for (int i = n; i-- > 0;) {
+      const uint64_t w = nbits >> 6;
+      if (w >= words->size()) words->push_back(0);
+      if ((v >> i) & 1) (*words)[w] |= 1ull << (63 - (nbits & 63));
+      ++nbits;
     }
   }
-  void flush() {
-    if (t != 8) ++n;
-  }
+  void ones(int n) { for (int i = 0; i < n; ++i) put(1, 1); }
+  void zeros(int n) { for (int i = 0; i < n; ++i) put(0, 1); }
 };
 
+// Tag-tree coding of one leaf up to `threshold` (tagtree.c:310-373). At each
+// node from the root down the reference emits one bit per threshold step: a
+// 0 while the node value is still >= the raised threshold, then a 1 once it
+// passes the value (if that happens below `threshold`). That is a run of
+// min(v, threshold) - th zeros followed by a 1 when v < threshold.
 template <class W>
 void tagtree_encode(TagTree &tt, W &w, uint32_t threshold, uint64_t leaf, int est) {
   int64_t branch[64];
@@ -69,9 +115,11 @@ void tagtree_encode(TagTree &tt, W &w, uint32_t threshold, uint64_t leaf, int es
     uint32_t th = tt.threshold[n];
     if (th < tmin) th = tmin;
     const uint16_t v = tt.value[n];
-    while (v >= th && th < threshold) {
-      ++th;
-      w.bit(v >= th ? 0 : 1);
+    if (v >= th && th < threshold) {
+      const uint32_t hi = std::min<uint32_t>(v, threshold);
+      w.zeros((int)(hi - th));
+      if (v < threshold) w.put(1, 1);
+      th = std::min<uint32_t>((uint32_t)v + 1, threshold);
     }
     tmin = std::min<uint32_t>(v, th);
     if (!est) tt.threshold[n] = (uint16_t)th;
@@ -80,25 +128,26 @@ void tagtree_encode(TagTree &tt, W &w, uint32_t threshold, uint64_t leaf, int es
   }
 }
 
+// Δz code of the number of new passes (tier2.c:100-153); nothing is written
+// for 293 or more, as in the reference.
 template <class W>
 void delta_z_code(W &w, int dz) {
   if (dz == 1) {
-    w.bit(0);
+    w.put(0, 1);
   } else if (dz == 2) {
-    w.bit(1), w.bit(0);
+    w.put(2, 2);
   } else if (dz < 6) {
-    dz -= 3;
-    w.bit(1), w.bit(1), w.bit(dz & 2), w.bit(dz & 1);
+    w.put(0xCu | (uint32_t)(dz - 3), 4);
   } else if (dz < 37) {
-    dz -= 6;
-    for (int i = 0; i < 4; ++i) w.bit(1);
-    for (int m = 0x10; m; m >>= 1) w.bit(dz & m);
+    w.put((0xFu << 5) | (uint32_t)(dz - 6), 9);
   } else if (dz < 293) {
-    dz -= 37;
-    for (int i = 0; i < 9; ++i) w.bit(1);
-    for (int m = 0x80; m; m >>= 1) w.bit(dz & m);
+    w.put((0x1FFu << 8) | (uint32_t)(dz - 37), 17);
   }
 }
+
+// floor(log2(x)) of a positive integer as the reference computes it through
+// the double log2 (exact for 32-bit values); x == 0 keeps the library call
+inline uint16_t ilog2_u32(uint32_t x) { return x ? (uint16_t)(31 - __builtin_clz(x)) : (uint16_t)std::log2(x); }
 
 struct T2 {
   bwc_codec *c;
@@ -116,15 +165,17 @@ struct T2 {
     uint16_t maxb = beta;
     const uint32_t *L = &R.L[i * kMaxPasses];
     const uint32_t len = L[zc] - ((zp >= 0) ? L[zp] : 0);
-    const uint16_t cpb = (uint16_t)std::log2(zc - zp);
-    int32_t lb = (uint16_t)std::log2(len) + 1 - cpb;
+    const uint16_t cpb = ilog2_u32((uint32_t)(zc - zp));
+    int32_t lb = ilog2_u32(len) + 1 - cpb;
     if (maxb < lb) maxb = (uint16_t)lb;
-    while (beta < maxb) {
-      w.bit(1);
-      ++beta;
-    }
-    w.bit(0);
-    for (int32_t m = 1 << (beta + cpb - 1); m > 0; m >>= 1) w.bit(len & (uint32_t)m);
+    // β increments as a run of ones, a zero, then the length in β + ⌊log2 Δz⌋ bits
+    w.ones(maxb - beta);
+    beta = std::max(beta, maxb);
+    w.put(0, 1);
+    // (a shift of 31 or more turns the reference's int mask negative and it
+    // writes no length bits; kept as is)
+    const int nb = beta + cpb;
+    if (nb > 0 && nb < 32) w.put(len & ((1u << nb) - 1u), nb);
     if (est == 0) {
       R.beta[i] = beta;
       R.beta_est[i] = beta;
@@ -145,9 +196,14 @@ struct T2 {
         int16_t *cp = &R.cp[i * (nl + 1) + 1];
         const int dz = cp[q] - cp[q - 1];
         if (cp[q - 1] == 0) {
-          tagtree_encode(pr.incl, w, (uint32_t)q + 1, k, est);
-          if (cp[q])
-            for (uint32_t t = 1; t <= R.Kmsbs[i] + 1u; ++t) tagtree_encode(pr.msbs, w, t, k, est);
+          if (est == 2 && lc) {
+            lc->incl[j].feed(w, i);
+            if (cp[q]) lc->msbs[j].feed(w, i);
+          } else {
+            tagtree_encode(pr.incl, w, (uint32_t)q + 1, k, est);
+            if (cp[q])
+              for (uint32_t t = 1; t <= R.Kmsbs[i] + 1u; ++t) tagtree_encode(pr.msbs, w, t, k, est);
+          }
         } else {
           w.bit((uint64_t)dz);
         }
@@ -178,7 +234,7 @@ struct T2 {
     packet_header(w, j, res, p, q, est, est == 1 ? &body : nullptr);
     if (est == 1) pk.body_size += body;
     if (est == 2) pk.e = 0;
-    return (int64_t)w.n;
+    return (int64_t)w.sink.n;
   }
 
   void real_packet(int j, Resolution &res, uint64_t p, int q) {
@@ -191,8 +247,120 @@ struct T2 {
     }
     BitWriter w;
     packet_header(w, j, res, p, q, 0, nullptr);
-    pk.header = std::move(w.out);
+    pk.header = std::move(w.sink.out);
     pk.size = (uint32_t)(pk.body_size + pk.header.size());
+  }
+
+  // Per parameter: the passes with a nonzero slope (the convex-hull points)
+  // in pass order. Hull slopes never increase along the hull, so the cut for
+  // a threshold T >= 1, max{z : S[z] >= T}, is found by bisection over these
+  // points instead of a scan over all passes. A codeblock whose nonzero
+  // slopes are not ordered that way keeps the scan (flag `scan`).
+  struct HullIndex {
+    std::vector<uint32_t> off;
+    std::vector<uint8_t> z;
+    std::vector<uint16_t> s;
+    std::vector<uint8_t> scan;
+  };
+  std::vector<HullIndex> hull;
+
+  void build_hull_index() {
+    hull.assign(c->info.nPar, HullIndex{});
+    for (int j = 0; j < c->info.nPar; ++j) {
+      CblkRecords &R = rec(j);
+      HullIndex &H = hull[j];
+      H.off.assign(R.n + 1, 0);
+      H.scan.assign(R.n, 0);
+      for (uint64_t i = 0; i < R.n; ++i) {
+        H.off[i] = (uint32_t)H.z.size();
+        const uint16_t *S = &R.S[i * (kMaxPasses + 1)];
+        uint16_t prev = 0xFFFF;
+        for (int z = 0; z <= R.Z[i]; ++z) {
+          if (!S[z]) continue;
+          if (S[z] > prev) H.scan[i] = 1;
+          prev = S[z];
+          H.z.push_back((uint8_t)z);
+          H.s.push_back(S[z]);
+        }
+        if (S[0] == 0) H.scan[i] = 1;
+      }
+      H.off[R.n] = (uint32_t)H.z.size();
+    }
+  }
+
+  int16_t cut_of(int j, const CblkRecords &R, uint64_t i, uint16_t threshold) const {
+    if (threshold >= 1 && !hull.empty() && !hull[j].scan[i]) {
+      const HullIndex &H = hull[j];
+      uint32_t lo = H.off[i], hi = H.off[i + 1];   // S[z[lo]] = S[0] >= threshold
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (H.s[mid] >= threshold) lo = mid;
+        else hi = mid;
+      }
+      return H.z[lo];
+    }
+    const uint16_t *S = &R.S[i * (kMaxPasses + 1)];
+    int16_t z;
+    for (z = R.Z[i]; z >= 0 && S[z] < threshold; --z) {
+    }
+    return z;
+  }
+
+  // Tag-tree bits of every not-yet-included codeblock for the probes of one
+  // layer: inclusion to threshold q + 1 and the MSB sequence. Probes do not
+  // change tree states, so these bits are the same for every probe.
+  struct BitCache {
+    std::vector<uint64_t> words;
+    std::vector<uint32_t> woff, nbits;
+    template <class W>
+    void feed(W &w, uint64_t i) const {
+      uint32_t n = nbits[i];
+      const uint64_t *p = &words[woff[i]];
+      while (n >= 64) {
+        w.put(*p >> 32, 32);
+        w.put(*p & 0xFFFFFFFFull, 32);
+        ++p;
+        n -= 64;
+      }
+      if (n) w.put(*p >> (64 - n), (int)n);
+    }
+  };
+  struct LayerCache {
+    std::vector<BitCache> incl, msbs;
+  };
+  LayerCache layer_cache;
+  const LayerCache *lc = nullptr;
+
+  void build_layer_cache(int q) {
+    layer_cache.incl.assign(c->info.nPar, BitCache{});
+    layer_cache.msbs.assign(c->info.nPar, BitCache{});
+    for (int j = 0; j < c->info.nPar; ++j) {
+      Param &par = tl->par[j];
+      CblkRecords &R = rec(j);
+      BitCache &BI = layer_cache.incl[j], &BM = layer_cache.msbs[j];
+      BI.woff.assign(R.n, 0), BI.nbits.assign(R.n, 0);
+      BM.woff.assign(R.n, 0), BM.nbits.assign(R.n, 0);
+      for (auto &res : par.res)
+        for (auto &sb : res.sb)
+          for (uint64_t p = 0; p < res.nprec; ++p) {
+            Precinct &pr = sb.prec[p];
+            for (uint64_t k = 0; k < pr.ncbs; ++k) {
+              const uint64_t i = pr.first_cb + k;
+              if (R.cp[i * (nl + 1) + q] != 0) continue;   // cp[q - 1]
+              BI.woff[i] = (uint32_t)BI.words.size();
+              BitString bi{&BI.words};
+              tagtree_encode(pr.incl, bi, (uint32_t)q + 1, k, 2);
+              BI.nbits[i] = (uint32_t)bi.nbits;
+              if (bi.nbits == 0) BI.words.push_back(0);
+              BM.woff[i] = (uint32_t)BM.words.size();
+              BitString bm{&BM.words};
+              for (uint32_t t = 1; t <= R.Kmsbs[i] + 1u; ++t) tagtree_encode(pr.msbs, bm, t, k, 2);
+              BM.nbits[i] = (uint32_t)bm.nbits;
+              if (bm.nbits == 0) BM.words.push_back(0);
+            }
+          }
+    }
+    lc = &layer_cache;
   }
 
   int64_t quality_layer(uint16_t threshold, int q, int est) {
@@ -207,11 +375,8 @@ struct T2 {
             for (uint64_t k = 0; k < pr.ncbs; ++k) {
               const uint64_t i = pr.first_cb + k;
               int16_t *cp = &R.cp[i * (nl + 1) + 1];
-              const uint16_t *S = &R.S[i * (kMaxPasses + 1)];
               const uint32_t *L = &R.L[i * kMaxPasses];
-              int16_t z;
-              for (z = R.Z[i]; z >= 0 && S[z] < threshold; --z) {
-              }
+              const int16_t z = cut_of(j, R, i, threshold);
               if (z < cp[q - 1]) {
                 cp[q] = cp[q - 1];
                 if (cp[q] == 1) ql += L[0];
@@ -243,10 +408,12 @@ struct T2 {
     int64_t est_tile = 0, est_ql = 0;
     const uint64_t tsz = tl->box.volume() * (uint64_t)c->info.nPar;
     const uint64_t hdr = (uint64_t)std::ceil((float)((uint32_t)c->control.headerSize + tl->header_size) / nl);
+    build_hull_index();
     for (int l = 0; l < nl; ++l) {
       const int64_t target = (int64_t)std::ceil(c->control.bitrate[l] * tsz / 8.0f);
       uint16_t slope;
       if (target > 0.0f) {
+        build_layer_cache(l);
         uint16_t smin = tl->slope_min, smax = tl->slope_max, tmp = tl->slope_min;
         slope = smin;
         while (smin < smax) {
@@ -267,6 +434,7 @@ struct T2 {
       } else {
         slope = 0;
       }
+      lc = nullptr;
       est_ql = quality_layer(slope, l, 0);
       est_tile += est_ql + (int64_t)hdr;
     }
