@@ -42,7 +42,7 @@ const char *last_error() { return g_err; }
 Workspace::~Workspace() {
   if (device < 0) return;
   (void)hipSetDevice(device);
-  void *ptrs[] = {d_input, d_work,  d_red,  d_norm, d_cbbox, d_cbsb, d_cbmeta, d_qoff,   d_doff, d_q,
+  void *ptrs[] = {d_input, d_work,  d_work2, d_red,  d_norm, d_cbbox, d_cbsb, d_cbmeta, d_qoff,   d_doff, d_q,
                   d_state, d_data,  d_K,    d_Kmsbs, d_Z,    d_L,    d_S,      d_snap,   d_mse,  d_flags,
                   d_packed, d_gather, d_bp, d_bpoff, d_sym, d_symoff, d_pcnt, d_pmse, d_list};
   for (void *p : ptrs)
@@ -332,6 +332,157 @@ __global__ void dwt_axis_kernel(double *__restrict__ data, int64_t len, int par0
   }
 }
 
+// ---------------------------------------------------------------------------
+// Forward CDF 9/7, fast kernels. Both compute, for every line, exactly the
+// reference's lifting steps over its extended ranges (dwt.c:354-404) on the
+// whole-point symmetric extension (dwt.c:286-342), so each coefficient is the
+// same sequence of IEEE binary64 operations as in the reference; only the
+// order in which independent positions are visited differs.
+//
+// Working index i addresses line position p = i + par0. Step s is applied at
+// i = lo_s, lo_s + 2, ... (cnt_s positions): lo = {-3, -2, -1, 0},
+// cnt = {(len+7)/2, (len+5)/2, (len+3)/2, (len+1)/2}.
+//   st0 (odd i):  a_i = x_i + A (x_{i-1} + x_{i+1})
+//   st1 (even i): b_i = a_i + B (a_{i-1} + a_{i+1})
+//   st2 (odd i):  c_i = KH (b_i + G (b_{i-1} + b_{i+1}))
+//   st3 (even i): d_i = KL (c_i + D (c_{i-1} + c_{i+1}))
+// Lowpass o < half takes i = 2o, highpass o >= half takes the odd
+// i = 2(o - half) + 1 - 2 par0 (buffer_flush_forward, dwt.c:645-721).
+// ---------------------------------------------------------------------------
+struct Lift97 {
+  static constexpr double A = -1.586134342059924f, B = -0.052980118572961f, G = 0.882911075530934f,
+                          D = 0.360523644801462f, KH = 1.230174104914001f, KL = 0.812893066115961f;
+};
+
+__device__ __forceinline__ int mirror32(int p, int len) {
+  if (len == 1) return 0;
+  while (p < 0 || p > len - 1) p = (p < 0) ? -p : 2 * (len - 1) - p;
+  return p;
+}
+
+// X axis (contiguous lines): one wavefront per line. The extended line is
+// split into even and odd working positions (Ev, Od) in LDS, so every step
+// reads and writes consecutive LDS words and the output is two contiguous
+// runs (lowpass = Ev, highpass = Od). src and dst may be the same buffer.
+// Ev[j] holds i = 2j - 8, Od[j] holds i = 2j - 7.
+__global__ void __launch_bounds__(256) dwt97_x_kernel(const double *src, double *dst, int len, int par0, int half,
+                                                      uint32_t n0, uint64_t s0, uint32_t n1, uint64_t s1,
+                                                      uint64_t s2) {
+  extern __shared__ double lds[];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int hp = (len + 16) / 2 + 1;
+  double *Ev = lds + (size_t)wv * 2 * hp, *Od = Ev + hp;
+  const uint32_t i0 = blockIdx.x * 4 + wv;
+  const uint32_t i1 = blockIdx.y % n1, i2 = blockIdx.y / n1;
+  const bool act = i0 < n0;
+  const uint64_t base = (uint64_t)i0 * s0 + (uint64_t)i1 * s1 + (uint64_t)i2 * s2;
+  if (act) {
+    const double *s = src + base;
+    for (int e = lane; e < len + 16; e += 64) {
+      const int i = e - 8, p = i + par0;
+      const double v = s[((unsigned)p < (unsigned)len) ? p : mirror32(p, len)];
+      if (e & 1) Od[e >> 1] = v;
+      else Ev[e >> 1] = v;
+    }
+  }
+  __syncthreads();
+  if (act)
+    for (int m = lane; m < (len + 7) / 2; m += 64) Od[2 + m] += Lift97::A * (Ev[2 + m] + Ev[3 + m]);
+  __syncthreads();
+  if (act)
+    for (int m = lane; m < (len + 5) / 2; m += 64) Ev[3 + m] += Lift97::B * (Od[2 + m] + Od[3 + m]);
+  __syncthreads();
+  if (act)
+    for (int m = lane; m < (len + 3) / 2; m += 64) {
+      const double t = Od[3 + m] + Lift97::G * (Ev[3 + m] + Ev[4 + m]);
+      Od[3 + m] = Lift97::KH * t;
+    }
+  __syncthreads();
+  if (act)
+    for (int m = lane; m < (len + 1) / 2; m += 64) {
+      const double t = Ev[4 + m] + Lift97::D * (Od[3 + m] + Od[4 + m]);
+      Ev[4 + m] = Lift97::KL * t;
+    }
+  __syncthreads();
+  if (act) {
+    double *d = dst + base;
+    for (int o = lane; o < len; o += 64) d[o] = (o < half) ? Ev[o + 4] : Od[o - half + 4 - par0];
+  }
+}
+
+// Strided axes: one thread per line, consecutive lanes on consecutive lines
+// of the fastest other axis, so every load and store of a wavefront is one
+// contiguous row segment. The four steps run as a pipeline along the line:
+// after reading x_{2k} and x_{2k+1} the thread finishes a_{2k-1}, b_{2k-2},
+// c_{2k-3} and d_{2k-4}. Highpass outputs land ahead of the read position,
+// so the pass is out of place (src != dst).
+template <int U>
+__global__ void __launch_bounds__(256) dwt97_stream_kernel(const double *__restrict__ src, double *__restrict__ dst,
+                                                           int len, int par0, int half, uint64_t sa, uint32_t n0,
+                                                           uint64_t s0, uint32_t n1, uint64_t s1, uint64_t s2,
+                                                           uint32_t nlines) {
+  const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= nlines) return;
+  const uint32_t i0 = l % n0, r = l / n0;
+  const uint32_t i1 = r % n1, i2 = r / n1;
+  const uint64_t base = (uint64_t)i0 * s0 + (uint64_t)i1 * s1 + (uint64_t)i2 * s2;
+  const double *s = src + base;
+  double *d = dst + base;
+  const int last0 = -3 + 2 * ((len + 7) / 2 - 1), last1 = -2 + 2 * ((len + 5) / 2 - 1);
+  const int last2 = -1 + 2 * ((len + 3) / 2 - 1), last3 = 2 * ((len + 1) / 2 - 1);
+  const int ilo = -par0, ihi = len - par0;   // output working indices [ilo, ihi)
+  auto ld = [&](int i) -> double {
+    const int p = i + par0;
+    return s[(int64_t)(((unsigned)p < (unsigned)len) ? p : mirror32(p, len)) * (int64_t)sa];
+  };
+  // carried: x_{2k-2}, x_{2k-1}, a_{2k-3}, b_{2k-4}, c_{2k-5}; positions below
+  // -4 are never combined into an output, so they start as zero
+  double xm2 = ld(-6), xm1 = ld(-5), am3 = 0, bm4 = 0, cm5 = 0;
+  for (int k = -2; 2 * k - 4 < ihi; k += U) {
+    double xv[2 * U];
+#pragma unroll
+    for (int u = 0; u < 2 * U; ++u) xv[u] = ld(2 * k + u);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int kk = k + u;
+      if (2 * kk - 4 >= ihi) break;
+      const double x0 = xv[2 * u], x1 = xv[2 * u + 1];
+      int i = 2 * kk - 1;
+      const double a1 = (i >= -3 && i <= last0) ? xm1 + Lift97::A * (xm2 + x0) : xm1;
+      i = 2 * kk - 2;
+      const double b2 = (i >= -2 && i <= last1) ? xm2 + Lift97::B * (am3 + a1) : xm2;
+      i = 2 * kk - 3;
+      const double c3 = (i >= -1 && i <= last2) ? Lift97::KH * (am3 + Lift97::G * (bm4 + b2)) : am3;
+      i = 2 * kk - 4;
+      const double d4 = (i >= 0 && i <= last3) ? Lift97::KL * (bm4 + Lift97::D * (cm5 + c3)) : bm4;
+      if (i >= ilo && i < ihi) d[(int64_t)(i >> 1) * (int64_t)sa] = d4;
+      i = 2 * kk - 3;
+      if (i >= ilo && i < ihi) d[(int64_t)(half + ((i + 2 * par0 - 1) >> 1)) * (int64_t)sa] = c3;
+      xm2 = x0;
+      xm1 = x1;
+      am3 = a1;
+      bm4 = b2;
+      cm5 = c3;
+    }
+  }
+}
+
+// copy the box [0,n0) x [0,n1) x [0,n2) x [0,n3) (strides 1, s1, s2, s3)
+__global__ void box_copy_kernel(const double *__restrict__ src, double *__restrict__ dst, uint32_t n0, uint32_t n1,
+                                uint32_t n2, uint64_t s1, uint64_t s2, uint64_t s3, uint64_t total) {
+  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
+       e += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t r = e;
+    const uint64_t x = r % n0;
+    r /= n0;
+    const uint64_t y = r % n1;
+    r /= n1;
+    const uint64_t z = r % n2, t = r / n2;
+    const uint64_t o = x + y * s1 + z * s2 + t * s3;
+    dst[o] = src[o];
+  }
+}
+
 static int dwt_run(Workspace *ws, const bwc_codec *c, const Param &par, double *work, bool fwd) {
   const bwc_gl_ctrl &ctl = c->control;
   const uint64_t W = par.box.len(0), H = par.box.len(1), D = par.box.len(2);
@@ -340,14 +491,34 @@ static int dwt_run(Workspace *ws, const bwc_codec *c, const Param &par, double *
   const int kern[4] = {ctl.KernelX, ctl.KernelY, ctl.KernelZ, ctl.KernelTS};
   const int nd = ctl.nDecomp;
   ws->last_dwt_launches = 0;
+  // BWC_AMD_DWT=0 keeps every pass on the generic in-place kernel
+  const char *dwt_env = std::getenv("BWC_AMD_DWT");
+  const bool generic_only = dwt_env && dwt_env[0] == '0';
   for (int step = 0; step < nd; ++step) {
     const int level = fwd ? step : nd - 1 - step;
     const Box &rb = par.res[nd - level].box;
     uint64_t sz[4];
     for (int a = 0; a < 4; ++a) sz[a] = rb.len(a);
+    auto active = [&](int a) { return level < dec[a] && sz[a] > 1; };
+    auto fast97 = [&](int a) {
+      if (!fwd || generic_only || kern[a] != bwc_dwt_9_7 || !active(a)) return false;
+      if (a == 0) return (size_t)4 * 2 * ((sz[0] + 16) / 2 + 1) * sizeof(double) <= 160 * 1024;
+      return true;
+    };
+    // every strided fast pass moves the box to the other buffer; an odd count
+    // is evened out by running the X pass out of place, else by a copy back
+    const int nstrided = (int)fast97(1) + (int)fast97(2) + (int)fast97(3);
+    const bool x_oop = (nstrided & 1) && fast97(0);
+    double *cur = work, *oth = ws->d_work2;
+    if (nstrided) {
+      size_t cap = ws->work2_cap;
+      CHK(ensure((void **)&ws->d_work2, &cap, par.box.volume() * sizeof(double)));
+      ws->work2_cap = cap;
+      oth = ws->d_work2;
+    }
     for (int oi = 0; oi < 4; ++oi) {
       const int a = fwd ? oi : 3 - oi;
-      if (!(level < dec[a] && sz[a] > 1)) continue;
+      if (!active(a)) continue;
       if (kern[a] == bwc_dwt_haar) {
         std::snprintf(g_err, sizeof g_err, "Haar kernel has no boundary extension in the codec");
         return 1;
@@ -367,15 +538,41 @@ static int dwt_run(Workspace *ws, const bwc_codec *c, const Param &par, double *
       while (G > 1 && (size_t)G * pitch * sizeof(double) > 96 * 1024) --G;
       if ((uint64_t)G > sz[o[0]]) G = (int)sz[o[0]];
       const size_t lds = (size_t)G * (len + 2 * DWT_E + 1) * sizeof(double);
+      const uint64_t gx = (sz[o[0]] + G - 1) / G;
+      const uint64_t gy = sz[o[1]] * sz[o[2]];
+      const int threads = 256;
+      // forward 9/7 takes the fast kernels: LDS wavefront-per-line along X
+      // (in place, or out of place to keep the level's buffer parity), a
+      // streaming thread-per-line pass from the current buffer into the
+      // other one along the strided axes
+      const int half = par0 ? (int)(len / 2) : (int)((len + 1) / 2);
+      const size_t xlds = (size_t)4 * 2 * ((len + 16) / 2 + 1) * sizeof(double);
+      if (fast97(a) && a == 0) {
+        double *dst = x_oop ? oth : cur;
+        hipLaunchKernelGGL(dwt97_x_kernel, dim3((uint32_t)((sz[o[0]] + 3) / 4), (uint32_t)gy), dim3(256), xlds,
+                           ws->stream, cur, dst, (int)len, par0, half, (uint32_t)sz[o[0]], inc[o[0]],
+                           (uint32_t)sz[o[1]], inc[o[1]], inc[o[2]]);
+        HIP_OK(hipGetLastError());
+        if (x_oop) std::swap(cur, oth);
+        ++ws->last_dwt_launches;
+        continue;
+      }
+      if (fast97(a) && a != 0) {
+        const uint64_t nlines = sz[o[0]] * sz[o[1]] * sz[o[2]];
+        hipLaunchKernelGGL(dwt97_stream_kernel<4>, dim3((uint32_t)((nlines + 255) / 256)), dim3(256), 0, ws->stream,
+                           cur, oth, (int)len, par0, half, inc[a], (uint32_t)sz[o[0]], inc[o[0]], (uint32_t)sz[o[1]],
+                           inc[o[1]], inc[o[2]], (uint32_t)nlines);
+        HIP_OK(hipGetLastError());
+        std::swap(cur, oth);
+        ++ws->last_dwt_launches;
+        continue;
+      }
       if (lds > 160 * 1024) {
         std::snprintf(g_err, sizeof g_err, "DWT line of %lld samples exceeds the LDS line buffer", (long long)len);
         return 1;
       }
-      const uint64_t gx = (sz[o[0]] + G - 1) / G;
-      const uint64_t gy = sz[o[1]] * sz[o[2]];
-      const int threads = 256;
       auto launch = [&](auto kfn) {
-        hipLaunchKernelGGL(kfn, dim3((uint32_t)gx, (uint32_t)gy), dim3(threads), lds, ws->stream, work, len, par0,
+        hipLaunchKernelGGL(kfn, dim3((uint32_t)gx, (uint32_t)gy), dim3(threads), lds, ws->stream, cur, len, par0,
                            inc[a], sz[o[0]], inc[o[0]], sz[o[1]], inc[o[1]], inc[o[2]], G, contiguous ? 1 : 0);
       };
       if (kern[a] == bwc_dwt_9_7) {
@@ -385,6 +582,14 @@ static int dwt_run(Workspace *ws, const bwc_codec *c, const Param &par, double *
         if (fwd) launch(dwt_axis_kernel<1, true>);
         else launch(dwt_axis_kernel<1, false>);
       }
+      HIP_OK(hipGetLastError());
+      ++ws->last_dwt_launches;
+    }
+    if (cur != work) {
+      const uint64_t total = sz[0] * sz[1] * sz[2] * sz[3];
+      hipLaunchKernelGGL(box_copy_kernel, dim3((uint32_t)std::min<uint64_t>((total + 255) / 256, 16384)), dim3(256),
+                         0, ws->stream, cur, work, (uint32_t)sz[0], (uint32_t)sz[1], (uint32_t)sz[2], inc[1], inc[2],
+                         inc[3], total);
       HIP_OK(hipGetLastError());
       ++ws->last_dwt_launches;
     }
@@ -1359,6 +1564,7 @@ int init_device(Workspace *ws) {
     HIP_OK(hipFuncSetAttribute((const void *)dwt_axis_kernel<0, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
     HIP_OK(hipFuncSetAttribute((const void *)dwt_axis_kernel<1, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
     HIP_OK(hipFuncSetAttribute((const void *)dwt_axis_kernel<1, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
+    HIP_OK(hipFuncSetAttribute((const void *)dwt97_x_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
     tables = true;
   }
   return 0;

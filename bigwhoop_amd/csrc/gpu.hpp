@@ -21,6 +21,8 @@ struct Workspace {
   size_t input_cap = 0;
   double *d_work = nullptr;
   size_t work_cap = 0;
+  double *d_work2 = nullptr;      // second buffer for out-of-place DWT passes
+  size_t work2_cap = 0;
   // reductions
   double *d_red = nullptr;        // partial min/max
   double *d_norm = nullptr;       // {min, max, alpha, beta}

@@ -32,9 +32,12 @@ def _codec(case):
     return bwc.Codec(case["gen"]["shape"], npar=case["npar"], prec=case["prec"], rate=case["rate"], **codec_kw(case))
 
 
+@pytest.mark.parametrize("path", ["fast", "generic"])
 @pytest.mark.parametrize("case", STAGE, ids=[c["name"] for c in STAGE])
-def test_dwt_bit_exact(case):
-    _say("dwt", case["name"])
+def test_dwt_bit_exact(case, path, monkeypatch):
+    # "fast": LDS X pass + streaming strided passes; "generic": in-place LDS kernel only
+    monkeypatch.setenv("BWC_AMD_DWT", "1" if path == "fast" else "0")
+    _say("dwt", path, case["name"])
     data = make_input(case)
     c = _codec(case)
     o = ol.Oracle(case["gen"]["shape"], npar=case["npar"], prec=case["prec"], rate=case["rate"], **codec_kw(case))
@@ -51,6 +54,43 @@ def test_dwt_bit_exact(case):
                 allc.append(g)
         dw = load(case, "dwt")
         assert hashlib.sha256(np.concatenate(allc).tobytes()).digest() == dw["sha256"].tobytes()
+    finally:
+        c.close()
+        o.close()
+
+
+# line lengths from 2 upwards, odd and even, tile offsets that make the
+# lowpass start on odd positions, 1-4 transformed axes
+DWT_SHAPES = [
+    dict(shape=(37, 5, 9), decomp=(3, 2, 3, 0)),
+    dict(shape=(3, 70, 2), decomp=(1, 4, 1, 0)),
+    dict(shape=(130, 1, 1), decomp=(5, 0, 0, 0)),
+    dict(shape=(2, 3, 67), decomp=(1, 1, 5, 0)),
+    dict(shape=(21, 19, 17, 6), decomp=(2, 2, 2, 1)),
+    dict(shape=(45, 33, 29), decomp=(3, 3, 3, 0), tiles=(24, 24, 24, 1)),
+    dict(shape=(600, 7, 3), decomp=(4, 1, 1, 0)),
+]
+
+
+@pytest.mark.parametrize("path", ["fast", "generic"])
+@pytest.mark.parametrize("spec", DWT_SHAPES, ids=[str(s["shape"]) for s in DWT_SHAPES])
+def test_dwt_shapes_bit_exact(spec, path, monkeypatch):
+    monkeypatch.setenv("BWC_AMD_DWT", "1" if path == "fast" else "0")
+    _say("dwt shapes", path, spec)
+    from bigwhoop_amd.fields import grf
+    shape = spec["shape"]
+    data = grf(shape, seed=7)
+    kw = dict(decomp=spec["decomp"])
+    if "tiles" in spec:
+        kw["tiles"] = spec["tiles"]
+    c = bwc.Codec(shape, rate="2", **kw)
+    o = ol.Oracle(shape, rate="2", **kw)
+    try:
+        for t in range(o.ntiles):
+            g, _ = c.stage_dwt(data, t, 0)
+            w, _ = o.fill_normalize(data, t, 0)
+            o.dwt_forward(w, t, 0)
+            assert np.array_equal(g.view(np.uint64), w.view(np.uint64)), (t, spec)
     finally:
         c.close()
         o.close()
