@@ -125,6 +125,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run one plain device copy of the field first (counter calibration runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
@@ -136,6 +138,13 @@ def main():
 
     n = args.size
     field = make_field_gpu(n)
+    if args.copy_calibration:
+        # one device copy of exactly n^3 * 8 bytes: the reference reading for
+        # the FETCH_SIZE / WRITE_SIZE counters of the DWT kernels
+        scratch = torch.empty_like(field)
+        scratch.copy_(field)
+        torch.cuda.synchronize()
+        del scratch
     codec = bwc.Codec((n, n, n), **SETTINGS)
     ptr = field.data_ptr()
     size = 0

@@ -399,9 +399,9 @@ size_t bwc_compress(bwc_codec *const codec, bwc_stream *const stream) {
       gpu::record(ws, 0);
       if (gpu::fill_minmax(ws, codec, tl, p, stream->inp, on_dev, ws->d_work)) goto fail;
       gpu::record(ws, 1);
-      if (gpu::normalize(ws, ws->d_work, par.box.volume(), ctl.Qm)) goto fail;
+      // normalisation is fused into the DWT's first pass (stage time 0 here)
       gpu::record(ws, 2);
-      if (gpu::dwt_forward(ws, codec, par, ws->d_work)) goto fail;
+      if (gpu::dwt_forward(ws, codec, par, ws->d_work, ctl.Qm)) goto fail;
       gpu::record(ws, 3);
       {
         double mm[4];
@@ -536,8 +536,7 @@ int bwc_amd_stage_dwt(bwc_codec *codec, uint64_t tile, uint8_t param, const void
   const uint64_t n = par.box.volume();
   if (gpu::ensure_work(ws, n)) return 1;
   if (gpu::fill_minmax(ws, codec, tl, param, input, gpu::is_device_pointer(input) != 0, ws->d_work)) return 1;
-  if (gpu::normalize(ws, ws->d_work, n, codec->control.Qm)) return 1;
-  if (gpu::dwt_forward(ws, codec, par, ws->d_work)) return 1;
+  if (gpu::dwt_forward(ws, codec, par, ws->d_work, codec->control.Qm)) return 1;
   if (gpu::read_norm(ws, mm)) return 1;
   return gpu::copy_d2h(ws, out, ws->d_work, n * sizeof(double));
 }

@@ -126,15 +126,20 @@ __global__ void minmax_final_kernel(const double *red, int nblk, double *norm) {
   }
 }
 
+// (x - alpha) * beta, then Qm added to the exponent field of every value
+// other than +-0 (libbwc.c:993-1078; denormals are shifted the same way)
+__device__ __forceinline__ double norm_value(double x, double alpha, double beta, uint64_t shift) {
+  const double v = (x - alpha) * beta;
+  uint64_t raw = __double_as_longlong(v);
+  raw += ((raw & 0x7FF0000000000000ull) == 0 && (raw & 0x000FFFFFFFFFFFFFull) == 0) ? 0 : shift;
+  return __longlong_as_double((long long)raw);
+}
+
 __global__ void normalize_kernel(double *__restrict__ data, uint64_t n, const double *__restrict__ norm,
                                  uint64_t shift) {
   const double alpha = norm[2], beta = norm[3];
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-    const double v = (data[i] - alpha) * beta;
-    uint64_t raw = __double_as_longlong(v);
-    raw += ((raw & 0x7FF0000000000000ull) == 0 && (raw & 0x000FFFFFFFFFFFFFull) == 0) ? 0 : shift;
-    data[i] = __longlong_as_double((long long)raw);
-  }
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    data[i] = norm_value(data[i], alpha, beta, shift);
 }
 
 int fill_minmax(Workspace *ws, const bwc_codec *c, const Tile &tl, int param, const void *input, bool on_dev,
@@ -365,9 +370,11 @@ __device__ __forceinline__ int mirror32(int p, int len) {
 // reads and writes consecutive LDS words and the output is two contiguous
 // runs (lowpass = Ev, highpass = Od). src and dst may be the same buffer.
 // Ev[j] holds i = 2j - 8, Od[j] holds i = 2j - 7.
+// With `norm` set (the first pass over a freshly filled tile) every sample is
+// normalised as it is read, as normalize_kernel would have done in place.
 __global__ void __launch_bounds__(256) dwt97_x_kernel(const double *src, double *dst, int len, int par0, int half,
                                                       uint32_t n0, uint64_t s0, uint32_t n1, uint64_t s1,
-                                                      uint64_t s2) {
+                                                      uint64_t s2, const double *norm, uint64_t shift) {
   extern __shared__ double lds[];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int hp = (len + 16) / 2 + 1;
@@ -378,9 +385,11 @@ __global__ void __launch_bounds__(256) dwt97_x_kernel(const double *src, double 
   const uint64_t base = (uint64_t)i0 * s0 + (uint64_t)i1 * s1 + (uint64_t)i2 * s2;
   if (act) {
     const double *s = src + base;
+    const double alpha = norm ? norm[2] : 0, beta = norm ? norm[3] : 0;
     for (int e = lane; e < len + 16; e += 64) {
       const int i = e - 8, p = i + par0;
-      const double v = s[((unsigned)p < (unsigned)len) ? p : mirror32(p, len)];
+      double v = s[((unsigned)p < (unsigned)len) ? p : mirror32(p, len)];
+      if (norm) v = norm_value(v, alpha, beta, shift);
       if (e & 1) Od[e >> 1] = v;
       else Ev[e >> 1] = v;
     }
@@ -410,19 +419,24 @@ __global__ void __launch_bounds__(256) dwt97_x_kernel(const double *src, double 
   }
 }
 
-// Strided axes: one thread per line, consecutive lanes on consecutive lines
-// of the fastest other axis, so every load and store of a wavefront is one
-// contiguous row segment. The four steps run as a pipeline along the line:
-// after reading x_{2k} and x_{2k+1} the thread finishes a_{2k-1}, b_{2k-2},
-// c_{2k-3} and d_{2k-4}. Highpass outputs land ahead of the read position,
-// so the pass is out of place (src != dst).
+// Strided axes: one thread per line segment, consecutive lanes on
+// consecutive lines of the fastest other axis, so every load and store of a
+// wavefront is one contiguous row segment. The four steps run as a pipeline
+// along the line: after reading x_{2k} and x_{2k+1} the thread finishes
+// a_{2k-1}, b_{2k-2}, c_{2k-3} and d_{2k-4} (outputs at i = 2k-4, 2k-3).
+// Output iterations k = 1 .. kend-1 are split into nseg segments of ksz; a
+// segment starts its pipeline 4 iterations early: the carried values start
+// as zeros and, as each step reaches back at most two iterations, are exact
+// again after three, so only exact values are stored. Highpass outputs land
+// ahead of the read position, so the pass is out of place (src != dst).
 template <int U>
 __global__ void __launch_bounds__(256) dwt97_stream_kernel(const double *__restrict__ src, double *__restrict__ dst,
                                                            int len, int par0, int half, uint64_t sa, uint32_t n0,
                                                            uint64_t s0, uint32_t n1, uint64_t s1, uint64_t s2,
-                                                           uint32_t nlines) {
-  const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
-  if (l >= nlines) return;
+                                                           uint32_t nlines, int nseg, int ksz) {
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= nlines * (uint32_t)nseg) return;
+  const uint32_t l = tid % nlines, seg = tid / nlines;
   const uint32_t i0 = l % n0, r = l / n0;
   const uint32_t i1 = r % n1, i2 = r / n1;
   const uint64_t base = (uint64_t)i0 * s0 + (uint64_t)i1 * s1 + (uint64_t)i2 * s2;
@@ -431,21 +445,24 @@ __global__ void __launch_bounds__(256) dwt97_stream_kernel(const double *__restr
   const int last0 = -3 + 2 * ((len + 7) / 2 - 1), last1 = -2 + 2 * ((len + 5) / 2 - 1);
   const int last2 = -1 + 2 * ((len + 3) / 2 - 1), last3 = 2 * ((len + 1) / 2 - 1);
   const int ilo = -par0, ihi = len - par0;   // output working indices [ilo, ihi)
+  const int kend = (ihi + 5) / 2;            // first k with 2k - 4 >= ihi
+  const int kA = 1 + (int)seg * ksz, kB = min(kend, kA + ksz);
+  if (kA >= kB) return;
   auto ld = [&](int i) -> double {
     const int p = i + par0;
     return s[(int64_t)(((unsigned)p < (unsigned)len) ? p : mirror32(p, len)) * (int64_t)sa];
   };
-  // carried: x_{2k-2}, x_{2k-1}, a_{2k-3}, b_{2k-4}, c_{2k-5}; positions below
-  // -4 are never combined into an output, so they start as zero
-  double xm2 = ld(-6), xm1 = ld(-5), am3 = 0, bm4 = 0, cm5 = 0;
-  for (int k = -2; 2 * k - 4 < ihi; k += U) {
+  // carried: x_{2k-2}, x_{2k-1}, a_{2k-3}, b_{2k-4}, c_{2k-5}
+  const int k0 = kA - 4;
+  double xm2 = ld(2 * k0 - 2), xm1 = ld(2 * k0 - 1), am3 = 0, bm4 = 0, cm5 = 0;
+  for (int k = k0; k < kB; k += U) {
     double xv[2 * U];
 #pragma unroll
     for (int u = 0; u < 2 * U; ++u) xv[u] = ld(2 * k + u);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int kk = k + u;
-      if (2 * kk - 4 >= ihi) break;
+      if (kk >= kB) break;
       const double x0 = xv[2 * u], x1 = xv[2 * u + 1];
       int i = 2 * kk - 1;
       const double a1 = (i >= -3 && i <= last0) ? xm1 + Lift97::A * (xm2 + x0) : xm1;
@@ -455,9 +472,9 @@ __global__ void __launch_bounds__(256) dwt97_stream_kernel(const double *__restr
       const double c3 = (i >= -1 && i <= last2) ? Lift97::KH * (am3 + Lift97::G * (bm4 + b2)) : am3;
       i = 2 * kk - 4;
       const double d4 = (i >= 0 && i <= last3) ? Lift97::KL * (bm4 + Lift97::D * (cm5 + c3)) : bm4;
-      if (i >= ilo && i < ihi) d[(int64_t)(i >> 1) * (int64_t)sa] = d4;
+      if (kk >= kA && i >= ilo && i < ihi) d[(int64_t)(i >> 1) * (int64_t)sa] = d4;
       i = 2 * kk - 3;
-      if (i >= ilo && i < ihi) d[(int64_t)(half + ((i + 2 * par0 - 1) >> 1)) * (int64_t)sa] = c3;
+      if (kk >= kA && i >= ilo && i < ihi) d[(int64_t)(half + ((i + 2 * par0 - 1) >> 1)) * (int64_t)sa] = c3;
       xm2 = x0;
       xm1 = x1;
       am3 = a1;
@@ -483,8 +500,11 @@ __global__ void box_copy_kernel(const double *__restrict__ src, double *__restri
   }
 }
 
-static int dwt_run(Workspace *ws, const bwc_codec *c, const Param &par, double *work, bool fwd) {
+static int dwt_run(Workspace *ws, const bwc_codec *c, const Param &par, double *work, bool fwd, int norm_qm) {
   const bwc_gl_ctrl &ctl = c->control;
+  // normalisation still to apply (forward only): fused into the first X pass
+  // when that pass runs on the fast kernel, else a separate in-place pass
+  bool norm_pending = fwd && norm_qm >= 0;
   const uint64_t W = par.box.len(0), H = par.box.len(1), D = par.box.len(2);
   const uint64_t inc[4] = {1, W, W * H, W * H * D};
   const uint8_t dec[4] = {ctl.decompX, ctl.decompY, ctl.decompZ, ctl.decompTS};
@@ -515,6 +535,10 @@ static int dwt_run(Workspace *ws, const bwc_codec *c, const Param &par, double *
       CHK(ensure((void **)&ws->d_work2, &cap, par.box.volume() * sizeof(double)));
       ws->work2_cap = cap;
       oth = ws->d_work2;
+    }
+    if (norm_pending && !fast97(0)) {
+      CHK(normalize(ws, work, par.box.volume(), (uint8_t)norm_qm));
+      norm_pending = false;
     }
     for (int oi = 0; oi < 4; ++oi) {
       const int a = fwd ? oi : 3 - oi;
@@ -551,17 +575,26 @@ static int dwt_run(Workspace *ws, const bwc_codec *c, const Param &par, double *
         double *dst = x_oop ? oth : cur;
         hipLaunchKernelGGL(dwt97_x_kernel, dim3((uint32_t)((sz[o[0]] + 3) / 4), (uint32_t)gy), dim3(256), xlds,
                            ws->stream, cur, dst, (int)len, par0, half, (uint32_t)sz[o[0]], inc[o[0]],
-                           (uint32_t)sz[o[1]], inc[o[1]], inc[o[2]]);
+                           (uint32_t)sz[o[1]], inc[o[1]], inc[o[2]], norm_pending ? ws->d_norm : nullptr,
+                           (uint64_t)(norm_pending ? norm_qm : 0) << 52);
         HIP_OK(hipGetLastError());
+        norm_pending = false;
         if (x_oop) std::swap(cur, oth);
         ++ws->last_dwt_launches;
         continue;
       }
       if (fast97(a) && a != 0) {
         const uint64_t nlines = sz[o[0]] * sz[o[1]] * sz[o[2]];
-        hipLaunchKernelGGL(dwt97_stream_kernel<4>, dim3((uint32_t)((nlines + 255) / 256)), dim3(256), 0, ws->stream,
+        // split lines into segments until about 128K threads run (short
+        // lines of the coarse levels), keeping >= 8 pair iterations each
+        const int kout = (int)((len - par0 + 5) / 2) - 1;
+        int nseg = 1;
+        while ((uint64_t)nseg * 2 * nlines <= (1u << 17) && kout / (nseg * 2) >= 8) nseg *= 2;
+        const int ksz = (kout + nseg - 1) / nseg;
+        const uint64_t nthreads = nlines * (uint64_t)nseg;
+        hipLaunchKernelGGL(dwt97_stream_kernel<4>, dim3((uint32_t)((nthreads + 255) / 256)), dim3(256), 0, ws->stream,
                            cur, oth, (int)len, par0, half, inc[a], (uint32_t)sz[o[0]], inc[o[0]], (uint32_t)sz[o[1]],
-                           inc[o[1]], inc[o[2]], (uint32_t)nlines);
+                           inc[o[1]], inc[o[2]], (uint32_t)nlines, nseg, ksz);
         HIP_OK(hipGetLastError());
         std::swap(cur, oth);
         ++ws->last_dwt_launches;
@@ -594,14 +627,15 @@ static int dwt_run(Workspace *ws, const bwc_codec *c, const Param &par, double *
       ++ws->last_dwt_launches;
     }
   }
+  if (norm_pending) CHK(normalize(ws, work, par.box.volume(), (uint8_t)norm_qm));
   return 0;
 }
 
-int dwt_forward(Workspace *ws, const bwc_codec *c, const Param &par, double *work) {
-  return dwt_run(ws, c, par, work, true);
+int dwt_forward(Workspace *ws, const bwc_codec *c, const Param &par, double *work, int norm_qm) {
+  return dwt_run(ws, c, par, work, true, norm_qm);
 }
 int dwt_inverse(Workspace *ws, const bwc_codec *c, const Param &par, double *work) {
-  return dwt_run(ws, c, par, work, false);
+  return dwt_run(ws, c, par, work, false, -1);
 }
 
 // ===========================================================================

@@ -66,7 +66,9 @@ Workspace *workspace_of(Job *job);
 int fill_minmax(Workspace *ws, const bwc_codec *c, const Tile &tl, int param, const void *input, bool input_on_device,
                 double *work);
 int normalize(Workspace *ws, double *work, uint64_t n, uint8_t Qm);
-int dwt_forward(Workspace *ws, const bwc_codec *c, const Param &par, double *work);
+// norm_qm >= 0: the tile still needs normalize(..., norm_qm); the DWT applies
+// it (fused into its first pass where it can)
+int dwt_forward(Workspace *ws, const bwc_codec *c, const Param &par, double *work, int norm_qm = -1);
 int dwt_inverse(Workspace *ws, const bwc_codec *c, const Param &par, double *work);
 // quantise + tier-1 for all codeblocks of (tile, param); results land in rec
 int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double *work, CblkRecords &rec,
