@@ -386,12 +386,22 @@ __global__ void __launch_bounds__(256) dwt97_x_kernel(const double *src, double 
   if (act) {
     const double *s = src + base;
     const double alpha = norm ? norm[2] : 0, beta = norm ? norm[3] : 0;
-    for (int e = lane; e < len + 16; e += 64) {
-      const int i = e - 8, p = i + par0;
-      double v = s[((unsigned)p < (unsigned)len) ? p : mirror32(p, len)];
-      if (norm) v = norm_value(v, alpha, beta, shift);
-      if (e & 1) Od[e >> 1] = v;
-      else Ev[e >> 1] = v;
+    // eight loads in flight per lane before the first LDS write
+    for (int e0 = 0; e0 < len + 16; e0 += 64 * 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + 64 * u + lane, p = e - 8 + par0;
+        v[u] = (e < len + 16) ? s[((unsigned)p < (unsigned)len) ? p : mirror32(p, len)] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + 64 * u + lane;
+        if (e >= len + 16) break;
+        const double x = norm ? norm_value(v[u], alpha, beta, shift) : v[u];
+        if (e & 1) Od[e >> 1] = x;
+        else Ev[e >> 1] = x;
+      }
     }
   }
   __syncthreads();
@@ -514,6 +524,14 @@ static int dwt_run(Workspace *ws, const bwc_codec *c, const Param &par, double *
   // BWC_AMD_DWT=0 keeps every pass on the generic in-place kernel
   const char *dwt_env = std::getenv("BWC_AMD_DWT");
   const bool generic_only = dwt_env && dwt_env[0] == '0';
+  // launch shape of the streaming passes: pair iterations unrolled per load
+  // batch, thread target 2^seg_log2 and minimum iterations per segment
+  struct {
+    int unroll = 4, seg_log2 = 17, seg_min = 8;
+  } tune;
+  if (const char *e = std::getenv("BWC_AMD_DWT_UNROLL")) tune.unroll = std::atoi(e);
+  if (const char *e = std::getenv("BWC_AMD_DWT_SEG_LOG2")) tune.seg_log2 = std::max(8, std::min(24, std::atoi(e)));
+  if (const char *e = std::getenv("BWC_AMD_DWT_SEG_MIN")) tune.seg_min = std::max(2, std::atoi(e));
   for (int step = 0; step < nd; ++step) {
     const int level = fwd ? step : nd - 1 - step;
     const Box &rb = par.res[nd - level].box;
@@ -589,12 +607,18 @@ static int dwt_run(Workspace *ws, const bwc_codec *c, const Param &par, double *
         // lines of the coarse levels), keeping >= 8 pair iterations each
         const int kout = (int)((len - par0 + 5) / 2) - 1;
         int nseg = 1;
-        while ((uint64_t)nseg * 2 * nlines <= (1u << 17) && kout / (nseg * 2) >= 8) nseg *= 2;
+        while ((uint64_t)nseg * 2 * nlines <= (1ull << tune.seg_log2) && kout / (nseg * 2) >= tune.seg_min) nseg *= 2;
         const int ksz = (kout + nseg - 1) / nseg;
         const uint64_t nthreads = nlines * (uint64_t)nseg;
-        hipLaunchKernelGGL(dwt97_stream_kernel<4>, dim3((uint32_t)((nthreads + 255) / 256)), dim3(256), 0, ws->stream,
-                           cur, oth, (int)len, par0, half, inc[a], (uint32_t)sz[o[0]], inc[o[0]], (uint32_t)sz[o[1]],
-                           inc[o[1]], inc[o[2]], (uint32_t)nlines, nseg, ksz);
+        const uint32_t nblk = (uint32_t)((nthreads + 255) / 256);
+#define STREAM_LAUNCH(UU)                                                                                      \
+  hipLaunchKernelGGL(dwt97_stream_kernel<UU>, dim3(nblk), dim3(256), 0, ws->stream, cur, oth, (int)len, par0, \
+                     half, inc[a], (uint32_t)sz[o[0]], inc[o[0]], (uint32_t)sz[o[1]], inc[o[1]], inc[o[2]],    \
+                     (uint32_t)nlines, nseg, ksz)
+        if (tune.unroll == 2) STREAM_LAUNCH(2);
+        else if (tune.unroll == 8) STREAM_LAUNCH(8);
+        else STREAM_LAUNCH(4);
+#undef STREAM_LAUNCH
         HIP_OK(hipGetLastError());
         std::swap(cur, oth);
         ++ws->last_dwt_launches;
@@ -1352,7 +1376,7 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
     qoff[i] = qo;
     qo += 4 * ncol;
     doff[i] = dof;
-    dof += (cw * ch * cd * ct) * 10 + 256;
+    dof += ((cw * ch * cd * ct) * 10 + 256 + 63) & ~63ull;   // 64-byte aligned buffers
   }
   qoff[ncb] = qo;
   doff[ncb] = dof;
@@ -1498,6 +1522,7 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
         case 2: MQ_LAUNCH(2); break;
         case 4: MQ_LAUNCH(4); break;
         case 8: MQ_LAUNCH(8); break;
+        case 32: MQ_LAUNCH(32); break;
         default: MQ_LAUNCH(16); break;
       }
 #undef MQ_LAUNCH
