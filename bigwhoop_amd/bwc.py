@@ -98,6 +98,10 @@ def lib():
                                               C.POINTER(u32), C.POINTER(u16), C.POINTER(u8), C.POINTER(u64),
                                               C.POINTER(C.c_double)]),
             "bwc_amd_encode_records": (C.c_size_t, [vp, vp]),
+            "bwc_amd_set_pass_skip": (C.c_int, [vp, C.c_int, C.c_int]),
+            "bwc_amd_set_skip_model": (C.c_int, [vp, C.POINTER(C.c_float), C.c_size_t]),
+            "bwc_amd_skip_stats": (C.c_int, [vp, C.POINTER(u64)]),
+            "bwc_amd_cblk_cuts": (C.c_int, [vp, u64, u8, C.POINTER(C.c_int16)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -250,6 +254,28 @@ class Codec:
         la = (C.c_int * 4)()
         lib().bwc_amd_last_timings(self.h, ms, la)
         return list(ms), list(la)
+
+    # ---- pass skipping ----
+    def set_pass_skip(self, mode: int, param: int = 0):
+        if lib().bwc_amd_set_pass_skip(self.h, mode, param):
+            raise RuntimeError("bwc_amd_set_pass_skip failed: " + last_error())
+
+    def set_skip_model(self, weights):
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        if lib().bwc_amd_set_skip_model(self.h, _p(w, C.c_float), w.size):
+            raise RuntimeError("bwc_amd_set_skip_model failed: " + last_error())
+
+    def skip_stats(self):
+        s = (C.c_uint64 * 8)()
+        lib().bwc_amd_skip_stats(self.h, s)
+        keys = ["codeblocks", "passes_coded", "passes_complete", "extra_rounds", "extended", "fallbacks"]
+        return dict(zip(keys, list(s)[:6]))
+
+    def cuts(self, tile=0, param=0):
+        out = np.zeros(self.ncblk(tile, param), np.int16)
+        if lib().bwc_amd_cblk_cuts(self.h, tile, param, _p(out, C.c_int16)):
+            raise RuntimeError("bwc_amd_cblk_cuts failed")
+        return out
 
     # ---- single stages (bwc_amd.h) ----
     def ncblk(self, tile=0, param=0):

@@ -15,11 +15,6 @@
 
 using namespace bwc;
 
-namespace bwc {
-int t2_rate_control(bwc_codec *codec, Job *job, uint64_t tile);
-int t2_packets(bwc_codec *codec, Job *job, uint64_t tile);
-}
-
 namespace {
 
 thread_local char g_api_err[512];
@@ -67,6 +62,82 @@ void reset_for_encode(bwc_codec *c, Job *job) {
       }
   }
   job->rec.assign(job->plan.tiles.size() * c->info.nPar, CblkRecords());
+}
+
+// tier-2 state of one tile back to "before rate control", so rate control
+// can run again after codeblocks were coded further
+void reset_tile_t2(bwc_codec *c, Job *job, uint64_t ti) {
+  Tile &tl = job->plan.tiles[ti];
+  const int nl = c->control.nLayers;
+  for (auto &par : tl.par)
+    for (auto &res : par.res) {
+      for (auto &pk : res.pk) pk = Packet();
+      for (auto &sb : res.sb)
+        for (auto &pr : sb.prec) {
+          std::fill(pr.incl.value.begin(), pr.incl.value.end(), 0xFFFF);
+          std::fill(pr.incl.threshold.begin(), pr.incl.threshold.end(), 0);
+          std::fill(pr.msbs.value.begin(), pr.msbs.value.end(), 0xFFFF);
+          std::fill(pr.msbs.threshold.begin(), pr.msbs.threshold.end(), 0);
+        }
+    }
+  // slope bounds over the current records (nonzero slopes, as tier-1 reports)
+  tl.slope_min = 0xFFFF;
+  tl.slope_max = 0;
+  for (int p = 0; p < c->info.nPar; ++p) {
+    CblkRecords &R = job->rec[ti * c->info.nPar + p];
+    R.beta.assign(R.n, 3);
+    R.beta_est.assign(R.n, 3);
+    R.cp.assign(R.n * (nl + 1), 0);
+    for (uint64_t i = 0; i < R.n; ++i) {
+      if (!R.K[i]) continue;
+      const uint16_t *S = &R.S[i * (kMaxPasses + 1)];
+      for (int z = 0; z <= R.Z[i]; ++z)
+        if (S[z]) {
+          tl.slope_min = std::min(tl.slope_min, S[z]);
+          tl.slope_max = std::max(tl.slope_max, S[z]);
+        }
+    }
+  }
+}
+
+// Rate control of one tile whose codeblocks may be coded to a prefix of
+// their passes: run it, and while its outcome is not provably that of
+// complete codings, code the flagged codeblocks further (limits grow 9, 18,
+// 36 passes per round; complete coding from the fourth round or for the
+// whole tile on a fallback) and run it again.
+int rate_control_prefix(bwc_codec *c, Job *job, gpu::Workspace *ws, uint64_t ti) {
+  Tile &tl = job->plan.tiles[ti];
+  const int np = c->info.nPar;
+  SkipCheck chk;
+  chk.margin = 3;
+  for (int round = 0;; ++round) {
+    reset_tile_t2(c, job, ti);
+    t2_rate_control(c, job, ti, &chk);
+    uint64_t nflag = 0;
+    for (auto &f : chk.flagged) nflag += f.size();
+    if (!chk.fallback && !nflag) return 0;
+    job->skip_stats[3] += 1;
+    if (chk.fallback) job->skip_stats[5] += 1;
+    for (int p = 0; p < np; ++p) {
+      CblkRecords &R = job->rec[ti * np + p];
+      std::vector<uint64_t> list;
+      if (chk.fallback) {
+        for (uint64_t i = 0; i < R.n; ++i)
+          if (R.Z[i] < R.Zfull[i]) list.push_back(i);
+      } else {
+        list = chk.flagged[p];
+      }
+      if (list.empty()) continue;
+      for (uint64_t i : list) {
+        const int step = 9 << std::min(round, 3);
+        int nz = (chk.fallback || round >= 3) ? R.Zfull[i] : std::min<int>(R.Zfull[i], R.Z[i] + step);
+        R.zlim[i] = (uint8_t)nz;
+        job->skip_stats[1] += (uint64_t)nz;   // passes coded again
+      }
+      job->skip_stats[4] += list.size();
+      if (gpu::t1_extend(ws, c, tl.par[p], R, p, list, R.zlim.data())) return 1;
+    }
+  }
 }
 
 }  // namespace
@@ -383,6 +454,7 @@ size_t bwc_compress(bwc_codec *const codec, bwc_stream *const stream) {
   bwc_gl_ctrl &ctl = codec->control;
   const double t_start = now_ms();
   reset_for_encode(codec, job);
+  std::fill(std::begin(job->skip_stats), std::end(job->skip_stats), 0);
   const bool on_dev = gpu::is_device_pointer(stream->inp) != 0;
   uint64_t maxsz = 0;
   for (auto &tl : job->plan.tiles) maxsz = std::max<uint64_t>(maxsz, tl.box.volume());
@@ -416,16 +488,33 @@ size_t bwc_compress(bwc_codec *const codec, bwc_stream *const stream) {
       ms[2] += gpu::elapsed(ws, 2, 3);
       dwt_launch += ws->last_dwt_launches;
       gpu::record(ws, 4);
-      if (gpu::t1_encode(ws, codec, par, ws->d_work, job->rec[ti * codec->info.nPar + p], &tl.slope_min,
-                         &tl.slope_max, p))
-        goto fail;
+      {
+        CblkRecords &R = job->rec[ti * codec->info.nPar + p];
+        std::vector<uint8_t> zl;
+        if (job->skip_mode == 1) zl.assign(par.ncb, (uint8_t)std::min(255, std::max(1, job->skip_param)));
+        if (gpu::t1_encode(ws, codec, par, ws->d_work, R, &tl.slope_min, &tl.slope_max, p,
+                           zl.empty() ? nullptr : zl.data()))
+          goto fail;
+        if (zl.empty()) zl.assign(par.ncb, 255);
+        R.zlim = zl;
+        for (uint64_t i = 0; i < R.n; ++i) {
+          if (!R.K[i]) continue;
+          job->skip_stats[0] += 1;
+          job->skip_stats[1] += R.Z[i];
+          job->skip_stats[2] += R.Zfull[i];
+        }
+      }
       gpu::record(ws, 5);
       gpu::sync(ws);
       ms[4] += gpu::elapsed(ws, 4, 5);
     }
     {
       const double t0 = now_ms();
-      t2_rate_control(codec, job, ti);
+      if (job->skip_mode) {
+        if (rate_control_prefix(codec, job, ws, ti)) goto fail;
+      } else {
+        t2_rate_control(codec, job, ti);
+      }
       ms[6] += now_ms() - t0;
       const double t1 = now_ms();
       const int nl = ctl.nLayers;
@@ -605,6 +694,8 @@ int bwc_amd_set_records(bwc_codec *codec, uint64_t tile, uint8_t param, const ui
   std::memcpy(R.K.data(), K, par.ncb);
   std::memcpy(R.Kmsbs.data(), Kmsbs, par.ncb);
   std::memcpy(R.Z.data(), Z, par.ncb);
+  std::memcpy(R.Zfull.data(), Z, par.ncb);   // complete codings
+  std::memcpy(R.U.data(), Z, par.ncb);
   std::memcpy(R.L.data(), L, par.ncb * kMaxPasses * sizeof(uint32_t));
   std::memcpy(R.S.data(), S, par.ncb * (kMaxPasses + 1) * sizeof(uint16_t));
   R.data.assign(data, data + data_off[par.ncb]);
@@ -643,6 +734,49 @@ int bwc_amd_last_timings(bwc_codec *codec, double ms[10], int launches[4]) {
   if (!job) return 1;
   std::memcpy(ms, job->ms, sizeof job->ms);
   std::memcpy(launches, job->launches, sizeof job->launches);
+  return 0;
+}
+
+int bwc_amd_set_pass_skip(bwc_codec *codec, int mode, int param) {
+  Job *job = job_of(codec);
+  if (!job || mode < 0 || mode > 2 || param < 0 || param > 255) {
+    set_err("bwc_amd_set_pass_skip: invalid codec, mode or parameter");
+    return 1;
+  }
+  if (mode == 2 && job->skip_w.empty()) {
+    set_err("bwc_amd_set_pass_skip: mode 2 needs bwc_amd_set_skip_model first");
+    return 1;
+  }
+  job->skip_mode = mode;
+  job->skip_param = param;
+  return 0;
+}
+
+int bwc_amd_set_skip_model(bwc_codec *codec, const float *w, size_t n) {
+  Job *job = job_of(codec);
+  if (!job || !w || n != (size_t)kSkipModelSize) {
+    set_err("bwc_amd_set_skip_model: invalid codec or weight count");
+    return 1;
+  }
+  job->skip_w.assign(w, w + n);
+  return 0;
+}
+
+int bwc_amd_skip_stats(bwc_codec *codec, uint64_t stats[8]) {
+  Job *job = job_of(codec);
+  if (!job) return 1;
+  std::memcpy(stats, job->skip_stats, sizeof job->skip_stats);
+  return 0;
+}
+
+int bwc_amd_cblk_cuts(bwc_codec *codec, uint64_t tile, uint8_t param, int16_t *cuts) {
+  Job *job = job_of(codec);
+  if (!job || tile >= job->plan.tiles.size() || param >= codec->info.nPar) return 1;
+  const uint64_t r = tile * codec->info.nPar + param;
+  if (r >= job->rec.size()) return 1;
+  const CblkRecords &R = job->rec[r];
+  const int nl = codec->control.nLayers;
+  for (uint64_t i = 0; i < R.n; ++i) cuts[i] = R.cp.empty() ? 0 : R.cp[i * (nl + 1) + nl];
   return 0;
 }
 

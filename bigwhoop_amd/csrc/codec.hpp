@@ -19,6 +19,10 @@ namespace bwc {
 constexpr int kMaxPasses = 190;     // (64 * 3) - 2 fractional bitplanes, FP64 build
 constexpr int kPrecBits = 63;       // PREC_BIT of the FP64 build
 constexpr int kPrecBytes = 8;
+// pass-limit network 16 -> 64 -> 64 -> 1 (+ 16 feature means and scales)
+constexpr int kSkipFeatures = 16, kSkipHidden = 64;
+constexpr int kSkipModelSize = kSkipHidden * kSkipFeatures + kSkipHidden + kSkipHidden * kSkipHidden + kSkipHidden +
+                               kSkipHidden + 1 + 2 * kSkipFeatures;
 
 struct Box {
   uint64_t lo[4] = {0, 0, 0, 0};
@@ -97,6 +101,10 @@ struct Tile {
 struct CblkRecords {
   uint64_t n = 0;
   std::vector<uint8_t> K, Kmsbs, Z;
+  // Z = passes coded; Zfull = 3K - 2 passes of a complete coding; U = first
+  // pass whose length may still change when the codeblock is coded further
+  // (Z when coded completely)
+  std::vector<uint8_t> Zfull, U;
   std::vector<uint32_t> L;      // kMaxPasses per codeblock, bytes
   std::vector<uint16_t> S;      // kMaxPasses + 1 per codeblock
   // tier-2 state
@@ -109,6 +117,8 @@ struct CblkRecords {
   // per-codeblock offsets inside it
   int slot = 0;
   std::vector<uint64_t> src_off;
+  // pass limits the codeblocks were coded with (pass skipping)
+  std::vector<uint8_t> zlim;
   void resize(uint64_t ncb, int nlayers);
 };
 
@@ -133,6 +143,14 @@ struct Job {
   // stage timings of the last compression (bwc_amd_last_timings)
   double ms[10] = {0};
   int launches[4] = {0};
+  // pass skipping (bwc_amd_set_pass_skip): 0 off, 1 fixed pass limit
+  // (skip_param passes), 2 learned limit (skip_param margin passes)
+  int skip_mode = 0, skip_param = 0;
+  std::vector<float> skip_w;        // network weights, fp32 (mode 2)
+  // last compression: [0] codeblocks coded, [1] passes coded (all rounds),
+  // [2] passes of complete codings, [3] extra rate-control rounds,
+  // [4] codeblocks coded further, [5] fallbacks to complete coding
+  uint64_t skip_stats[8] = {0};
   ~Job() {
     if (ws && ws_free) ws_free(ws);
   }
@@ -149,6 +167,16 @@ Job *job_of(bwc_codec *c);
 
 // ---- tier-2 + codestream (tier2.cpp) ----
 int t2_encode_tile(bwc_codec *codec, Job *job, uint64_t tile);
+// Outcome of rate control over prefix-coded records (pass skipping):
+// codeblocks per parameter that must be coded further, or `fallback` when
+// the tile's thresholds are not provably those of complete codings.
+struct SkipCheck {
+  int margin = 3;
+  bool fallback = false;
+  std::vector<std::vector<uint64_t>> flagged;
+};
+int t2_rate_control(bwc_codec *codec, Job *job, uint64_t tile, SkipCheck *chk = nullptr);
+int t2_packets(bwc_codec *codec, Job *job, uint64_t tile);
 size_t assemble_codestream(bwc_codec *codec, Job *job, const bwc_stream *st, uint8_t *out);
 uint64_t header_size(const bwc_codec *codec, const bwc_stream *st);
 

@@ -361,6 +361,8 @@ void CblkRecords::resize(uint64_t ncb, int nlayers) {
   K.assign(ncb, 0);
   Kmsbs.assign(ncb, 0);
   Z.assign(ncb, 0);
+  Zfull.assign(ncb, 0);
+  U.assign(ncb, 0);
   L.assign(ncb * kMaxPasses, 0);
   S.assign(ncb * (kMaxPasses + 1), 0);
   beta.assign(ncb, 3);

@@ -44,9 +44,11 @@ Workspace::~Workspace() {
   (void)hipSetDevice(device);
   void *ptrs[] = {d_input, d_work,  d_work2, d_red,  d_norm, d_cbbox, d_cbsb, d_cbmeta, d_qoff,   d_doff, d_q,
                   d_state, d_data,  d_K,    d_Kmsbs, d_Z,    d_L,    d_S,      d_snap,   d_mse,  d_flags,
-                  d_packed, d_gather, d_bp, d_bpoff, d_sym, d_symoff, d_pcnt, d_pmse, d_list};
+                  d_packed, d_gather, d_bp, d_bpoff, d_sym, d_symoff, d_pcnt, d_pmse, d_list, d_U, d_zlim};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  for (auto &b : bp_slots)
+    if (b.bp) (void)hipFree(b.bp);
   for (void *p : d_datap)
     if (p) (void)hipFree(p);
   if (h_pinned) (void)hipHostFree(h_pinned);
@@ -1011,7 +1013,8 @@ __device__ __forceinline__ uint32_t byte_at(const uint8_t *buf, uint64_t cap, in
   return (i >= 0 && (uint64_t)i < cap) ? buf[i] : 0u;
 }
 
-__device__ int64_t trunc_len(MqState s, const uint8_t *buf, uint64_t cap) {
+// *reach: one past the highest byte index the length search read
+__device__ int64_t trunc_len(MqState s, const uint8_t *buf, uint64_t cap, int64_t *reach) {
   int64_t Cr = ((int64_t)s.T << 27) + ((int64_t)s.C << s.t);
   int64_t CAr = ((int64_t)s.A << s.t) + Cr;
   int sh = 8;
@@ -1026,6 +1029,7 @@ __device__ int64_t trunc_len(MqState s, const uint8_t *buf, uint64_t cap) {
     CAr <<= sh;
     sh = (v == 0xFF) ? 7 : 8;
   }
+  *reach = s.L;
   if (s.L > 0 && byte_at(buf, cap, s.L - 1) == 0xFF) s.L--;
   while (s.L > 1 && byte_at(buf, cap, s.L - 2) == 0xFF && byte_at(buf, cap, s.L - 1) == 0x7F) s.L -= 2;
   return s.L;
@@ -1041,8 +1045,11 @@ __device__ uint16_t slope2log(double lambda) {
   return (uint16_t)v;
 }
 
-__device__ void finish_block(MqState last, uint8_t *buf, uint64_t cap, const uint64_t *sn, const double *mse, int Z,
-                             uint32_t *L, uint16_t *S, uint32_t *flags);
+// nfinal: bytes [0, nfinal) of buf are final (a longer coding of the same
+// codeblock starts with them); *U receives the first pass whose length
+// search read past them (Z if none)
+__device__ void finish_block(uint8_t *buf, uint64_t cap, const uint64_t *sn, const double *mse, int Z,
+                             uint32_t *L, uint16_t *S, uint32_t *flags, int64_t nfinal, uint8_t *U);
 
 __global__ void __launch_bounds__(64) t1_encode_kernel(
     uint64_t ncb, const uint32_t *__restrict__ cbbox, const double *__restrict__ cbsb,
@@ -1141,15 +1148,15 @@ __global__ void __launch_bounds__(64) t1_encode_kernel(
   }
   Kmsbsp[cb] = (uint8_t)(Kmax - K);
   Zp[cb] = (uint8_t)Z;
-  finish_block(C.s, C.buf, C.cap, sn, mse, Z, L, S, flags);
+  finish_block(C.buf, C.cap, sn, mse, Z, L, S, flags, INT64_MAX, nullptr);
 }
 
 // Truncation lengths of every pass snapshot (mq.c:706-759), the convex hull
 // and the log slopes (tier1.c:2161-2267) of one finished codeblock; folds its
 // non-zero slopes into the tile bounds (tier1.c:3047-3061).
-__device__ void finish_block(MqState last, uint8_t *buf, uint64_t cap, const uint64_t *sn, const double *mse, int Z,
-                             uint32_t *L, uint16_t *S, uint32_t *flags) {
-  (void)last;
+__device__ void finish_block(uint8_t *buf, uint64_t cap, const uint64_t *sn, const double *mse, int Z,
+                             uint32_t *L, uint16_t *S, uint32_t *flags, int64_t nfinal, uint8_t *U) {
+  int u = Z;
   for (int z = 0; z < Z; ++z) {
     MqState s;
     s.L = (int64_t)sn[2 * z];
@@ -1158,8 +1165,11 @@ __device__ void finish_block(MqState last, uint8_t *buf, uint64_t cap, const uin
     s.A = (uint16_t)(w >> 32);
     s.t = (int8_t)(uint8_t)(w >> 48);
     s.T = (uint8_t)(w >> 56);
-    L[z] = (uint32_t)trunc_len(s, buf, cap);
+    int64_t reach = 0;
+    L[z] = (uint32_t)trunc_len(s, buf, cap, &reach);
+    if (reach > nfinal && z < u) u = z;
   }
+  if (U) *U = (uint8_t)u;
   uint8_t hull[192];
   double lam[191];
   for (int z = 0; z <= Z; ++z) lam[z] = 0;
@@ -1354,43 +1364,178 @@ static int upload_tables() {
 
 Workspace *workspace_of(Job *job);
 
-int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double *work, CblkRecords &rec,
-              uint16_t *slope_min, uint16_t *slope_max, int slot) {
+// Per-codeblock tables of one (tile, param), rebuilt identically for every
+// launch that touches it: quantisation steps and gain, hb | Kmax << 8, the
+// offsets of its stripe columns, MQ byte buffer (64-byte aligned) and
+// bitplanes.
+struct CbTables {
+  std::vector<double> sb;
+  std::vector<uint32_t> meta;
+  std::vector<uint64_t> qoff, doff, bpoff;
+  uint64_t qo = 0, dof = 0, bo = 0;
+};
+
+static void build_tables(const Param &par, CbTables &t) {
   const uint64_t ncb = par.ncb;
-  rec.resize(ncb, c->control.nLayers);
-  if (!ncb) return 0;
-  // per-codeblock tables
-  std::vector<double> sb(3 * ncb);
-  std::vector<uint32_t> meta(ncb);
-  std::vector<uint64_t> qoff(ncb + 1), doff(ncb + 1);
-  uint64_t qo = 0, dof = 0;
+  t.sb.assign(3 * ncb, 0);
+  t.meta.assign(ncb, 0);
+  t.qoff.assign(ncb + 1, 0);
+  t.doff.assign(ncb + 1, 0);
+  t.bpoff.assign(ncb + 1, 0);
+  t.qo = t.dof = t.bo = 0;
   for (uint64_t i = 0; i < ncb; ++i) {
     const uint32_t *b = &par.cb_box[8 * i];
     const Subband &s = par.res[par.cb_sbref[i] >> 8].sb[par.cb_sbref[i] & 0xFF];
-    sb[3 * i] = s.step_eff;
-    sb[3 * i + 1] = s.step;
-    sb[3 * i + 2] = s.gain;
-    meta[i] = (uint32_t)s.hb | ((uint32_t)s.Kmax << 8);
+    t.sb[3 * i] = s.step_eff;
+    t.sb[3 * i + 1] = s.step;
+    t.sb[3 * i + 2] = s.gain;
+    t.meta[i] = (uint32_t)s.hb | ((uint32_t)s.Kmax << 8);
     const uint64_t cw = b[4] - b[0], ch = b[5] - b[1], cd = b[6] - b[2], ct = b[7] - b[3];
     const uint64_t ncol = cw * ((ch + 3) / 4) * cd * ct;
-    qoff[i] = qo;
-    qo += 4 * ncol;
-    doff[i] = dof;
-    dof += ((cw * ch * cd * ct) * 10 + 256 + 63) & ~63ull;   // 64-byte aligned buffers
+    t.qoff[i] = t.qo;
+    t.qo += 4 * ncol;
+    t.doff[i] = t.dof;
+    t.dof += ((cw * ch * cd * ct) * 10 + 256 + 63) & ~63ull;   // 64-byte aligned buffers
+    t.bpoff[i] = t.bo;
+    t.bo += 64 * (ncol + 1) / 2 + 64;
   }
-  qoff[ncb] = qo;
-  doff[ncb] = dof;
+  t.qoff[ncb] = t.qo;
+  t.doff[ncb] = t.dof;
+  t.bpoff[ncb] = t.bo;
+}
+
+// symbol stream offsets from the actual K: per plane one significance or
+// refinement symbol per value, one sign per value overall, three run
+// symbols per stripe column, four segmentation symbols; only the first
+// zlim[i] passes when limits are given
+static uint64_t stream_offsets(const Param &par, const CbTables &t, const uint8_t *K, const uint8_t *zlim,
+                               std::vector<uint64_t> &symoff) {
+  const uint64_t ncb = par.ncb;
+  symoff.assign(ncb + 1, 0);
+  uint64_t so = 0;
+  for (uint64_t i = 0; i < ncb; ++i) {
+    symoff[i] = so;
+    const uint32_t *b = &par.cb_box[8 * i];
+    const uint64_t nv = (uint64_t)(b[4] - b[0]) * (b[5] - b[1]) * (b[6] - b[2]) * (b[7] - b[3]);
+    const uint64_t ncol = (t.qoff[i + 1] - t.qoff[i]) / 4;
+    uint64_t planes = K[i];
+    if (zlim && K[i] && zlim[i] < 3u * K[i] - 2) planes = (zlim[i] + 2) / 3 + 1;
+    so += (nv * (planes + 1) + 3 * ncol * planes + 4 * planes + 32 + 15) & ~15ull;
+  }
+  symoff[ncb] = so;
+  return so;
+}
+
+// LDS of the symbol generator: per codeblock it packs P = 64 / pow2ceil(width)
+// slices side by side, each with its own pass-output buffer, so size for the
+// largest P times the largest per-slice bound
+struct GenParams {
+  uint32_t maxcol = 0, obuf_cap = 0, maxcw = 0, P = 1;
+  size_t lds = 0;
+};
+static GenParams gen_params(const Param &par) {
+  GenParams g;
+  for (uint64_t i = 0; i < par.ncb; ++i) {
+    const uint32_t *b = &par.cb_box[8 * i];
+    const uint32_t cw = b[4] - b[0], ch = b[5] - b[1], ns = (ch + 3) / 4;
+    const uint32_t ncol = cw * ns * (b[6] - b[2]) * (b[7] - b[3]);
+    g.maxcol = std::max(g.maxcol, ncol);
+    g.maxcw = std::max(g.maxcw, cw);
+    g.obuf_cap = std::max(g.obuf_cap, cw * ch * 2 + cw * ns * 3 + 16);
+    uint32_t gw = 1;
+    while (gw < cw) gw <<= 1;
+    const uint32_t nsl = (b[6] - b[2]) * (b[7] - b[3]);
+    if (gw <= 64) g.P = std::max(g.P, std::min(64 / gw, nsl));
+  }
+  g.lds = (size_t)g.maxcol * 2 + (size_t)((g.maxcol + 1) / 2) * 8 + (size_t)g.P * g.obuf_cap;
+  return g;
+}
+
+// Symbol generation and MQ coding of the codeblocks in `list`; the
+// (tile, param) tables, K and bitplane offsets are on the device already.
+// zlim: per-codeblock pass limits (host array of ncb) or null for all passes.
+static int v2_code(Workspace *ws, const bwc_codec *c, const Param &par, const CbTables &tb, const uint8_t *Kh,
+                   const std::vector<uint64_t> &list, const uint8_t *zlim, uint8_t *d_data, const uint8_t *d_bp,
+                   const GenParams &gp) {
+  const uint64_t ncb = par.ncb;
+  std::vector<uint64_t> symoff;
+  const uint64_t so = stream_offsets(par, tb, Kh, zlim, symoff);
+  size_t cap = ws->cap_sym;
+  CHK(ensure((void **)&ws->d_sym, &cap, so + 64));
+  ws->cap_sym = cap;
+  cap = ws->cap_symoff;
+  CHK(ensure((void **)&ws->d_symoff, &cap, (ncb + 1) * sizeof(uint64_t)));
+  ws->cap_symoff = cap;
+  cap = ws->cap_pc;
+  CHK(ensure((void **)&ws->d_pcnt, &cap, (ncb + 1) * kMaxPasses * sizeof(uint32_t)));
+  ws->cap_pc = cap;
+  cap = ws->cap_pm;
+  CHK(ensure((void **)&ws->d_pmse, &cap, (ncb + 1) * kMaxPasses * sizeof(int64_t)));
+  ws->cap_pm = cap;
+  cap = ws->cap_list;
+  CHK(ensure((void **)&ws->d_list, &cap, (list.size() + 1) * sizeof(uint64_t)));
+  ws->cap_list = cap;
+  HIP_OK(hipMemcpyAsync(ws->d_symoff, symoff.data(), (ncb + 1) * sizeof(uint64_t), hipMemcpyHostToDevice,
+                        ws->stream));
+  if (zlim) HIP_OK(hipMemcpyAsync(ws->d_zlim, zlim, ncb, hipMemcpyHostToDevice, ws->stream));
+  const uint8_t *dz = zlim ? ws->d_zlim : nullptr;
+  if (list.empty()) return 0;
+  const int erres = (int)c->control.error_resilience;
+  HIP_OK(hipMemcpyAsync(ws->d_list, list.data(), list.size() * sizeof(uint64_t), hipMemcpyHostToDevice,
+                        ws->stream));
+  hipLaunchKernelGGL(t1_gen_kernel, dim3((uint32_t)list.size()), dim3(64), gp.lds, ws->stream, ws->d_list,
+                     ws->d_cbbox, ws->d_cbmeta, ws->d_K, ws->d_bpoff, d_bp, ws->d_symoff, ws->d_sym, ws->d_pcnt,
+                     ws->d_pmse, ws->d_flags, erres, gp.maxcol, gp.obuf_cap, dz);
+  HIP_OK(hipGetLastError());
+  // codeblocks per wavefront of the MQ kernel (lanes beyond LPW idle):
+  // fewer lanes means less divergence between codeblocks of unequal
+  // length, more means fewer waves; BWC_AMD_MQ_LPW = 1, 2, 4, 8, 16 or 32
+  const char *lpw_env = std::getenv("BWC_AMD_MQ_LPW");
+  const int lpw = lpw_env ? std::atoi(lpw_env) : 16;
+  const uint32_t nl = (uint32_t)list.size();
+#define MQ_LAUNCH(N)                                                                                              \
+  hipLaunchKernelGGL(t1_mq_kernel<N>, dim3((nl + N - 1) / N), dim3(64), 0, ws->stream, ws->d_list, (uint64_t)nl, \
+                     ws->d_cbmeta, ws->d_cbsb, ws->d_K, ws->d_symoff, ws->d_sym, ws->d_pcnt, ws->d_pmse,          \
+                     ws->d_doff, d_data, ws->d_Kmsbs, ws->d_Z, ws->d_L, ws->d_S, ws->d_snap, ws->d_mse,            \
+                     ws->d_flags, dz, ws->d_U)
+  switch (lpw) {
+    case 1: MQ_LAUNCH(1); break;
+    case 2: MQ_LAUNCH(2); break;
+    case 4: MQ_LAUNCH(4); break;
+    case 8: MQ_LAUNCH(8); break;
+    case 32: MQ_LAUNCH(32); break;
+    default: MQ_LAUNCH(16); break;
+  }
+#undef MQ_LAUNCH
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double *work, CblkRecords &rec,
+              uint16_t *slope_min, uint16_t *slope_max, int slot, const uint8_t *zlim) {
+  const uint64_t ncb = par.ncb;
+  rec.resize(ncb, c->control.nLayers);
+  if (!ncb) return 0;
+  CbTables tb;
+  build_tables(par, tb);
+  const std::vector<double> &sb = tb.sb;
+  const std::vector<uint32_t> &meta = tb.meta;
+  const std::vector<uint64_t> &qoff = tb.qoff, &doff = tb.doff;
+  const uint64_t qo = tb.qo, dof = tb.dof;
   size_t cap;
   if (ncb > ws->cap_cb) {
     void **tabs[] = {(void **)&ws->d_cbbox, (void **)&ws->d_cbsb, (void **)&ws->d_cbmeta, (void **)&ws->d_qoff,
                      (void **)&ws->d_doff,  (void **)&ws->d_K,    (void **)&ws->d_Kmsbs,  (void **)&ws->d_Z,
-                     (void **)&ws->d_L,     (void **)&ws->d_S,    (void **)&ws->d_snap,   (void **)&ws->d_mse};
+                     (void **)&ws->d_L,     (void **)&ws->d_S,    (void **)&ws->d_snap,   (void **)&ws->d_mse,
+                     (void **)&ws->d_U,     (void **)&ws->d_zlim};
     for (void **p : tabs)
       if (*p) {
         HIP_OK(hipFree(*p));
         *p = nullptr;
       }
     const uint64_t n = ncb + 1;
+    HIP_OK(hipMalloc((void **)&ws->d_U, n));
+    HIP_OK(hipMalloc((void **)&ws->d_zlim, n));
     HIP_OK(hipMalloc((void **)&ws->d_cbbox, 8 * n * sizeof(uint32_t)));
     HIP_OK(hipMalloc((void **)&ws->d_cbsb, 3 * n * sizeof(double)));
     HIP_OK(hipMalloc((void **)&ws->d_cbmeta, n * sizeof(uint32_t)));
@@ -1427,107 +1572,32 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
 
   // fast path: wavefront-per-codeblock symbol generation + few-lane MQ, when
   // every codeblock row fits one wavefront and its state fits LDS
-  // LDS: per codeblock the kernel packs P = 64 / pow2ceil(width) slices side
-  // by side, each with its own pass-output buffer, so size for the largest
-  // P times the largest per-slice bound
-  uint32_t maxcol = 0, obuf_cap = 0, maxcw = 0, P = 1;
-  for (uint64_t i = 0; i < ncb; ++i) {
-    const uint32_t *b = &par.cb_box[8 * i];
-    const uint32_t cw = b[4] - b[0], ch = b[5] - b[1], ns = (ch + 3) / 4;
-    const uint32_t ncol = cw * ns * (b[6] - b[2]) * (b[7] - b[3]);
-    maxcol = std::max(maxcol, ncol);
-    maxcw = std::max(maxcw, cw);
-    obuf_cap = std::max(obuf_cap, cw * ch * 2 + cw * ns * 3 + 16);
-    uint32_t gw = 1;
-    while (gw < cw) gw <<= 1;
-    const uint32_t nsl = (b[6] - b[2]) * (b[7] - b[3]);
-    if (gw <= 64) P = std::max(P, std::min(64 / gw, nsl));
-  }
-  const size_t gen_lds = (size_t)maxcol * 2 + (size_t)((maxcol + 1) / 2) * 8 + (size_t)P * obuf_cap;
+  const GenParams gp = gen_params(par);
   const char *force = std::getenv("BWC_AMD_T1");
-  const bool v2 = maxcw <= 64 && gen_lds <= 64 * 1024 && !(force && force[0] == '1');
+  const bool v2 = gp.maxcw <= 64 && gp.lds <= 64 * 1024 && !(force && force[0] == '1');
   const int erres = (int)c->control.error_resilience;
+  rec.U.assign(ncb, 0);
+  rec.Zfull.assign(ncb, 0);
   if (v2) {
-    std::vector<uint64_t> bpoff(ncb + 1);
-    uint64_t bo = 0;
-    for (uint64_t i = 0; i < ncb; ++i) {
-      bpoff[i] = bo;
-      bo += 64 * ((qoff[i + 1] - qoff[i]) / 4 + 1) / 2 + 64;
-    }
-    bpoff[ncb] = bo;
-    cap = ws->cap_bp;
-    CHK(ensure((void **)&ws->d_bp, &cap, bo));
-    ws->cap_bp = cap;
+    if (slot >= (int)ws->bp_slots.size()) ws->bp_slots.resize(slot + 1);
+    Workspace::BpSlot &bs = ws->bp_slots[slot];
+    cap = bs.cap_bp;
+    CHK(ensure((void **)&bs.bp, &cap, tb.bo));
+    bs.cap_bp = cap;
     cap = ws->cap_bpoff;
     CHK(ensure((void **)&ws->d_bpoff, &cap, (ncb + 1) * sizeof(uint64_t)));
     ws->cap_bpoff = cap;
-    HIP_OK(hipMemcpyAsync(ws->d_bpoff, bpoff.data(), (ncb + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, ws->stream));
+    HIP_OK(hipMemcpyAsync(ws->d_bpoff, tb.bpoff.data(), (ncb + 1) * sizeof(uint64_t), hipMemcpyHostToDevice,
+                          ws->stream));
     hipLaunchKernelGGL(bitplane_kernel, dim3((uint32_t)std::min<uint64_t>(ncb, 65535)), dim3(256), 0, ws->stream,
-                       work, W, H, D, ws->d_cbbox, ws->d_cbsb, ws->d_bpoff, ws->d_bp, ws->d_K, ncb);
+                       work, W, H, D, ws->d_cbbox, ws->d_cbsb, ws->d_bpoff, bs.bp, ws->d_K, ncb);
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(rec.K.data(), ws->d_K, ncb, hipMemcpyDeviceToHost, ws->stream));
     HIP_OK(hipStreamSynchronize(ws->stream));
-    // symbol stream bounds from the actual K: per plane one significance or
-    // refinement symbol per value, one sign per value overall, three run
-    // symbols per stripe column, four segmentation symbols
-    std::vector<uint64_t> symoff(ncb + 1), list;
-    uint64_t so = 0;
-    for (uint64_t i = 0; i < ncb; ++i) {
-      symoff[i] = so;
-      const uint32_t *b = &par.cb_box[8 * i];
-      const uint64_t nv = (uint64_t)(b[4] - b[0]) * (b[5] - b[1]) * (b[6] - b[2]) * (b[7] - b[3]);
-      const uint64_t ncol = (qoff[i + 1] - qoff[i]) / 4;
-      const uint64_t K = rec.K[i];
-      if (K) list.push_back(i);
-      so += (nv * (K + 1) + 3 * ncol * K + 4 * K + 32 + 15) & ~15ull;
-    }
-    symoff[ncb] = so;
-    cap = ws->cap_sym;
-    CHK(ensure((void **)&ws->d_sym, &cap, so + 64));
-    ws->cap_sym = cap;
-    cap = ws->cap_symoff;
-    CHK(ensure((void **)&ws->d_symoff, &cap, (ncb + 1) * sizeof(uint64_t)));
-    ws->cap_symoff = cap;
-    cap = ws->cap_pc;
-    CHK(ensure((void **)&ws->d_pcnt, &cap, (ncb + 1) * kMaxPasses * sizeof(uint32_t)));
-    ws->cap_pc = cap;
-    cap = ws->cap_pm;
-    CHK(ensure((void **)&ws->d_pmse, &cap, (ncb + 1) * kMaxPasses * sizeof(int64_t)));
-    ws->cap_pm = cap;
-    cap = ws->cap_list;
-    CHK(ensure((void **)&ws->d_list, &cap, (list.size() + 1) * sizeof(uint64_t)));
-    ws->cap_list = cap;
-    HIP_OK(hipMemcpyAsync(ws->d_symoff, symoff.data(), (ncb + 1) * sizeof(uint64_t), hipMemcpyHostToDevice,
-                          ws->stream));
-    if (!list.empty()) {
-      HIP_OK(hipMemcpyAsync(ws->d_list, list.data(), list.size() * sizeof(uint64_t), hipMemcpyHostToDevice,
-                            ws->stream));
-      hipLaunchKernelGGL(t1_gen_kernel, dim3((uint32_t)list.size()), dim3(64), gen_lds, ws->stream, ws->d_list,
-                         ws->d_cbbox, ws->d_cbmeta, ws->d_K, ws->d_bpoff, ws->d_bp, ws->d_symoff, ws->d_sym,
-                         ws->d_pcnt, ws->d_pmse, ws->d_flags, erres, maxcol, obuf_cap);
-      HIP_OK(hipGetLastError());
-      // codeblocks per wavefront of the MQ kernel (lanes beyond LPW idle):
-      // fewer lanes means less divergence between codeblocks of unequal
-      // length, more means fewer waves; BWC_AMD_MQ_LPW = 1, 2, 4, 8 or 16
-      const char *lpw_env = std::getenv("BWC_AMD_MQ_LPW");
-      const int lpw = lpw_env ? std::atoi(lpw_env) : 16;
-      const uint32_t nl = (uint32_t)list.size();
-#define MQ_LAUNCH(N)                                                                                              \
-  hipLaunchKernelGGL(t1_mq_kernel<N>, dim3((nl + N - 1) / N), dim3(64), 0, ws->stream, ws->d_list, (uint64_t)nl, \
-                     ws->d_cbmeta, ws->d_cbsb, ws->d_K, ws->d_symoff, ws->d_sym, ws->d_pcnt, ws->d_pmse,          \
-                     ws->d_doff, d_data, ws->d_Kmsbs, ws->d_Z, ws->d_L, ws->d_S, ws->d_snap, ws->d_mse,            \
-                     ws->d_flags)
-      switch (lpw) {
-        case 1: MQ_LAUNCH(1); break;
-        case 2: MQ_LAUNCH(2); break;
-        case 4: MQ_LAUNCH(4); break;
-        case 8: MQ_LAUNCH(8); break;
-        case 32: MQ_LAUNCH(32); break;
-        default: MQ_LAUNCH(16); break;
-      }
-#undef MQ_LAUNCH
-      HIP_OK(hipGetLastError());
-    }
+    std::vector<uint64_t> list;
+    for (uint64_t i = 0; i < ncb; ++i)
+      if (rec.K[i]) list.push_back(i);
+    CHK(v2_code(ws, c, par, tb, rec.K.data(), list, zlim, d_data, bs.bp, gp));
   } else {
     cap = ws->cap_q;
     CHK(ensure((void **)&ws->d_q, &cap, qo * sizeof(uint64_t)));
@@ -1551,6 +1621,7 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
                         ws->stream));
   HIP_OK(hipMemcpyAsync(rec.S.data(), ws->d_S, ncb * (kMaxPasses + 1) * sizeof(uint16_t), hipMemcpyDeviceToHost,
                         ws->stream));
+  if (v2) HIP_OK(hipMemcpyAsync(rec.U.data(), ws->d_U, ncb, hipMemcpyDeviceToHost, ws->stream));
   uint32_t hfl[4];
   HIP_OK(hipMemcpyAsync(hfl, ws->d_flags, sizeof hfl, hipMemcpyDeviceToHost, ws->stream));
   HIP_OK(hipStreamSynchronize(ws->stream));
@@ -1558,15 +1629,64 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
     std::snprintf(g_err, sizeof g_err, "tier-1 output exceeded a per-codeblock buffer (flags %u)", hfl[0]);
     return 1;
   }
-  // codeblocks without significant bits are never launched on the fast path
-  for (uint64_t i = 0; i < ncb; ++i)
+  for (uint64_t i = 0; i < ncb; ++i) {
+    rec.Zfull[i] = rec.K[i] ? (uint8_t)(3 * rec.K[i] - 2) : 0;
+    if (!v2) rec.U[i] = rec.Z[i];
+    // codeblocks without significant bits are never launched on the fast path
     if (rec.K[i] == 0) {
       rec.Z[i] = 0;
+      rec.U[i] = 0;
       rec.Kmsbs[i] = 0;
       rec.S[i * (kMaxPasses + 1)] = 0xFFFF;
     }
+  }
   if (hfl[1] < *slope_min) *slope_min = (uint16_t)hfl[1];
   if (hfl[2] > *slope_max) *slope_max = (uint16_t)hfl[2];
+  return 0;
+}
+
+// Re-code the codeblocks in `cbs` of slot `slot` (the parameter coded last
+// into that slot by t1_encode, bitplanes still on the device) to the pass
+// limits in `zlim` (ncb entries; those of other codeblocks are ignored), and
+// refresh their records.
+int t1_extend(Workspace *ws, const bwc_codec *c, const Param &par, CblkRecords &rec, int slot,
+              const std::vector<uint64_t> &cbs, const uint8_t *zlim) {
+  const uint64_t ncb = par.ncb;
+  if (cbs.empty()) return 0;
+  if (slot < 0 || slot >= (int)ws->bp_slots.size() || !ws->bp_slots[slot].bp || ncb > ws->cap_cb) {
+    std::snprintf(g_err, sizeof g_err, "no bitplanes kept for parameter slot %d", slot);
+    return 1;
+  }
+  CbTables tb;
+  build_tables(par, tb);
+  const GenParams gp = gen_params(par);
+  HIP_OK(hipMemcpyAsync(ws->d_cbbox, par.cb_box.data(), 8 * ncb * sizeof(uint32_t), hipMemcpyHostToDevice,
+                        ws->stream));
+  HIP_OK(hipMemcpyAsync(ws->d_cbsb, tb.sb.data(), 3 * ncb * sizeof(double), hipMemcpyHostToDevice, ws->stream));
+  HIP_OK(hipMemcpyAsync(ws->d_cbmeta, tb.meta.data(), ncb * sizeof(uint32_t), hipMemcpyHostToDevice, ws->stream));
+  HIP_OK(hipMemcpyAsync(ws->d_doff, tb.doff.data(), (ncb + 1) * sizeof(uint64_t), hipMemcpyHostToDevice,
+                        ws->stream));
+  HIP_OK(hipMemcpyAsync(ws->d_bpoff, tb.bpoff.data(), (ncb + 1) * sizeof(uint64_t), hipMemcpyHostToDevice,
+                        ws->stream));
+  HIP_OK(hipMemcpyAsync(ws->d_K, rec.K.data(), ncb, hipMemcpyHostToDevice, ws->stream));
+  hipLaunchKernelGGL(init_flags_kernel, dim3(1), dim3(64), 0, ws->stream, ws->d_flags);
+  CHK(v2_code(ws, c, par, tb, rec.K.data(), cbs, zlim, ws->d_datap[slot], ws->bp_slots[slot].bp, gp));
+  // the re-coded rows: Z, U, L, S (small copies; few codeblocks)
+  for (uint64_t i : cbs) {
+    HIP_OK(hipMemcpyAsync(&rec.Z[i], ws->d_Z + i, 1, hipMemcpyDeviceToHost, ws->stream));
+    HIP_OK(hipMemcpyAsync(&rec.U[i], ws->d_U + i, 1, hipMemcpyDeviceToHost, ws->stream));
+    HIP_OK(hipMemcpyAsync(&rec.L[i * kMaxPasses], ws->d_L + i * kMaxPasses, kMaxPasses * sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, ws->stream));
+    HIP_OK(hipMemcpyAsync(&rec.S[i * (kMaxPasses + 1)], ws->d_S + i * (kMaxPasses + 1),
+                          (kMaxPasses + 1) * sizeof(uint16_t), hipMemcpyDeviceToHost, ws->stream));
+  }
+  uint32_t hfl[4];
+  HIP_OK(hipMemcpyAsync(hfl, ws->d_flags, sizeof hfl, hipMemcpyDeviceToHost, ws->stream));
+  HIP_OK(hipStreamSynchronize(ws->stream));
+  if (hfl[0]) {
+    std::snprintf(g_err, sizeof g_err, "tier-1 output exceeded a per-codeblock buffer (flags %u)", hfl[0]);
+    return 1;
+  }
   return 0;
 }
 

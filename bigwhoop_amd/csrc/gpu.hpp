@@ -46,7 +46,16 @@ struct Workspace {
   uint8_t *d_packed = nullptr;
   uint64_t *d_gather = nullptr;
   size_t cap_cb = 0, cap_q = 0, cap_data = 0, cap_packed = 0, cap_gather = 0, cap_state = 0;
-  // fast tier-1 path: bitplanes, symbol streams, per-pass counts/distortion
+  // fast tier-1 path: bitplanes (kept per parameter slot until the next
+  // compression, so rate control can have codeblocks coded further),
+  // symbol streams, per-pass counts/distortion, pass limits, first
+  // unsettled pass per codeblock
+  struct BpSlot {
+    uint8_t *bp = nullptr;
+    size_t cap_bp = 0;
+  };
+  std::vector<BpSlot> bp_slots;
+  uint8_t *d_U = nullptr, *d_zlim = nullptr;
   uint8_t *d_bp = nullptr;
   uint64_t *d_bpoff = nullptr;
   uint8_t *d_sym = nullptr;
@@ -71,8 +80,13 @@ int normalize(Workspace *ws, double *work, uint64_t n, uint8_t Qm);
 int dwt_forward(Workspace *ws, const bwc_codec *c, const Param &par, double *work, int norm_qm = -1);
 int dwt_inverse(Workspace *ws, const bwc_codec *c, const Param &par, double *work);
 // quantise + tier-1 for all codeblocks of (tile, param); results land in rec
+// zlim: optional per-codeblock pass limits (host, par.ncb entries)
 int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double *work, CblkRecords &rec,
-              uint16_t *slope_min, uint16_t *slope_max, int slot);
+              uint16_t *slope_min, uint16_t *slope_max, int slot, const uint8_t *zlim = nullptr);
+// re-code codeblocks `cbs` of the parameter last coded into `slot` with new
+// pass limits (zlim has par.ncb entries) and refresh their records
+int t1_extend(Workspace *ws, const bwc_codec *c, const Param &par, CblkRecords &rec, int slot,
+              const std::vector<uint64_t> &cbs, const uint8_t *zlim);
 // gather each codeblock's prefix [0, len[i]) into rec.data / rec.data_off
 int gather(Workspace *ws, CblkRecords &rec, const std::vector<uint32_t> &len);
 int sync(Workspace *ws);

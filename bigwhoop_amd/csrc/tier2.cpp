@@ -6,6 +6,8 @@
 // estimate modes 2 = probe, 1 = final estimate, 0 = real header), :750-849,
 // :895-1254 (quality layers and the bisection); tagtree.c:310-373;
 // bitstream.c:277-363, :561-612; codestream.c:139-570, :1329-1413.
+#include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 
@@ -404,6 +406,35 @@ struct T2 {
     return ql;
   }
 
+  // Records coded only to a prefix of their passes (Z < Zfull): the layer's
+  // threshold T and cuts equal those of complete codings when
+  //  - the size estimate is strictly above the target at T - 1 and strictly
+  //    below it at T: estimates do not grow with the threshold, so any
+  //    bisection over an interval holding T - 1 returns T (a complete coding
+  //    has a slope <= T - 1, as its estimate changes between T - 1 and T);
+  //  - every prefix-coded codeblock cuts at T - 1 at least `margin` passes
+  //    before its first pass whose length may still change (U), so the
+  //    passes that decide its cuts at T - 1 and T are final.
+  // Otherwise the codeblocks are flagged (or the tile, `fallback`) for
+  // further coding.
+  SkipCheck *chk = nullptr;
+  void check_layer(uint16_t slope, int l, int64_t target, int64_t base) {
+    if (target > 0) {
+      const int64_t f0 = quality_layer(slope, l, 1) + base;
+      const int64_t fm = slope > 0 ? quality_layer((uint16_t)(slope - 1), l, 1) + base : INT64_MAX;
+      if (!(f0 < target && fm > target)) chk->fallback = true;
+    }
+    const uint16_t t = slope > 0 ? (uint16_t)(slope - 1) : 0;
+    for (int j = 0; j < c->info.nPar; ++j) {
+      CblkRecords &R = rec(j);
+      for (uint64_t i = 0; i < R.n; ++i) {
+        if (R.Z[i] >= R.Zfull[i]) continue;
+        const int z = cut_of(j, R, i, t);
+        if (z + chk->margin > (int)R.U[i]) chk->flagged[j].push_back(i);
+      }
+    }
+  }
+
   void rate_control() {
     int64_t est_tile = 0, est_ql = 0;
     const uint64_t tsz = tl->box.volume() * (uint64_t)c->info.nPar;
@@ -434,6 +465,7 @@ struct T2 {
       } else {
         slope = 0;
       }
+      if (chk) check_layer(slope, l, target, est_tile + (int64_t)hdr);
       lc = nullptr;
       est_ql = quality_layer(slope, l, 0);
       est_tile += est_ql + (int64_t)hdr;
@@ -452,9 +484,19 @@ struct T2 {
 
 // Rate control for one tile: fills cp (cut points per layer) and the packet
 // contribution flags / body sizes. Needs tile slope bounds and records.
-int t2_rate_control(bwc_codec *codec, Job *job, uint64_t tile) {
+int t2_rate_control(bwc_codec *codec, Job *job, uint64_t tile, SkipCheck *chk) {
   T2 t{codec, job, &job->plan.tiles[tile], tile, codec->control.nLayers};
+  if (chk) {
+    chk->fallback = false;
+    chk->flagged.assign(codec->info.nPar, {});
+    t.chk = chk;
+  }
   t.rate_control();
+  if (chk)
+    for (auto &f : chk->flagged) {
+      std::sort(f.begin(), f.end());
+      f.erase(std::unique(f.begin(), f.end()), f.end());
+    }
   return 0;
 }
 

@@ -80,6 +80,28 @@ size_t bwc_amd_encode_records(bwc_codec *codec, bwc_stream *stream);
    (host) [9] total; launches[0] = number of DWT kernel launches */
 int bwc_amd_last_timings(bwc_codec *codec, double ms[10], int launches[4]);
 
+/* Pass skipping (no counterpart in the reference; the codestream stays the
+   one bwc_compress writes without it). Tier-1 codes each codeblock only to a
+   pass limit; rate control then checks that its thresholds and cuts are
+   provably those of complete codings and codes flagged codeblocks further
+   until they are.
+   mode 0: off (default). mode 1: limit = `param` passes for every codeblock.
+   mode 2: limit = the network's predicted cut (bwc_amd_set_skip_model) plus
+   `param` passes. Returns 0 on success. */
+int bwc_amd_set_pass_skip(bwc_codec *codec, int mode, int param);
+/* Weights of the pass-limit network, fp32, in the order W1[64][16], b1[64],
+   W2[64][64], b2[64], w3[64], b3, then 16 feature means and 16 feature
+   scales (n = 5313 + 32 = 5345 values). */
+int bwc_amd_set_skip_model(bwc_codec *codec, const float *w, size_t n);
+/* Statistics of the last bwc_compress: [0] codeblocks coded, [1] passes
+   coded (all rounds), [2] passes of complete codings, [3] extra
+   rate-control rounds, [4] codeblocks coded further, [5] fallbacks to
+   complete coding. */
+int bwc_amd_skip_stats(bwc_codec *codec, uint64_t stats[8]);
+/* Last bwc_compress: number of passes each codeblock of (tile, param)
+   contributes to the last quality layer (ncblk entries). */
+int bwc_amd_cblk_cuts(bwc_codec *codec, uint64_t tile, uint8_t param, int16_t *cuts);
+
 #ifdef __cplusplus
 }
 #endif
