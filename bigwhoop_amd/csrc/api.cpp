@@ -112,6 +112,16 @@ int rate_control_prefix(bwc_codec *c, Job *job, gpu::Workspace *ws, uint64_t ti)
   chk.margin = 3;
   for (int round = 0;; ++round) {
     reset_tile_t2(c, job, ti);
+    // every codeblock coded completely: the outcome is exact as it stands
+    uint64_t ntrunc = 0;
+    for (int p = 0; p < np; ++p) {
+      const CblkRecords &R = job->rec[ti * np + p];
+      for (uint64_t i = 0; i < R.n; ++i) ntrunc += R.Z[i] < R.Zfull[i];
+    }
+    if (!ntrunc) {
+      t2_rate_control(c, job, ti);
+      return 0;
+    }
     t2_rate_control(c, job, ti, &chk);
     uint64_t nflag = 0;
     for (auto &f : chk.flagged) nflag += f.size();

@@ -1671,14 +1671,28 @@ int t1_extend(Workspace *ws, const bwc_codec *c, const Param &par, CblkRecords &
   HIP_OK(hipMemcpyAsync(ws->d_K, rec.K.data(), ncb, hipMemcpyHostToDevice, ws->stream));
   hipLaunchKernelGGL(init_flags_kernel, dim3(1), dim3(64), 0, ws->stream, ws->d_flags);
   CHK(v2_code(ws, c, par, tb, rec.K.data(), cbs, zlim, ws->d_datap[slot], ws->bp_slots[slot].bp, gp));
-  // the re-coded rows: Z, U, L, S (small copies; few codeblocks)
-  for (uint64_t i : cbs) {
-    HIP_OK(hipMemcpyAsync(&rec.Z[i], ws->d_Z + i, 1, hipMemcpyDeviceToHost, ws->stream));
-    HIP_OK(hipMemcpyAsync(&rec.U[i], ws->d_U + i, 1, hipMemcpyDeviceToHost, ws->stream));
-    HIP_OK(hipMemcpyAsync(&rec.L[i * kMaxPasses], ws->d_L + i * kMaxPasses, kMaxPasses * sizeof(uint32_t),
-                          hipMemcpyDeviceToHost, ws->stream));
-    HIP_OK(hipMemcpyAsync(&rec.S[i * (kMaxPasses + 1)], ws->d_S + i * (kMaxPasses + 1),
-                          (kMaxPasses + 1) * sizeof(uint16_t), hipMemcpyDeviceToHost, ws->stream));
+  // the re-coded rows: Z, U, L, S. The device arrays hold other parameters'
+  // rows as well, so only the listed rows reach the records: row copies for
+  // a few codeblocks, else whole arrays through host staging.
+  const bool bulk = cbs.size() > 64;
+  std::vector<uint8_t> hz, hu;
+  std::vector<uint32_t> hl;
+  std::vector<uint16_t> hs;
+  if (bulk) {
+    hz.resize(ncb), hu.resize(ncb), hl.resize(ncb * kMaxPasses), hs.resize(ncb * (kMaxPasses + 1));
+    HIP_OK(hipMemcpyAsync(hz.data(), ws->d_Z, ncb, hipMemcpyDeviceToHost, ws->stream));
+    HIP_OK(hipMemcpyAsync(hu.data(), ws->d_U, ncb, hipMemcpyDeviceToHost, ws->stream));
+    HIP_OK(hipMemcpyAsync(hl.data(), ws->d_L, hl.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, ws->stream));
+    HIP_OK(hipMemcpyAsync(hs.data(), ws->d_S, hs.size() * sizeof(uint16_t), hipMemcpyDeviceToHost, ws->stream));
+  } else {
+    for (uint64_t i : cbs) {
+      HIP_OK(hipMemcpyAsync(&rec.Z[i], ws->d_Z + i, 1, hipMemcpyDeviceToHost, ws->stream));
+      HIP_OK(hipMemcpyAsync(&rec.U[i], ws->d_U + i, 1, hipMemcpyDeviceToHost, ws->stream));
+      HIP_OK(hipMemcpyAsync(&rec.L[i * kMaxPasses], ws->d_L + i * kMaxPasses, kMaxPasses * sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, ws->stream));
+      HIP_OK(hipMemcpyAsync(&rec.S[i * (kMaxPasses + 1)], ws->d_S + i * (kMaxPasses + 1),
+                            (kMaxPasses + 1) * sizeof(uint16_t), hipMemcpyDeviceToHost, ws->stream));
+    }
   }
   uint32_t hfl[4];
   HIP_OK(hipMemcpyAsync(hfl, ws->d_flags, sizeof hfl, hipMemcpyDeviceToHost, ws->stream));
@@ -1687,6 +1701,13 @@ int t1_extend(Workspace *ws, const bwc_codec *c, const Param &par, CblkRecords &
     std::snprintf(g_err, sizeof g_err, "tier-1 output exceeded a per-codeblock buffer (flags %u)", hfl[0]);
     return 1;
   }
+  if (bulk)
+    for (uint64_t i : cbs) {
+      rec.Z[i] = hz[i];
+      rec.U[i] = hu[i];
+      std::memcpy(&rec.L[i * kMaxPasses], &hl[i * kMaxPasses], kMaxPasses * sizeof(uint32_t));
+      std::memcpy(&rec.S[i * (kMaxPasses + 1)], &hs[i * (kMaxPasses + 1)], (kMaxPasses + 1) * sizeof(uint16_t));
+    }
   return 0;
 }
 
