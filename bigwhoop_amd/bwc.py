@@ -102,6 +102,16 @@ def lib():
             "bwc_amd_set_skip_model": (C.c_int, [vp, C.POINTER(C.c_float), C.c_size_t]),
             "bwc_amd_skip_stats": (C.c_int, [vp, C.POINTER(u64)]),
             "bwc_amd_cblk_cuts": (C.c_int, [vp, u64, u8, C.POINTER(C.c_int16)]),
+            "bwc_amd_collect_features": (C.c_int, [vp, C.c_int]),
+            "bwc_amd_cblk_features": (C.c_int, [vp, u64, u8, C.POINTER(C.c_float)]),
+            "bwc_amd_skipnet_train": (C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float), u64,
+                                                C.POINTER(C.c_float), C.c_int, C.c_float, u64,
+                                                C.POINTER(C.c_float)]),
+            "bwc_amd_skipnet_grad": (C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float), u64,
+                                               C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                               C.POINTER(C.c_float)]),
+            "bwc_amd_skipnet_predict": (C.c_int, [C.POINTER(C.c_float), u64, C.POINTER(C.c_float),
+                                                  C.POINTER(C.c_float)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -275,6 +285,15 @@ class Codec:
         out = np.zeros(self.ncblk(tile, param), np.int16)
         if lib().bwc_amd_cblk_cuts(self.h, tile, param, _p(out, C.c_int16)):
             raise RuntimeError("bwc_amd_cblk_cuts failed")
+        return out
+
+    def collect_features(self, on: bool = True):
+        lib().bwc_amd_collect_features(self.h, 1 if on else 0)
+
+    def features(self, tile=0, param=0):
+        out = np.zeros((self.ncblk(tile, param), 16), np.float32)
+        if lib().bwc_amd_cblk_features(self.h, tile, param, _p(out, C.c_float)):
+            raise RuntimeError("bwc_amd_cblk_features failed: " + last_error())
         return out
 
     # ---- single stages (bwc_amd.h) ----

@@ -500,13 +500,23 @@ size_t bwc_compress(bwc_codec *const codec, bwc_stream *const stream) {
       gpu::record(ws, 4);
       {
         CblkRecords &R = job->rec[ti * codec->info.nPar + p];
-        std::vector<uint8_t> zl;
-        if (job->skip_mode == 1) zl.assign(par.ncb, (uint8_t)std::min(255, std::max(1, job->skip_param)));
-        if (gpu::t1_encode(ws, codec, par, ws->d_work, R, &tl.slope_min, &tl.slope_max, p,
-                           zl.empty() ? nullptr : zl.data()))
-          goto fail;
-        if (zl.empty()) zl.assign(par.ncb, 255);
+        std::vector<uint8_t> zl(par.ncb, 255);
+        gpu::PassLimits pl;
+        if (job->skip_mode == 1) {
+          std::fill(zl.begin(), zl.end(), (uint8_t)std::min(255, std::max(1, job->skip_param)));
+          pl.fixed = zl.data();
+        } else if (job->skip_mode == 2) {
+          pl.model = job->skip_w.data();
+          pl.margin = job->skip_param;
+          pl.out = zl.data();
+        }
+        pl.features = job->collect_features;
+        if (gpu::t1_encode(ws, codec, par, ws->d_work, R, &tl.slope_min, &tl.slope_max, p, pl)) goto fail;
         R.zlim = zl;
+        if (job->collect_features) {
+          R.feat.resize(par.ncb * 16);
+          if (par.ncb && gpu::last_features(ws, par.ncb, R.feat.data())) goto fail;
+        }
         for (uint64_t i = 0; i < R.n; ++i) {
           if (!R.K[i]) continue;
           job->skip_stats[0] += 1;
@@ -776,6 +786,54 @@ int bwc_amd_skip_stats(bwc_codec *codec, uint64_t stats[8]) {
   Job *job = job_of(codec);
   if (!job) return 1;
   std::memcpy(stats, job->skip_stats, sizeof job->skip_stats);
+  return 0;
+}
+
+int bwc_amd_collect_features(bwc_codec *codec, int on) {
+  Job *job = job_of(codec);
+  if (!job) return 1;
+  job->collect_features = on != 0;
+  return 0;
+}
+
+int bwc_amd_cblk_features(bwc_codec *codec, uint64_t tile, uint8_t param, float *feat) {
+  Job *job = job_of(codec);
+  if (!job || tile >= job->plan.tiles.size() || param >= codec->info.nPar) return 1;
+  const uint64_t r = tile * codec->info.nPar + param;
+  if (r >= job->rec.size() || job->rec[r].feat.size() != job->rec[r].n * 16) {
+    set_err("no features collected for this tile and parameter");
+    return 1;
+  }
+  std::memcpy(feat, job->rec[r].feat.data(), job->rec[r].feat.size() * sizeof(float));
+  return 0;
+}
+
+int bwc_amd_skipnet_train(const float *feat, const float *target, uint64_t n, float *weights, int epochs, float lr,
+                          uint64_t seed, float *loss) {
+  char e[256] = {0};
+  if (gpu::skipnet_fit(feat, target, n, weights, epochs, lr, seed, loss, e, sizeof e)) {
+    set_err(e);
+    return 1;
+  }
+  return 0;
+}
+
+int bwc_amd_skipnet_grad(const float *feat, const float *target, uint64_t n, const float *weights, float *grad,
+                         float *loss) {
+  char e[256] = {0};
+  if (gpu::skipnet_grad(feat, target, n, weights, grad, loss, e, sizeof e)) {
+    set_err(e);
+    return 1;
+  }
+  return 0;
+}
+
+int bwc_amd_skipnet_predict(const float *feat, uint64_t n, const float *weights, float *y) {
+  char e[256] = {0};
+  if (gpu::skipnet_predict(feat, n, weights, y, e, sizeof e)) {
+    set_err(e);
+    return 1;
+  }
   return 0;
 }
 

@@ -119,6 +119,8 @@ struct CblkRecords {
   std::vector<uint64_t> src_off;
   // pass limits the codeblocks were coded with (pass skipping)
   std::vector<uint8_t> zlim;
+  // pass-limit network features (16 per codeblock) when collected
+  std::vector<float> feat;
   void resize(uint64_t ncb, int nlayers);
 };
 
@@ -147,6 +149,7 @@ struct Job {
   // (skip_param passes), 2 learned limit (skip_param margin passes)
   int skip_mode = 0, skip_param = 0;
   std::vector<float> skip_w;        // network weights, fp32 (mode 2)
+  bool collect_features = false;    // keep the network's features per codeblock
   // last compression: [0] codeblocks coded, [1] passes coded (all rounds),
   // [2] passes of complete codings, [3] extra rate-control rounds,
   // [4] codeblocks coded further, [5] fallbacks to complete coding

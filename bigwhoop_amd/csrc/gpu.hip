@@ -44,7 +44,8 @@ Workspace::~Workspace() {
   (void)hipSetDevice(device);
   void *ptrs[] = {d_input, d_work,  d_work2, d_red,  d_norm, d_cbbox, d_cbsb, d_cbmeta, d_qoff,   d_doff, d_q,
                   d_state, d_data,  d_K,    d_Kmsbs, d_Z,    d_L,    d_S,      d_snap,   d_mse,  d_flags,
-                  d_packed, d_gather, d_bp, d_bpoff, d_sym, d_symoff, d_pcnt, d_pmse, d_list, d_U, d_zlim};
+                  d_packed, d_gather, d_bp, d_bpoff, d_sym, d_symoff, d_pcnt, d_pmse, d_list, d_U, d_zlim,
+                  d_feat,  d_tstat, d_skipw};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (auto &b : bp_slots)
@@ -1073,7 +1074,7 @@ __global__ void __launch_bounds__(64) t1_encode_kernel(
     return;
   }
   const uint32_t meta = cbmeta[cb];
-  const uint32_t hb = meta & 0xFF, Kmax = meta >> 8;
+  const uint32_t hb = meta & 0xFF, Kmax = (meta >> 8) & 0xFF;
   const int sidx = (hb == 1 || hb == 5 || hb == 9 || hb == 13) ? 1 : ((hb & 3) == 3 ? 2 : 0);
   const uint8_t *s2c = c_sig[sidx];
   Blk B;
@@ -1218,12 +1219,27 @@ __device__ void finish_block(uint8_t *buf, uint64_t cap, const uint64_t *sn, con
 // quantisation straight into bitplanes for the fast tier-1 path
 // (tier1.c:630-728): per codeblock, plane b holds one nibble per stripe
 // column (row 0 in bit 3), two columns per byte; plane 63 holds the signs.
+// Per-codeblock features for the pass-limit network (written when `feat` is
+// set; 16 floats per codeblock, entry 15 filled later from tile statistics):
+//  0 K/64   1 (Kmax-K)/16   2 resolution/8   3 highpass axes/4
+//  4 mean magnitude bit length/64   5 log2(mean q + 1)/64
+//  6 log2(rms q + 1)/64   7..12 fraction of values whose bit length is at
+//  least K - {0,1,2,4,6,9}   13 log2(gain * step^2)/64   14 log2(bits/value)
+// and in tstat[0..1] the sum and count of log2(rms q + 1) + log2(gain *
+// step^2)/2 (the codeblock's distortion-weighted level).
 __global__ void bitplane_kernel(const double *__restrict__ work, uint64_t W, uint64_t H, uint64_t D,
                                 const uint32_t *__restrict__ cbbox, const double *__restrict__ cbsb,
                                 const uint64_t *__restrict__ bpoff, uint8_t *__restrict__ bp,
-                                uint8_t *__restrict__ Kout, uint64_t ncb) {
+                                uint8_t *__restrict__ Kout, uint64_t ncb, const uint32_t *__restrict__ cbmeta,
+                                float *__restrict__ feat, float *__restrict__ tstat, float rate) {
   __shared__ int smax[256];
+  __shared__ uint32_t hist[65];
+  __shared__ double s1[256], s2[256];
   for (uint64_t cb = blockIdx.x; cb < ncb; cb += gridDim.x) {
+    if (feat)
+      for (int i = threadIdx.x; i < 65; i += blockDim.x) hist[i] = 0;
+    __syncthreads();
+    double sq = 0, sqq = 0;
     const uint32_t *b = cbbox + 8 * cb;
     const uint64_t cw = b[4] - b[0], ch = b[5] - b[1], cd = b[6] - b[2], ct = b[7] - b[3];
     const uint64_t ns = (ch + 3) / 4;
@@ -1249,6 +1265,12 @@ __global__ void bitplane_kernel(const double *__restrict__ work, uint64_t W, uin
               qq = x86_to_u64(__longlong_as_double((long long)raw) / step);
               const int nb = qq ? 64 - __clzll((long long)qq) : 0;
               kb = nb > kb ? nb : kb;
+              if (feat) {
+                atomicAdd(&hist[nb], 1u);
+                const double dq = (double)qq;
+                sq += dq;
+                sqq += dq * dq;
+              }
             }
           }
           q[4 * h + r] = qq;
@@ -1263,17 +1285,58 @@ __global__ void bitplane_kernel(const double *__restrict__ work, uint64_t W, uin
       planes[63 * ncolh + pr] = (uint8_t)sg;
     }
     smax[threadIdx.x] = kb;
+    s1[threadIdx.x] = sq;
+    s2[threadIdx.x] = sqq;
     __syncthreads();
     for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-      if ((int)threadIdx.x < s) smax[threadIdx.x] = max(smax[threadIdx.x], smax[threadIdx.x + s]);
+      if ((int)threadIdx.x < s) {
+        smax[threadIdx.x] = max(smax[threadIdx.x], smax[threadIdx.x + s]);
+        s1[threadIdx.x] += s1[threadIdx.x + s];
+        s2[threadIdx.x] += s2[threadIdx.x + s];
+      }
       __syncthreads();
     }
-    if (threadIdx.x == 0) Kout[cb] = (uint8_t)smax[0];
+    if (threadIdx.x == 0) {
+      const int K = smax[0];
+      Kout[cb] = (uint8_t)K;
+      if (feat) {
+        const uint32_t meta = cbmeta[cb];
+        const uint32_t hb = meta & 0xFF, Kmax = (meta >> 8) & 0xFF, res = (meta >> 16) & 0xFF;
+        const double nv = (double)(cw * ch * cd * ct);
+        double sb = 0;
+        uint32_t cum[6] = {0, 0, 0, 0, 0, 0};
+        for (int nb = 0; nb <= 64; ++nb) sb += (double)nb * hist[nb];
+        const int offs[6] = {0, 1, 2, 4, 6, 9};
+        for (int o = 0; o < 6; ++o)
+          for (int nb = K - offs[o] > 1 ? K - offs[o] : 1; nb <= 64; ++nb) cum[o] += hist[nb];
+        const double gain = cbsb[3 * cb + 2], stp = cbsb[3 * cb + 1];
+        const float lvl_w = (float)(log2(gain) + 2.0 * log2(stp));
+        const float lrms = (float)log2(sqrt(s2[0] / nv) + 1.0);
+        float *f = feat + 16 * cb;
+        f[0] = K / 64.0f;
+        f[1] = ((float)Kmax - (float)K) / 16.0f;
+        f[2] = res / 8.0f;
+        f[3] = __popc(hb) / 4.0f;
+        f[4] = (float)(sb / nv) / 64.0f;
+        f[5] = (float)log2(s1[0] / nv + 1.0) / 64.0f;
+        f[6] = lrms / 64.0f;
+        for (int o = 0; o < 6; ++o) f[7 + o] = (float)(cum[o] / nv);
+        f[13] = lvl_w / 64.0f;
+        f[14] = log2f(rate);
+        f[15] = lrms + 0.5f * lvl_w;   // made relative to the tile mean by the network kernel
+        atomicAdd(&tstat[0], f[15]);
+        atomicAdd(&tstat[1], 1.0f);
+      }
+    }
     __syncthreads();
   }
 }
 
 #include "t1v2.inc"
+
+__global__ void zero_f32_kernel(float *p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.f;
+}
 
 __global__ void init_flags_kernel(uint32_t *flags) {
   if (threadIdx.x == 0) {
@@ -1389,7 +1452,7 @@ static void build_tables(const Param &par, CbTables &t) {
     t.sb[3 * i] = s.step_eff;
     t.sb[3 * i + 1] = s.step;
     t.sb[3 * i + 2] = s.gain;
-    t.meta[i] = (uint32_t)s.hb | ((uint32_t)s.Kmax << 8);
+    t.meta[i] = (uint32_t)s.hb | ((uint32_t)s.Kmax << 8) | ((par.cb_sbref[i] >> 8) << 16);
     const uint64_t cw = b[4] - b[0], ch = b[5] - b[1], cd = b[6] - b[2], ct = b[7] - b[3];
     const uint64_t ncol = cw * ((ch + 3) / 4) * cd * ct;
     t.qoff[i] = t.qo;
@@ -1512,7 +1575,9 @@ static int v2_code(Workspace *ws, const bwc_codec *c, const Param &par, const Cb
 }
 
 int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double *work, CblkRecords &rec,
-              uint16_t *slope_min, uint16_t *slope_max, int slot, const uint8_t *zlim) {
+              uint16_t *slope_min, uint16_t *slope_max, int slot, const PassLimits &lim) {
+  const uint8_t *zlim = lim.fixed;
+  std::vector<uint8_t> zpred;
   const uint64_t ncb = par.ncb;
   rec.resize(ncb, c->control.nLayers);
   if (!ncb) return 0;
@@ -1589,11 +1654,39 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
     ws->cap_bpoff = cap;
     HIP_OK(hipMemcpyAsync(ws->d_bpoff, tb.bpoff.data(), (ncb + 1) * sizeof(uint64_t), hipMemcpyHostToDevice,
                           ws->stream));
+    float *d_feat = nullptr;
+    if (lim.model || lim.features) {
+      cap = ws->cap_feat;
+      CHK(ensure((void **)&ws->d_feat, &cap, (ncb + 1) * 16 * sizeof(float)));
+      ws->cap_feat = cap;
+      if (!ws->d_tstat) HIP_OK(hipMalloc((void **)&ws->d_tstat, 4 * sizeof(float)));
+      if (!ws->d_skipw) HIP_OK(hipMalloc((void **)&ws->d_skipw, kSkipModelSize * sizeof(float)));
+      if (lim.model)
+        HIP_OK(hipMemcpyAsync(ws->d_skipw, lim.model, kSkipModelSize * sizeof(float), hipMemcpyHostToDevice,
+                              ws->stream));
+      hipLaunchKernelGGL(zero_f32_kernel, dim3(1), dim3(64), 0, ws->stream, ws->d_tstat, 4);
+      d_feat = ws->d_feat;
+    }
+    const int nl = c->control.nLayers;
+    const float rate = nl > 0 && c->control.bitrate ? c->control.bitrate[nl - 1] : 1.0f;
     hipLaunchKernelGGL(bitplane_kernel, dim3((uint32_t)std::min<uint64_t>(ncb, 65535)), dim3(256), 0, ws->stream,
-                       work, W, H, D, ws->d_cbbox, ws->d_cbsb, ws->d_bpoff, bs.bp, ws->d_K, ncb);
+                       work, W, H, D, ws->d_cbbox, ws->d_cbsb, ws->d_bpoff, bs.bp, ws->d_K, ncb, ws->d_cbmeta,
+                       d_feat, ws->d_tstat, rate > 0 ? rate : 1.0f);
     HIP_OK(hipGetLastError());
+    if (lim.model) {
+      char e[200];
+      if (skipnet_infer(ws->stream, ws->d_feat, ncb, ws->d_skipw, ws->d_tstat, ws->d_K, lim.margin, ws->d_zlim,
+                        nullptr, e, sizeof e)) {
+        std::snprintf(g_err, sizeof g_err, "%s", e);
+        return 1;
+      }
+      zpred.resize(ncb);
+      HIP_OK(hipMemcpyAsync(zpred.data(), ws->d_zlim, ncb, hipMemcpyDeviceToHost, ws->stream));
+      zlim = zpred.data();
+    }
     HIP_OK(hipMemcpyAsync(rec.K.data(), ws->d_K, ncb, hipMemcpyDeviceToHost, ws->stream));
     HIP_OK(hipStreamSynchronize(ws->stream));
+    if (lim.out && zlim) std::memcpy(lim.out, zlim, ncb);
     std::vector<uint64_t> list;
     for (uint64_t i = 0; i < ncb; ++i)
       if (rec.K[i]) list.push_back(i);
@@ -1801,6 +1894,20 @@ double elapsed(Workspace *ws, int a, int b) {
   float ms = 0;
   if (hipEventElapsedTime(&ms, ws->ev[a], ws->ev[b]) != hipSuccess) return -1;
   return ms;
+}
+
+int last_features(Workspace *ws, uint64_t ncb, float *out) {
+  if (!ws->d_feat || !ws->d_tstat || ws->cap_feat < ncb * 16 * sizeof(float)) {
+    std::snprintf(g_err, sizeof g_err, "no codeblock features were computed");
+    return 1;
+  }
+  float ts[4];
+  HIP_OK(hipMemcpyAsync(out, ws->d_feat, ncb * 16 * sizeof(float), hipMemcpyDeviceToHost, ws->stream));
+  HIP_OK(hipMemcpyAsync(ts, ws->d_tstat, sizeof ts, hipMemcpyDeviceToHost, ws->stream));
+  HIP_OK(hipStreamSynchronize(ws->stream));
+  const float mean = ts[1] > 0.f ? ts[0] / ts[1] : 0.f;
+  for (uint64_t i = 0; i < ncb; ++i) out[16 * i + 15] -= mean;
+  return 0;
 }
 
 int read_norm(Workspace *ws, double mm[4]) {

@@ -56,6 +56,10 @@ struct Workspace {
   };
   std::vector<BpSlot> bp_slots;
   uint8_t *d_U = nullptr, *d_zlim = nullptr;
+  // pass-limit network: features (16 per codeblock), tile statistics,
+  // weights on the device
+  float *d_feat = nullptr, *d_tstat = nullptr, *d_skipw = nullptr;
+  size_t cap_feat = 0;
   uint8_t *d_bp = nullptr;
   uint64_t *d_bpoff = nullptr;
   uint8_t *d_sym = nullptr;
@@ -80,9 +84,35 @@ int normalize(Workspace *ws, double *work, uint64_t n, uint8_t Qm);
 int dwt_forward(Workspace *ws, const bwc_codec *c, const Param &par, double *work, int norm_qm = -1);
 int dwt_inverse(Workspace *ws, const bwc_codec *c, const Param &par, double *work);
 // quantise + tier-1 for all codeblocks of (tile, param); results land in rec
-// zlim: optional per-codeblock pass limits (host, par.ncb entries)
+// Pass limits for t1_encode: none, given per codeblock (host, par.ncb
+// entries), or predicted on the device by the pass-limit network from the
+// bitplane kernel's features (weights: kSkipModelSize floats, host).
+struct PassLimits {
+  const uint8_t *fixed = nullptr;
+  const float *model = nullptr;
+  int margin = 0;
+  uint8_t *out = nullptr;   // optional: the limits used (host, par.ncb)
+  bool features = false;    // compute the network's features even without a model
+};
 int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double *work, CblkRecords &rec,
-              uint16_t *slope_min, uint16_t *slope_max, int slot, const uint8_t *zlim = nullptr);
+              uint16_t *slope_min, uint16_t *slope_max, int slot, const PassLimits &lim = PassLimits());
+// features of the last t1_encode with a model (par.ncb x 16, feature 15
+// relative to the tile mean)
+int last_features(Workspace *ws, uint64_t ncb, float *out);
+
+// pass-limit network kernels (skipnet.hip)
+int skipnet_infer(hipStream_t s, const float *d_feat, uint64_t n, const float *d_w, const float *d_tstat,
+                  const uint8_t *d_K, int margin, uint8_t *d_lim, float *d_y, char *err, size_t errlen);
+size_t skipnet_train_lds();
+int skipnet_train(hipStream_t s, const float *d_feat, const float *d_tgt, const uint32_t *d_order, uint64_t n,
+                  uint64_t start, int steps, float lr, int t0, float *d_w, float *d_m, float *d_v, float *d_g,
+                  float *d_loss, int mode, char *err, size_t errlen);
+// host-buffer drivers of the network kernels on the current device
+int skipnet_fit(const float *feat, const float *tgt, uint64_t n, float *w, int epochs, float lr, uint64_t seed,
+                float *loss_out, char *err, size_t errlen);
+int skipnet_grad(const float *feat, const float *tgt, uint64_t n, const float *w, float *grad, float *loss,
+                 char *err, size_t errlen);
+int skipnet_predict(const float *feat, uint64_t n, const float *w, float *y, char *err, size_t errlen);
 // re-code codeblocks `cbs` of the parameter last coded into `slot` with new
 // pass limits (zlim has par.ncb entries) and refresh their records
 int t1_extend(Workspace *ws, const bwc_codec *c, const Param &par, CblkRecords &rec, int slot,

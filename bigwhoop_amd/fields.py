@@ -84,3 +84,33 @@ def grf(shape, slope=-5.0 / 3.0, seed=0, mean=10.0, std=1.0, dtype=np.float64):
         out *= std / sd
     out += mean
     return np.ascontiguousarray(out.astype(dtype))
+
+
+def grf_device(n: int, slope=-5.0 / 3.0, seed=1, mean=10.0, std=1.0):
+    """3-D n^3 float64 Gaussian random field with E(k) ~ k**slope, made in HBM
+    with torch's FFT (seeded white noise filtered by k**((slope - 2) / 2)),
+    rescaled to ``mean`` and ``std``. Returns a contiguous CUDA/HIP tensor
+    (numpy order z, y, x)."""
+    import torch
+
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    white = torch.randn((n, n, n), generator=g, device="cuda", dtype=torch.float64)
+    spec = torch.fft.rfftn(white)
+    del white
+    f = torch.fft.fftfreq(n, d=1.0 / n, device="cuda", dtype=torch.float64)
+    fr = torch.fft.rfftfreq(n, d=1.0 / n, device="cuda", dtype=torch.float64)
+    k2 = f[:, None, None] ** 2 + f[None, :, None] ** 2 + fr[None, None, :] ** 2
+    kmag = torch.sqrt(k2)
+    del k2
+    filt = torch.where(kmag > 0, kmag.clamp_min(1e-30) ** ((slope - 2.0) / 2.0), torch.zeros_like(kmag))
+    del kmag
+    spec *= filt
+    del filt
+    out = torch.fft.irfftn(spec, s=(n, n, n))
+    del spec
+    out -= out.mean()
+    out *= std / out.std()
+    out += mean
+    torch.cuda.synchronize()
+    return out.contiguous()

@@ -101,6 +101,24 @@ int bwc_amd_skip_stats(bwc_codec *codec, uint64_t stats[8]);
 /* Last bwc_compress: number of passes each codeblock of (tile, param)
    contributes to the last quality layer (ncblk entries). */
 int bwc_amd_cblk_cuts(bwc_codec *codec, uint64_t tile, uint8_t param, int16_t *cuts);
+/* Keep the pass-limit network's 16 features per codeblock during
+   bwc_compress (on != 0), and read them for (tile, param) afterwards
+   (ncblk x 16 floats; feature 15 relative to the tile mean). */
+int bwc_amd_collect_features(bwc_codec *codec, int on);
+int bwc_amd_cblk_features(bwc_codec *codec, uint64_t tile, uint8_t param, float *feat);
+/* Pass-limit network on the GPU (BF16 matrix cores, fp32 accumulation).
+   Targets are cuts in units of 8 passes; weights as for
+   bwc_amd_set_skip_model, feature means and scales set by the caller.
+   train: `epochs` passes of Adam (batches of 256, order shuffled from
+   `seed`), weights updated in place, mean batch loss per epoch in loss.
+   grad: loss gradient of the first n <= 256 samples (5313 values, the
+   weight order without means and scales), no update.
+   predict: network outputs (units of 8 passes). */
+int bwc_amd_skipnet_train(const float *feat, const float *target, uint64_t n, float *weights, int epochs, float lr,
+                          uint64_t seed, float *loss);
+int bwc_amd_skipnet_grad(const float *feat, const float *target, uint64_t n, const float *weights, float *grad,
+                         float *loss);
+int bwc_amd_skipnet_predict(const float *feat, uint64_t n, const float *weights, float *y);
 
 #ifdef __cplusplus
 }
