@@ -78,3 +78,40 @@ def test_fixed_limit_matches_plain(spec, limit):
     finally:
         plain.close()
         skip.close()
+
+
+@pytest.mark.parametrize("margin", [0, 3, 6])
+@pytest.mark.parametrize("spec", FIELDS, ids=[f["name"] for f in FIELDS])
+def test_learned_limit_matches_plain(spec, margin):
+    from bigwhoop_amd import skipnet
+    _say("skip learned", margin, spec["name"])
+    x = _field(spec)
+    kw = dict(rate=spec["rate"], decomp=(4, 4, 4, 0), cb=(5, 5, 3, 0))
+    plain = bwc.Codec(spec["shape"], **kw)
+    skip = bwc.Codec(spec["shape"], **kw)
+    try:
+        ref = plain.compress(x)
+        skip.set_skip_model(skipnet.default_weights())
+        skip.set_pass_skip(2, margin)
+        got = skip.compress(x)
+        st = skip.skip_stats()
+        assert got == ref, st
+        assert st["passes_coded"] < st["passes_complete"], st
+    finally:
+        plain.close()
+        skip.close()
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c["name"] for c in CASES])
+def test_learned_limit_matches_reference(case):
+    from bigwhoop_amd import skipnet
+    _say("skip learned ref", case["name"])
+    data = make_input(case)
+    c = bwc.Codec(case["gen"]["shape"], npar=case["npar"], prec=case["prec"], rate=case["rate"], **codec_kw(case))
+    try:
+        c.set_skip_model(skipnet.default_weights())
+        c.set_pass_skip(2, 2)
+        cs = c.compress(data)
+        assert sha(cs) == case["codestream_sha256"], c.skip_stats()
+    finally:
+        c.close()
