@@ -126,6 +126,9 @@ int rate_control_prefix(bwc_codec *c, Job *job, gpu::Workspace *ws, uint64_t ti)
     uint64_t nflag = 0;
     for (auto &f : chk.flagged) nflag += f.size();
     if (!chk.fallback && !nflag) return 0;
+    // the tile-level test speaks about settled cuts only: code flagged
+    // codeblocks further first, fall back only once none is flagged
+    if (nflag) chk.fallback = false;
     job->skip_stats[3] += 1;
     if (chk.fallback) job->skip_stats[5] += 1;
     for (int p = 0; p < np; ++p) {
