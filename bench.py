@@ -7,6 +7,12 @@ quality layer at 2 bits/value. The field is resident in HBM before timing
 starts (the C API reads a device pointer in place); the codestream is written
 to host memory inside the timed region.
 
+`value` is the learned pass-skip encoder (tier-1 codes each codeblock to the
+pass-limit network's prediction plus a margin; rate control proves the cuts
+are those of complete codings). The plain encoder (every pass coded) is timed
+on the same field, and the two codestreams must be byte-identical or the run
+fails.
+
 Prints one JSON line. Also reports the DWT's achieved HBM bandwidth against
 the 8 TB/s peak (algorithmic bytes: 16 B per value per axis pass, summed over
 levels) measured with HIP events on the library's stream, and the reference
@@ -28,6 +34,21 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 SETTINGS = dict(rate="2", decomp=(5, 5, 5, 0), cb=(5, 5, 3, 0))
+SKIP_MARGIN = 6
+TRAFFIC_PROFILE = ROOT / "profiles" / "dwt_512cube_hbm_traffic.md"
+
+
+def dwt_traffic():
+    """HBM bytes of the DWT stage measured with rocprofv3 counters (FETCH_SIZE
+    and WRITE_SIZE, each scaled by its reading for a 1 GiB device copy), as
+    recorded in the committed profile summary; None when that is absent."""
+    try:
+        for line in TRAFFIC_PROFILE.read_text().splitlines():
+            if line.startswith("dwt_stage_hbm_bytes:"):
+                return int(line.split(":", 1)[1].strip())
+    except OSError:
+        pass
+    return None
 
 
 def make_field_gpu(n: int, seed: int = 1, slope: float = -5.0 / 3.0):
@@ -145,27 +166,47 @@ def main():
         scratch.copy_(field)
         torch.cuda.synchronize()
         del scratch
-    codec = bwc.Codec((n, n, n), **SETTINGS)
+    from bigwhoop_amd import skipnet
+
     ptr = field.data_ptr()
-    size = 0
+
+    def run(codec, steps):
+        """`steps` timed bwc_compress calls; wall time, codestream, DWT stage ms."""
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dwt_ms, cs = [], b""
+        for _ in range(steps):
+            cs = codec.compress_ptr(ptr)
+            dwt_ms.append(codec.timings()[0][2])
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, cs, dwt_ms
+
+    # plain encoder (every pass of every codeblock coded)
+    plain = bwc.Codec((n, n, n), **SETTINGS)
     for _ in range(args.warmup):
-        size = len(codec.compress_ptr(ptr))
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    dwt_ms = []
-    for _ in range(args.steps):
-        size = len(codec.compress_ptr(ptr))
-        ms, _ = codec.timings()
-        dwt_ms.append(ms[2])
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+        plain.compress_ptr(ptr)
+    dt_plain, cs_plain, dwt_plain = run(plain, args.steps)
+    plain.close()
+    # learned pass skip (the headline encoder)
+    codec = bwc.Codec((n, n, n), **SETTINGS)
+    codec.set_skip_model(skipnet.default_weights())
+    codec.set_pass_skip(2, SKIP_MARGIN)
+    for _ in range(args.warmup):
+        codec.compress_ptr(ptr)
+    dt, cs, dwt_ms = run(codec, args.steps)
     ms_last, _ = codec.timings()
+    st = codec.skip_stats()
     codec.close()
+    if cs != cs_plain:
+        print("learned pass skip and plain encoder codestreams differ", file=sys.stderr)
+        sys.exit(1)
 
     bytes_in = n ** 3 * 8
     value = bytes_in * args.steps / dt / 1e9
+    plain_gbs = bytes_in * args.steps / dt_plain / 1e9
     alg = dwt_algorithmic_bytes(n, 5)
-    dwt_avg = sum(dwt_ms) / len(dwt_ms)
+    dwt_all = dwt_ms + dwt_plain
+    dwt_avg = sum(dwt_all) / len(dwt_all)
     achieved = alg / (dwt_avg / 1e3) / 1e9
     cpu = None if args.no_cpu_baseline else cpu_baseline()
     line = {
@@ -181,11 +222,19 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"{n}^3 float64 GRF k^-5/3, 5 levels, cb 32x32x8, 1 layer at 2 bpv (plain encoder)",
-                   "codestream_bytes": size, "encoder": "plain"},
+        "config": {"workload": f"{n}^3 float64 GRF k^-5/3, 5 levels, cb 32x32x8, 1 layer at 2 bpv",
+                   "encoder": f"learned pass skip (margin {SKIP_MARGIN} passes)", "codestream_bytes": len(cs),
+                   "codestream_identical_to_plain": True},
+        "plain_encoder": {"value": round(plain_gbs, 4), "unit": "GB/s",
+                          "ms_per_step": round(dt_plain / args.steps * 1e3, 3),
+                          "speedup_learned_vs_plain": round(dt_plain / dt, 3)},
+        "pass_skip": {"passes_coded": st["passes_coded"], "passes_complete": st["passes_complete"],
+                      "extra_rate_control_rounds": st["extra_rounds"], "codeblocks_coded_further": st["extended"],
+                      "fallbacks": st["fallbacks"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "DWT stage (15 axis passes)", "dwt_ms": round(dwt_avg, 4)},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": dwt_traffic(),
+                     "kernel": "DWT stage (15 axis passes, normalisation fused)", "dwt_ms": round(dwt_avg, 4),
+                     "algorithmic_bytes": alg},
         "stages_ms": {k: round(v, 3) for k, v in zip(
             ["fill_minmax", "normalize", "dwt", "quantize", "tier1", "records_d2h", "rate_control", "gather",
              "packets_assembly", "total"], ms_last)},
