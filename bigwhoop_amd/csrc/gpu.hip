@@ -1550,11 +1550,14 @@ static int v2_code(Workspace *ws, const bwc_codec *c, const Param &par, const Cb
                      ws->d_cbbox, ws->d_cbmeta, ws->d_K, ws->d_bpoff, d_bp, ws->d_symoff, ws->d_sym, ws->d_pcnt,
                      ws->d_pmse, ws->d_flags, erres, gp.maxcol, gp.obuf_cap, dz);
   HIP_OK(hipGetLastError());
-  // codeblocks per wavefront of the MQ kernel (lanes beyond LPW idle):
-  // fewer lanes means less divergence between codeblocks of unequal
-  // length, more means fewer waves; BWC_AMD_MQ_LPW = 1, 2, 4, 8, 16 or 32
+  // codeblocks per wavefront of the MQ kernel (lanes beyond LPW idle).
+  // Complete codings (~500k decisions per codeblock) are issue-bound: 16
+  // lanes share each instruction. Pass-limited codings are bound by the
+  // longest codeblock's decision latency, which more wavefronts per SIMD
+  // hide better: 4 lanes (measured on the 512^3 field). BWC_AMD_MQ_LPW =
+  // 1, 2, 4, 8, 16 or 32 overrides.
   const char *lpw_env = std::getenv("BWC_AMD_MQ_LPW");
-  const int lpw = lpw_env ? std::atoi(lpw_env) : 16;
+  const int lpw = lpw_env ? std::atoi(lpw_env) : (zlim ? 4 : 16);
   const uint32_t nl = (uint32_t)list.size();
 #define MQ_LAUNCH(N)                                                                                              \
   hipLaunchKernelGGL(t1_mq_kernel<N>, dim3((nl + N - 1) / N), dim3(64), 0, ws->stream, ws->d_list, (uint64_t)nl, \

@@ -9,7 +9,10 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <thread>
+#include <unordered_map>
 
 #include "codec.hpp"
 
@@ -84,11 +87,13 @@ using BitCounter = StuffedBits<CountSink>;
 // codeblock while the tree states stay fixed during a layer's probes.
 struct BitString {
   std::vector<uint64_t> *words;
+  size_t base;           // first word of this string in `words`
   uint64_t nbits = 0;
+  explicit BitString(std::vector<uint64_t> *w) : words(w), base(w->size()) {}
   void bit(uint64_t b) { put(b ? 1 : 0, 1); }
   void put(uint64_t v, int n) {
     for (int i = n; i-- > 0;) {
-      const uint64_t w = nbits >> 6;
+      const size_t w = base + (size_t)(nbits >> 6);
       if (w >= words->size()) words->push_back(0);
       if ((v >> i) & 1) (*words)[w] |= 1ull << (63 - (nbits & 63));
       ++nbits;
@@ -334,6 +339,12 @@ struct T2 {
   const LayerCache *lc = nullptr;
 
   void build_layer_cache(int q) {
+    // BWC_AMD_RC_NOCACHE=1 encodes the trees in every probe instead
+    if (const char *e = std::getenv("BWC_AMD_RC_NOCACHE"))
+      if (e[0] == '1') {
+        lc = nullptr;
+        return;
+      }
     layer_cache.incl.assign(c->info.nPar, BitCache{});
     layer_cache.msbs.assign(c->info.nPar, BitCache{});
     for (int j = 0; j < c->info.nPar; ++j) {
@@ -406,6 +417,166 @@ struct T2 {
     return ql;
   }
 
+  // ---- side-effect-free probes -------------------------------------------
+  // probe(T, q) returns what quality_layer(T, q, 1) returns, reading the
+  // shared state only (hull index, layer cache, previous layers' cuts,
+  // beta_est) and writing into its own buffer, so several thresholds can be
+  // estimated at once on host threads. quality_layer(.., 1) leaves no state
+  // behind that a later probe reads (packet flags are cleared by every
+  // estimate, cp[q] is rewritten for every codeblock), so the values are the
+  // same as the sequential probes'.
+  struct ProbeBuf {
+    std::vector<std::vector<int16_t>> cq;   // per param: cut of layer q per codeblock
+    std::vector<std::vector<uint8_t>> e;    // per param: contribution flag per (resolution, precinct)
+    std::vector<std::vector<uint64_t>> eoff;
+  };
+  void probe_init(ProbeBuf &pb) {
+    const int np = c->info.nPar;
+    pb.cq.assign(np, {});
+    pb.e.assign(np, {});
+    pb.eoff.assign(np, {});
+    for (int j = 0; j < np; ++j) {
+      pb.cq[j].assign(rec(j).n, 0);
+      uint64_t o = 0;
+      for (auto &res : tl->par[j].res) {
+        pb.eoff[j].push_back(o);
+        o += res.nprec;
+      }
+      pb.e[j].assign(o, 0);
+    }
+  }
+
+  template <class W>
+  void length_code_probe(W &w, const CblkRecords &R, uint64_t i, int16_t prev, int16_t cur) const {
+    const int16_t zc = (int16_t)(cur - 1), zp = (int16_t)(prev - 1);
+    uint16_t beta = R.beta_est[i];
+    uint16_t maxb = beta;
+    const uint32_t *L = &R.L[i * kMaxPasses];
+    const uint32_t len = L[zc] - ((zp >= 0) ? L[zp] : 0);
+    const uint16_t cpb = ilog2_u32((uint32_t)(zc - zp));
+    int32_t lb = ilog2_u32(len) + 1 - cpb;
+    if (maxb < lb) maxb = (uint16_t)lb;
+    w.ones(maxb - beta);
+    beta = std::max(beta, maxb);
+    w.put(0, 1);
+    const int nb = beta + cpb;
+    if (nb > 0 && nb < 32) w.put(len & ((1u << nb) - 1u), nb);
+  }
+
+  int64_t probe(uint16_t T, int q, ProbeBuf &pb) {
+    int64_t ql = 0;
+    for (int j = 0; j < c->info.nPar; ++j) {
+      Param &par = tl->par[j];
+      CblkRecords &R = rec(j);
+      int16_t *cq = pb.cq[j].data();
+      uint8_t *ef = pb.e[j].data();
+      std::fill(pb.e[j].begin(), pb.e[j].end(), 0);
+      for (size_t r = 0; r < par.res.size(); ++r) {
+        Resolution &res = par.res[r];
+        uint8_t *er = ef + pb.eoff[j][r];
+        for (auto &sb : res.sb)
+          for (uint64_t p = 0; p < res.nprec; ++p) {
+            const Precinct &pr = sb.prec[p];
+            for (uint64_t k = 0; k < pr.ncbs; ++k) {
+              const uint64_t i = pr.first_cb + k;
+              const int16_t prev = R.cp[i * (nl + 1) + q];
+              const uint32_t *L = &R.L[i * kMaxPasses];
+              const int16_t z = cut_of(j, R, i, T);
+              int16_t cur;
+              if (z < prev) {
+                cur = prev;
+                if (cur == 1) ql += L[0];
+              } else {
+                cur = z;
+                if (z > 0) {
+                  ql += L[z - 1];
+                  if (z > 1 && prev > 0) ql -= L[prev - 1];
+                }
+              }
+              cq[i] = cur;
+              if (cur > prev) er[p] = 1;
+            }
+          }
+        for (uint64_t p = 0; p < res.nprec; ++p) {
+          if (!er[p]) {
+            ql += 1;
+            continue;
+          }
+          BitCounter w;
+          w.bit(1);
+          for (auto &sb : res.sb) {
+            const Precinct &pr = sb.prec[p];
+            for (uint64_t k = 0; k < pr.ncbs; ++k) {
+              const uint64_t i = pr.first_cb + k;
+              const int16_t prev = R.cp[i * (nl + 1) + q], cur = cq[i];
+              const int dz = cur - prev;
+              if (prev == 0) {
+                if (lc) {
+                  lc->incl[j].feed(w, i);
+                  if (cur) lc->msbs[j].feed(w, i);
+                } else {   // probes leave tree thresholds alone (est 2)
+                  tagtree_encode(sb.prec[p].incl, w, (uint32_t)q + 1, k, 2);
+                  if (cur)
+                    for (uint32_t t = 1; t <= R.Kmsbs[i] + 1u; ++t) tagtree_encode(sb.prec[p].msbs, w, t, k, 2);
+                }
+              } else {
+                w.bit((uint64_t)dz);
+              }
+              if (dz) {
+                delta_z_code(w, dz);
+                length_code_probe(w, R, i, prev, cur);
+              }
+            }
+          }
+          w.flush();
+          ql += (int64_t)w.sink.n;
+        }
+      }
+    }
+    return ql;
+  }
+
+  // evaluate several thresholds of layer q at once
+  int threads = 1;
+  std::vector<ProbeBuf> pbufs;
+  void probe_many(const std::vector<uint16_t> &Ts, int q, std::unordered_map<uint32_t, int64_t> &memo) {
+    if ((int)pbufs.size() < std::max<int>(1, (int)Ts.size())) {
+      pbufs.resize(std::max<size_t>(1, Ts.size()));
+      for (auto &pb : pbufs)
+        if (pb.cq.empty()) probe_init(pb);
+    }
+    std::vector<int64_t> out(Ts.size());
+    if (Ts.size() == 1 || threads <= 1) {
+      for (size_t a = 0; a < Ts.size(); ++a) out[a] = probe(Ts[a], q, pbufs[0]);
+    } else {
+      std::vector<std::thread> th;
+      for (size_t a = 0; a < Ts.size(); ++a) th.emplace_back([&, a] { out[a] = probe(Ts[a], q, pbufs[a]); });
+      for (auto &t : th) t.join();
+    }
+    for (size_t a = 0; a < Ts.size(); ++a) memo[Ts[a]] = out[a];
+  }
+
+  // thresholds the reference bisection may probe in its next `depth` steps
+  // from (smin, smax): every step goes on with smin = slope or smax = slope
+  // (an equal estimate ends it)
+  static void frontier(uint16_t smin, uint16_t smax, int depth, std::vector<uint16_t> &out) {
+    if (depth == 0 || smin >= smax) return;
+    const uint16_t s = (uint16_t)((smax + smin) >> 1);
+    if (std::find(out.begin(), out.end(), s) == out.end()) out.push_back(s);
+    {  // estimate above the target
+      uint16_t a = smin, b = smax;
+      if (a == s) b = s;
+      a = s;
+      frontier(a, b, depth - 1, out);
+    }
+    {  // below
+      uint16_t a = smin, b = smax;
+      if (b == s) a = s;
+      b = s;
+      frontier(a, b, depth - 1, out);
+    }
+  }
+
   // Records coded only to a prefix of their passes (Z < Zfull): the layer's
   // threshold T and cuts equal those of complete codings when
   //  - the size estimate is strictly above the target at T - 1 and strictly
@@ -420,8 +591,16 @@ struct T2 {
   SkipCheck *chk = nullptr;
   void check_layer(uint16_t slope, int l, int64_t target, int64_t base) {
     if (target > 0) {
-      const int64_t f0 = quality_layer(slope, l, 1) + base;
-      const int64_t fm = slope > 0 ? quality_layer((uint16_t)(slope - 1), l, 1) + base : INT64_MAX;
+      std::unordered_map<uint32_t, int64_t> m;
+      std::vector<uint16_t> Ts{slope};
+      if (slope > 0) Ts.push_back((uint16_t)(slope - 1));
+      if (threads > 0) {
+        probe_many(Ts, l, m);
+      } else {
+        for (uint16_t T : Ts) m[T] = quality_layer(T, l, 1);
+      }
+      const int64_t f0 = m[slope] + base;
+      const int64_t fm = slope > 0 ? m[(uint16_t)(slope - 1)] + base : INT64_MAX;
       if (!(f0 < target && fm > target)) chk->fallback = true;
     }
     const uint16_t t = slope > 0 ? (uint16_t)(slope - 1) : 0;
@@ -447,9 +626,23 @@ struct T2 {
         build_layer_cache(l);
         uint16_t smin = tl->slope_min, smax = tl->slope_max, tmp = tl->slope_min;
         slope = smin;
+        // the reference's bisection, its probes estimated ahead on `threads`
+        // host threads: the next three levels of its decision tree at once
+        std::unordered_map<uint32_t, int64_t> memo;
         while (smin < smax) {
           slope = (uint16_t)((smax + smin) >> 1);
-          est_ql = quality_layer(slope, l, 1) + est_tile + (int64_t)hdr;
+          int64_t f;
+          if (threads > 0) {
+            if (!memo.count(slope)) {
+              std::vector<uint16_t> Ts;
+              frontier(smin, smax, threads >= 7 ? 3 : (threads >= 3 ? 2 : 1), Ts);
+              probe_many(Ts, l, memo);
+            }
+            f = memo[slope];
+          } else {
+            f = quality_layer(slope, l, 1);
+          }
+          est_ql = f + est_tile + (int64_t)hdr;
           if (est_ql > target) {
             if (smin == slope) smax = slope;
             smin = slope;
@@ -486,6 +679,16 @@ struct T2 {
 // contribution flags / body sizes. Needs tile slope bounds and records.
 int t2_rate_control(bwc_codec *codec, Job *job, uint64_t tile, SkipCheck *chk) {
   T2 t{codec, job, &job->plan.tiles[tile], tile, codec->control.nLayers};
+  // host threads for the bisection probes: BWC_AMD_RC_THREADS (0 = the
+  // sequential in-place probes, 1 = side-effect-free probes on one thread)
+  int rc_threads;
+  if (const char *e = std::getenv("BWC_AMD_RC_THREADS")) {
+    rc_threads = std::max(0, std::min(16, std::atoi(e)));
+  } else {
+    const unsigned hc = std::thread::hardware_concurrency();
+    rc_threads = (int)std::max(1u, std::min(7u, hc > 1 ? hc - 1 : 1u));
+  }
+  t.threads = rc_threads;
   if (chk) {
     chk->fallback = false;
     chk->flagged.assign(codec->info.nPar, {});

@@ -57,8 +57,14 @@ def _oracle_records(o, data, t, p):
     return K, Km, Z, L, S, b"".join(chunks), np.array(off, np.uint64), mm
 
 
+# BWC_AMD_RC_THREADS: 0 = the sequential in-place probes, 1 = side-effect-free
+# probes on one thread, 7 = the bisection's next three levels on threads;
+# "nc": tag trees encoded in every probe instead of the per-layer bit cache
+@pytest.mark.parametrize("threads", ["0", "1", "7", "7nc", "0nc"])
 @pytest.mark.parametrize("case", CASES, ids=[c["name"] for c in CASES])
-def test_geometry_and_tier2_against_golden(case):
+def test_geometry_and_tier2_against_golden(case, threads, monkeypatch):
+    monkeypatch.setenv("BWC_AMD_RC_THREADS", threads.replace("nc", ""))
+    monkeypatch.setenv("BWC_AMD_RC_NOCACHE", "1" if threads.endswith("nc") else "0")
     data = make_input(case)
     shape = case["gen"]["shape"]
     kw = codec_kw(case)
@@ -79,6 +85,29 @@ def test_geometry_and_tier2_against_golden(case):
         cs = c.encode_records()
         assert len(cs) == case["codestream_size"]
         assert sha(cs) == case["codestream_sha256"]
+    finally:
+        c.close()
+        o.close()
+
+
+@pytest.mark.parametrize("threads", ["0", "7", "7nc"])
+def test_rate_control_modes_agree_with_oracle_multilayer(threads, monkeypatch):
+    """A larger multi-layer case than the golden ones: oracle records of a
+    96^3 field at four rates through the host tier-2 in each probe mode must
+    give the oracle's codestream."""
+    from bigwhoop_amd.fields import grf
+    monkeypatch.setenv("BWC_AMD_RC_THREADS", threads.replace("nc", ""))
+    monkeypatch.setenv("BWC_AMD_RC_NOCACHE", "1" if threads.endswith("nc") else "0")
+    shape = (96, 96, 96)
+    kw = dict(rate="0.5,1,2,4", decomp=(4, 4, 4, 0), cb=(5, 5, 3, 0))
+    data = grf(shape, slope=-2.2, seed=21)
+    o = ol.Oracle(shape, **kw)
+    c = bwc.Codec(shape, **kw)
+    try:
+        ref = o.compress(data)
+        c.begin_records()
+        c.set_records(0, 0, *_oracle_records(o, data, 0, 0))
+        assert c.encode_records() == ref
     finally:
         c.close()
         o.close()
