@@ -55,6 +55,9 @@ Workspace::~Workspace() {
   if (h_pinned) (void)hipHostFree(h_pinned);
   for (auto &e : ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto &e : ev_join)
+    if (e) (void)hipEventDestroy(e);
+  if (stream2) (void)hipStreamDestroy(stream2);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -1544,36 +1547,82 @@ static int v2_code(Workspace *ws, const bwc_codec *c, const Param &par, const Cb
   const uint8_t *dz = zlim ? ws->d_zlim : nullptr;
   if (list.empty()) return 0;
   const int erres = (int)c->control.error_resilience;
-  HIP_OK(hipMemcpyAsync(ws->d_list, list.data(), list.size() * sizeof(uint64_t), hipMemcpyHostToDevice,
-                        ws->stream));
-  hipLaunchKernelGGL(t1_gen_kernel, dim3((uint32_t)list.size()), dim3(64), gp.lds, ws->stream, ws->d_list,
-                     ws->d_cbbox, ws->d_cbmeta, ws->d_K, ws->d_bpoff, d_bp, ws->d_symoff, ws->d_sym, ws->d_pcnt,
-                     ws->d_pmse, ws->d_flags, erres, gp.maxcol, gp.obuf_cap, dz);
-  HIP_OK(hipGetLastError());
   // codeblocks per wavefront of the MQ kernel (lanes beyond LPW idle).
   // Complete codings (~500k decisions per codeblock) are issue-bound: 16
   // lanes share each instruction. Pass-limited codings are bound by the
-  // longest codeblock's decision latency, which more wavefronts per SIMD
+  // longest codeblocks' decision latency, which more wavefronts per SIMD
   // hide better: 4 lanes (measured on the 512^3 field). BWC_AMD_MQ_LPW =
   // 1, 2, 4, 8, 16 or 32 overrides.
   const char *lpw_env = std::getenv("BWC_AMD_MQ_LPW");
   const int lpw = lpw_env ? std::atoi(lpw_env) : (zlim ? 4 : 16);
-  const uint32_t nl = (uint32_t)list.size();
-#define MQ_LAUNCH(N)                                                                                              \
-  hipLaunchKernelGGL(t1_mq_kernel<N>, dim3((nl + N - 1) / N), dim3(64), 0, ws->stream, ws->d_list, (uint64_t)nl, \
-                     ws->d_cbmeta, ws->d_cbsb, ws->d_K, ws->d_symoff, ws->d_sym, ws->d_pcnt, ws->d_pmse,          \
-                     ws->d_doff, d_data, ws->d_Kmsbs, ws->d_Z, ws->d_L, ws->d_S, ws->d_snap, ws->d_mse,            \
-                     ws->d_flags, dz, ws->d_U)
-  switch (lpw) {
-    case 1: MQ_LAUNCH(1); break;
-    case 2: MQ_LAUNCH(2); break;
-    case 4: MQ_LAUNCH(4); break;
-    case 8: MQ_LAUNCH(8); break;
-    case 32: MQ_LAUNCH(32); break;
-    default: MQ_LAUNCH(16); break;
-  }
+  auto launch = [&](const uint64_t *dl, uint32_t nl, hipStream_t s, int lanes) -> int {
+    hipLaunchKernelGGL(t1_gen_kernel, dim3(nl), dim3(64), gp.lds, s, dl, ws->d_cbbox, ws->d_cbmeta, ws->d_K,
+                       ws->d_bpoff, d_bp, ws->d_symoff, ws->d_sym, ws->d_pcnt, ws->d_pmse, ws->d_flags, erres,
+                       gp.maxcol, gp.obuf_cap, dz);
+    HIP_OK(hipGetLastError());
+#define MQ_LAUNCH(N)                                                                                           \
+  hipLaunchKernelGGL(t1_mq_kernel<N>, dim3((nl + N - 1) / N), dim3(64), 0, s, dl, (uint64_t)nl, ws->d_cbmeta, \
+                     ws->d_cbsb, ws->d_K, ws->d_symoff, ws->d_sym, ws->d_pcnt, ws->d_pmse, ws->d_doff, d_data,    \
+                     ws->d_Kmsbs, ws->d_Z, ws->d_L, ws->d_S, ws->d_snap, ws->d_mse, ws->d_flags, dz, ws->d_U)
+    switch (lanes) {
+      case 1: MQ_LAUNCH(1); break;
+      case 2: MQ_LAUNCH(2); break;
+      case 4: MQ_LAUNCH(4); break;
+      case 8: MQ_LAUNCH(8); break;
+      case 32: MQ_LAUNCH(32); break;
+      default: MQ_LAUNCH(16); break;
+    }
 #undef MQ_LAUNCH
-  HIP_OK(hipGetLastError());
+    HIP_OK(hipGetLastError());
+    return 0;
+  };
+  // With pass limits the tier-1 time is set by the longest codings (deep
+  // cuts of the coarse subbands). Those codeblocks (estimated decisions =
+  // passes x values within 30% of the largest, at most 1/8 of the list) are
+  // generated and coded on a second stream, so their serial MQ chains run
+  // while the bulk is still in symbol generation. BWC_AMD_T1_SPLIT=0 keeps
+  // one stream.
+  const char *split_env = std::getenv("BWC_AMD_T1_SPLIT");
+  const bool split = zlim && list.size() >= 512 && !(split_env && split_env[0] == '0');
+  std::vector<uint64_t> order;
+  uint32_t ndeep = 0;
+  if (split) {
+    std::vector<std::pair<uint64_t, uint64_t>> est(list.size());
+    uint64_t emax = 0;
+    for (size_t a = 0; a < list.size(); ++a) {
+      const uint64_t i = list[a];
+      const uint32_t *b = &par.cb_box[8 * i];
+      const uint64_t nv = (uint64_t)(b[4] - b[0]) * (b[5] - b[1]) * (b[6] - b[2]) * (b[7] - b[3]);
+      const uint64_t full = Kh[i] ? 3ull * Kh[i] - 2 : 1;
+      const uint64_t z = std::min<uint64_t>(zlim[i], full);
+      est[a] = {z * nv, i};
+      emax = std::max(emax, z * nv);
+    }
+    std::vector<uint8_t> deep(est.size(), 0);
+    for (size_t a = 0; a < est.size(); ++a)
+      if (est[a].first * 10 >= emax * 7 && ndeep < list.size() / 8) {
+        deep[a] = 1;
+        order.push_back(est[a].second);
+        ++ndeep;
+      }
+    for (size_t a = 0; a < est.size(); ++a)
+      if (!deep[a]) order.push_back(est[a].second);
+  }
+  const std::vector<uint64_t> &L = split ? order : list;
+  HIP_OK(hipMemcpyAsync(ws->d_list, L.data(), L.size() * sizeof(uint64_t), hipMemcpyHostToDevice, ws->stream));
+  if (split && ndeep) {
+    if (!ws->stream2) HIP_OK(hipStreamCreateWithFlags(&ws->stream2, hipStreamNonBlocking));
+    for (auto &e : ws->ev_join)
+      if (!e) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIP_OK(hipEventRecord(ws->ev_join[0], ws->stream));
+    HIP_OK(hipStreamWaitEvent(ws->stream2, ws->ev_join[0], 0));
+    CHK(launch(ws->d_list, ndeep, ws->stream2, 1));
+    CHK(launch(ws->d_list + ndeep, (uint32_t)(L.size() - ndeep), ws->stream, lpw));
+    HIP_OK(hipEventRecord(ws->ev_join[1], ws->stream2));
+    HIP_OK(hipStreamWaitEvent(ws->stream, ws->ev_join[1], 0));
+  } else {
+    CHK(launch(ws->d_list, (uint32_t)L.size(), ws->stream, lpw));
+  }
   return 0;
 }
 
