@@ -55,9 +55,6 @@ Workspace::~Workspace() {
   if (h_pinned) (void)hipHostFree(h_pinned);
   for (auto &e : ev)
     if (e) (void)hipEventDestroy(e);
-  for (auto &e : ev_join)
-    if (e) (void)hipEventDestroy(e);
-  if (stream2) (void)hipStreamDestroy(stream2);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -1576,54 +1573,9 @@ static int v2_code(Workspace *ws, const bwc_codec *c, const Param &par, const Cb
     HIP_OK(hipGetLastError());
     return 0;
   };
-  // With pass limits the tier-1 time is set by the longest codings (deep
-  // cuts of the coarse subbands). Those codeblocks (estimated decisions =
-  // passes x values within 30% of the largest, at most 1/8 of the list) are
-  // generated and coded on a second stream, so their serial MQ chains run
-  // while the bulk is still in symbol generation. BWC_AMD_T1_SPLIT=0 keeps
-  // one stream.
-  const char *split_env = std::getenv("BWC_AMD_T1_SPLIT");
-  const bool split = zlim && list.size() >= 512 && !(split_env && split_env[0] == '0');
-  std::vector<uint64_t> order;
-  uint32_t ndeep = 0;
-  if (split) {
-    std::vector<std::pair<uint64_t, uint64_t>> est(list.size());
-    uint64_t emax = 0;
-    for (size_t a = 0; a < list.size(); ++a) {
-      const uint64_t i = list[a];
-      const uint32_t *b = &par.cb_box[8 * i];
-      const uint64_t nv = (uint64_t)(b[4] - b[0]) * (b[5] - b[1]) * (b[6] - b[2]) * (b[7] - b[3]);
-      const uint64_t full = Kh[i] ? 3ull * Kh[i] - 2 : 1;
-      const uint64_t z = std::min<uint64_t>(zlim[i], full);
-      est[a] = {z * nv, i};
-      emax = std::max(emax, z * nv);
-    }
-    std::vector<uint8_t> deep(est.size(), 0);
-    for (size_t a = 0; a < est.size(); ++a)
-      if (est[a].first * 10 >= emax * 7 && ndeep < list.size() / 8) {
-        deep[a] = 1;
-        order.push_back(est[a].second);
-        ++ndeep;
-      }
-    for (size_t a = 0; a < est.size(); ++a)
-      if (!deep[a]) order.push_back(est[a].second);
-  }
-  const std::vector<uint64_t> &L = split ? order : list;
-  HIP_OK(hipMemcpyAsync(ws->d_list, L.data(), L.size() * sizeof(uint64_t), hipMemcpyHostToDevice, ws->stream));
-  if (split && ndeep) {
-    if (!ws->stream2) HIP_OK(hipStreamCreateWithFlags(&ws->stream2, hipStreamNonBlocking));
-    for (auto &e : ws->ev_join)
-      if (!e) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    HIP_OK(hipEventRecord(ws->ev_join[0], ws->stream));
-    HIP_OK(hipStreamWaitEvent(ws->stream2, ws->ev_join[0], 0));
-    CHK(launch(ws->d_list, ndeep, ws->stream2, 1));
-    CHK(launch(ws->d_list + ndeep, (uint32_t)(L.size() - ndeep), ws->stream, lpw));
-    HIP_OK(hipEventRecord(ws->ev_join[1], ws->stream2));
-    HIP_OK(hipStreamWaitEvent(ws->stream, ws->ev_join[1], 0));
-  } else {
-    CHK(launch(ws->d_list, (uint32_t)L.size(), ws->stream, lpw));
-  }
-  return 0;
+  HIP_OK(hipMemcpyAsync(ws->d_list, list.data(), list.size() * sizeof(uint64_t), hipMemcpyHostToDevice,
+                        ws->stream));
+  return launch(ws->d_list, (uint32_t)list.size(), ws->stream, lpw);
 }
 
 int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double *work, CblkRecords &rec,

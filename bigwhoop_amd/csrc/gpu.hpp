@@ -15,9 +15,7 @@ namespace gpu {
 struct Workspace {
   int device = -1;
   hipStream_t stream = nullptr;
-  hipStream_t stream2 = nullptr;    // tier-1 of the longest codings, joined back into `stream`
   hipEvent_t ev[8] = {};
-  hipEvent_t ev_join[2] = {};
   // field staging (host input copied here) and the tile working buffer (f64)
   void *d_input = nullptr;
   size_t input_cap = 0;
