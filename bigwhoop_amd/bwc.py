@@ -82,6 +82,8 @@ def lib():
             "bwc_close_header": (None, [vp]),
             "bwc_amd_version": (C.c_int, []),
             "bwc_amd_device_count": (C.c_int, []),
+            "bwc_amd_host_alloc": (vp, [C.c_size_t]),
+            "bwc_amd_host_free": (None, [vp]),
             "bwc_amd_last_error": (C.c_char_p, []),
             "bwc_amd_reset_gain_lut": (None, []),
             "bwc_amd_ncblk": (u64, [vp, u64, u8]),
@@ -199,7 +201,9 @@ class Codec:
     """
 
     def __init__(self, shape, npar=1, prec=8, rate="2", decomp=None, precincts=None, cb=None, qm=None,
-                 tiles=None, erres=False, fresh_gain_table=True):
+                 tiles=None, erres=False, fresh_gain_table=True, pinned=False):
+        """``pinned``: the codestream buffer is page-locked host memory
+        (bwc_amd_host_alloc), so packet bodies reach it by DMA."""
         L = lib()
         if fresh_gain_table:
             L.bwc_amd_reset_gain_lut()
@@ -223,7 +227,15 @@ class Codec:
             L.bwc_set_tiles(self.h, *[int(x) for x in tiles], 0)
         if erres:
             L.bwc_set_error_resilience(self.h)
-        self.out = np.zeros(self.nbytes * 2 + (1 << 20), np.uint8)
+        nout = self.nbytes * 2 + (1 << 20)
+        self._pinned = None
+        if pinned:
+            self._pinned = L.bwc_amd_host_alloc(nout)
+            if not self._pinned:
+                raise RuntimeError("bwc_amd_host_alloc failed: " + last_error())
+            self.out = np.ctypeslib.as_array((C.c_uint8 * nout).from_address(self._pinned))
+        else:
+            self.out = np.zeros(nout, np.uint8)
         self.stream = L.bwc_init_stream(None, self.out.ctypes.data_as(C.c_void_p), 0)
         self._rate = C.create_string_buffer(rate.encode())
         if L.bwc_create_compression(self.h, self.stream, self._rate):
@@ -236,6 +248,10 @@ class Codec:
         if getattr(self, "stream", None):
             C.CDLL(None).free(C.c_void_p(self.stream))
             self.stream = None
+        if getattr(self, "_pinned", None):
+            self.out = None
+            lib().bwc_amd_host_free(self._pinned)
+            self._pinned = None
 
     def __del__(self):
         try:
@@ -243,13 +259,20 @@ class Codec:
         except Exception:
             pass
 
-    def compress_ptr(self, ptr: int) -> bytes:
-        """Compress from a host or HIP device address (e.g. tensor.data_ptr())."""
+    def compress_into(self, ptr: int) -> int:
+        """Compress from a host or HIP device address (e.g. tensor.data_ptr())
+        into the codec's output buffer; returns the codestream size
+        (self.out[:size] holds it until the next call)."""
         st = _Stream.from_address(self.stream)
         st.inp = ptr
         n = lib().bwc_compress(self.h, self.stream)
         if n == 0:
             raise RuntimeError("bwc_compress failed: " + last_error())
+        return int(n)
+
+    def compress_ptr(self, ptr: int) -> bytes:
+        """Compress from a host or HIP device address; returns the codestream."""
+        n = self.compress_into(ptr)
         return self.out[:n].tobytes()
 
     def compress(self, data: np.ndarray) -> bytes:

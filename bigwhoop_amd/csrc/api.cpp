@@ -105,6 +105,48 @@ void reset_tile_t2(bwc_codec *c, Job *job, uint64_t ti) {
 // complete codings, code the flagged codeblocks further (limits grow 9, 18,
 // 36 passes per round; complete coding from the fourth round or for the
 // whole tile on a fallback) and run it again.
+// Gather the packet bodies of tile ti on the device, from [body_end] on:
+// each packet's chunks back to back (Packet::dev_off, its body_size bytes).
+int gather_tile_bodies(bwc_codec *c, Job *job, gpu::Workspace *ws, uint64_t ti, uint64_t &body_end) {
+  Tile &tl = job->plan.tiles[ti];
+  const int nl = c->control.nLayers;
+  std::vector<uint64_t> desc;
+  std::vector<int> slots;
+  std::vector<std::pair<uint64_t, uint64_t>> ch;
+  uint64_t off = body_end;
+  for (int j = 0; j < c->info.nPar; ++j) {
+    const CblkRecords &R = job->rec[ti * c->info.nPar + j];
+    for (size_t r = 0; r < tl.par[j].res.size(); ++r) {
+      Resolution &res = tl.par[j].res[r];
+      for (uint64_t p = 0; p < res.nprec; ++p)
+        for (int l = 0; l < nl; ++l) {
+          Packet &pk = res.pk[l * res.nprec + p];
+          pk.dev_off = off;
+          if (!pk.e) continue;
+          ch.clear();
+          packet_body_chunks(c, job, ti, j, (int)r, p, l, ch);
+          uint64_t sum = 0;
+          for (auto &x : ch) {
+            if (!x.second) continue;
+            desc.push_back(off + sum);
+            desc.push_back(x.second);
+            desc.push_back(x.first);
+            slots.push_back(R.slot);
+            sum += x.second;
+          }
+          if (sum != pk.body_size) {
+            set_err("packet body size differs from its chunks");
+            return 1;
+          }
+          off += sum;
+        }
+    }
+  }
+  if (gpu::gather_bodies(ws, desc, slots, off)) return 1;
+  body_end = off;
+  return 0;
+}
+
 int rate_control_prefix(bwc_codec *c, Job *job, gpu::Workspace *ws, uint64_t ti) {
   Tile &tl = job->plan.tiles[ti];
   const int np = c->info.nPar;
@@ -477,6 +519,12 @@ size_t bwc_compress(bwc_codec *const codec, bwc_stream *const stream) {
   }
   double ms[10] = {0};
   int dwt_launch = 0;
+  uint64_t body_end = 0;   // packet bodies gathered so far (device body buffer)
+  BodySource bodies;
+  bodies.ctx = ws;
+  bodies.copy = [](void *w, uint8_t *dst, uint64_t off, uint64_t n) {
+    return gpu::body_d2h(static_cast<gpu::Workspace *>(w), dst, off, n);
+  };
   for (uint64_t ti = 0; ti < job->plan.tiles.size(); ++ti) {
     Tile &tl = job->plan.tiles[ti];
     for (int p = 0; p < codec->info.nPar; ++p) {
@@ -539,26 +587,19 @@ size_t bwc_compress(bwc_codec *const codec, bwc_stream *const stream) {
         t2_rate_control(codec, job, ti);
       }
       ms[6] += now_ms() - t0;
+      // packet headers, then every packet body gathered on the device into
+      // one contiguous range of the body buffer (the parameter slots are
+      // coded over by the next tile)
       const double t1 = now_ms();
-      const int nl = ctl.nLayers;
-      for (int p = 0; p < codec->info.nPar; ++p) {
-        CblkRecords &R = job->rec[ti * codec->info.nPar + p];
-        std::vector<uint32_t> len(R.n);
-        for (uint64_t i = 0; i < R.n; ++i) {
-          const int16_t cut = R.cp[i * (nl + 1) + nl];
-          len[i] = cut > 0 ? R.L[i * kMaxPasses + cut - 1] : 0;
-        }
-        if (gpu::gather(ws, R, len)) goto fail;
-      }
-      ms[7] += now_ms() - t1;
-      const double t2 = now_ms();
       t2_packets(codec, job, ti);
-      ms[8] += now_ms() - t2;
+      if (gather_tile_bodies(codec, job, ws, ti, body_end)) goto fail;
+      ms[7] += now_ms() - t1;
     }
   }
   {
     const double t3 = now_ms();
-    const size_t n = assemble_codestream(codec, job, stream, static_cast<uint8_t *>(stream->out));
+    size_t n = assemble_codestream(codec, job, stream, static_cast<uint8_t *>(stream->out), &bodies);
+    if (gpu::sync(ws)) n = 0;
     ms[8] += now_ms() - t3;
     ms[9] = now_ms() - t_start;
     std::memcpy(job->ms, ms, sizeof ms);
@@ -604,6 +645,8 @@ int bwc_amd_version(void) { return gpu::device_count() > 0 ? 1 : 0; }
 
 void bwc_amd_reset_gain_lut(void) { reset_gain_lut(); }
 int bwc_amd_device_count(void) { return gpu::device_count(); }
+void *bwc_amd_host_alloc(size_t bytes) { return gpu::host_alloc(bytes); }
+void bwc_amd_host_free(void *p) { gpu::host_free(p); }
 const char *bwc_amd_last_error(void) { return g_api_err[0] ? g_api_err : gpu::last_error(); }
 
 uint64_t bwc_amd_ncblk(bwc_codec *codec, uint64_t tile, uint8_t param) {

@@ -64,6 +64,7 @@ struct Packet {
   uint8_t e = 0;
   uint32_t size = 0;
   bool built = false;
+  uint64_t dev_off = 0;   // body offset in the device body buffer (bodies gathered on the GPU)
 };
 
 struct Resolution {
@@ -180,7 +181,22 @@ struct SkipCheck {
 };
 int t2_rate_control(bwc_codec *codec, Job *job, uint64_t tile, SkipCheck *chk = nullptr);
 int t2_packets(bwc_codec *codec, Job *job, uint64_t tile);
-size_t assemble_codestream(bwc_codec *codec, Job *job, const bwc_stream *st, uint8_t *out);
+// Packet bodies already laid out elsewhere (on the device): copy(ctx, dst,
+// dev_off, n) writes the n body bytes of a packet, found at Packet::dev_off,
+// to dst; nonzero return = failure.
+struct BodySource {
+  void *ctx = nullptr;
+  int (*copy)(void *ctx, uint8_t *dst, uint64_t dev_off, uint64_t n) = nullptr;
+};
+// Codestream assembly into out. Bodies come from each record's data (host)
+// or, with `bodies`, from the packets' device layout.
+size_t assemble_codestream(bwc_codec *codec, Job *job, const bwc_stream *st, uint8_t *out,
+                           const BodySource *bodies = nullptr);
+// The (source offset in the parameter's coded data, length) of every body
+// chunk of packet (param j, resolution r, precinct p, layer l) of a tile, in
+// codestream order.
+void packet_body_chunks(bwc_codec *codec, Job *job, uint64_t tile, int j, int r, uint64_t p, int l,
+                        std::vector<std::pair<uint64_t, uint64_t>> &out);
 uint64_t header_size(const bwc_codec *codec, const bwc_stream *st);
 
 // ---- decode (decode.cpp) ----

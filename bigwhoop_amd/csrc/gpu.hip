@@ -45,7 +45,7 @@ Workspace::~Workspace() {
   void *ptrs[] = {d_input, d_work,  d_work2, d_red,  d_norm, d_cbbox, d_cbsb, d_cbmeta, d_qoff,   d_doff, d_q,
                   d_state, d_data,  d_K,    d_Kmsbs, d_Z,    d_L,    d_S,      d_snap,   d_mse,  d_flags,
                   d_packed, d_gather, d_bp, d_bpoff, d_sym, d_symoff, d_pcnt, d_pmse, d_list, d_U, d_zlim,
-                  d_feat,  d_tstat, d_skipw};
+                  d_feat,  d_tstat, d_skipw, d_bodies};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (auto &b : bp_slots)
@@ -1836,6 +1836,67 @@ int gather(Workspace *ws, CblkRecords &rec, const std::vector<uint32_t> &len) {
   return 0;
 }
 
+// chunk k: desc[3k] = offset in the body buffer, desc[3k + 1] = length,
+// desc[3k + 2] = source address
+__global__ void gather_abs_kernel(const uint64_t *__restrict__ desc, uint64_t n, uint8_t *__restrict__ out) {
+  for (uint64_t k = blockIdx.x; k < n; k += gridDim.x) {
+    const uint64_t dst = desc[3 * k], len = desc[3 * k + 1];
+    const uint8_t *src = reinterpret_cast<const uint8_t *>(desc[3 * k + 2]);
+    for (uint64_t i = threadIdx.x; i < len; i += blockDim.x) out[dst + i] = src[i];
+  }
+}
+
+int gather_bodies(Workspace *ws, const std::vector<uint64_t> &desc, const std::vector<int> &slots, uint64_t end) {
+  const uint64_t n = desc.size() / 3;
+  if (slots.size() != n) {
+    std::snprintf(g_err, sizeof g_err, "gather_bodies: one slot per chunk expected");
+    return 1;
+  }
+  if (end > ws->cap_bodies) {
+    // grow, keeping what earlier tiles placed
+    uint8_t *nb = nullptr;
+    const size_t cap = std::max<size_t>(end + end / 4, 1 << 20);
+    HIP_OK(hipMalloc((void **)&nb, cap));
+    if (ws->d_bodies) {
+      HIP_OK(hipMemcpyAsync(nb, ws->d_bodies, ws->cap_bodies, hipMemcpyDeviceToDevice, ws->stream));
+      HIP_OK(hipStreamSynchronize(ws->stream));
+      HIP_OK(hipFree(ws->d_bodies));
+    }
+    ws->d_bodies = nb;
+    ws->cap_bodies = cap;
+  }
+  if (!n) return 0;
+  std::vector<uint64_t> g(desc);
+  for (uint64_t k = 0; k < n; ++k) {
+    const int s = slots[k];
+    if (s < 0 || s > 255 || !ws->d_datap[s] || g[3 * k + 2] + g[3 * k + 1] > ws->cap_datap[s] ||
+        g[3 * k] + g[3 * k + 1] > end) {
+      std::snprintf(g_err, sizeof g_err, "gather_bodies: chunk %llu out of range", (unsigned long long)k);
+      return 1;
+    }
+    g[3 * k + 2] += (uint64_t)(uintptr_t)ws->d_datap[s];
+  }
+  size_t cap = ws->cap_gather;
+  CHK(ensure((void **)&ws->d_gather, &cap, g.size() * sizeof(uint64_t)));
+  ws->cap_gather = cap;
+  HIP_OK(hipMemcpyAsync(ws->d_gather, g.data(), g.size() * sizeof(uint64_t), hipMemcpyHostToDevice, ws->stream));
+  hipLaunchKernelGGL(gather_abs_kernel, dim3((uint32_t)std::min<uint64_t>(n, 65535)), dim3(256), 0, ws->stream,
+                     ws->d_gather, n, ws->d_bodies);
+  HIP_OK(hipGetLastError());
+  // the descriptor upload reads g before this returns
+  HIP_OK(hipStreamSynchronize(ws->stream));
+  return 0;
+}
+
+int body_d2h(Workspace *ws, uint8_t *dst, uint64_t off, uint64_t n) {
+  if (off + n > ws->cap_bodies) {
+    std::snprintf(g_err, sizeof g_err, "body_d2h: range outside the body buffer");
+    return 1;
+  }
+  HIP_OK(hipMemcpyAsync(dst, ws->d_bodies + off, n, hipMemcpyDeviceToHost, ws->stream));
+  return 0;
+}
+
 int sync(Workspace *ws) {
   HIP_OK(hipStreamSynchronize(ws->stream));
   return 0;
@@ -1920,6 +1981,20 @@ int read_norm(Workspace *ws, double mm[4]) {
   HIP_OK(hipStreamSynchronize(ws->stream));
   std::memcpy(mm, ws->h_pinned, 4 * sizeof(double));
   return 0;
+}
+
+void *host_alloc(size_t bytes) {
+  void *p = nullptr;
+  if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    std::snprintf(g_err, sizeof g_err, "hipHostMalloc of %zu bytes failed", bytes);
+    return nullptr;
+  }
+  return p;
+}
+
+void host_free(void *p) {
+  if (p) (void)hipHostFree(p);
 }
 
 int is_device_pointer(const void *p) {

@@ -45,6 +45,8 @@ struct Workspace {
   uint32_t *d_flags = nullptr;    // [0] overflow, [1] slope_min, [2] slope_max
   uint8_t *d_packed = nullptr;
   uint64_t *d_gather = nullptr;
+  uint8_t *d_bodies = nullptr;    // packet bodies of the codestream being written
+  size_t cap_bodies = 0;
   size_t cap_cb = 0, cap_q = 0, cap_data = 0, cap_packed = 0, cap_gather = 0, cap_state = 0;
   // fast tier-1 path: bitplanes (kept per parameter slot until the next
   // compression, so rate control can have codeblocks coded further),
@@ -119,6 +121,12 @@ int t1_extend(Workspace *ws, const bwc_codec *c, const Param &par, CblkRecords &
               const std::vector<uint64_t> &cbs, const uint8_t *zlim);
 // gather each codeblock's prefix [0, len[i]) into rec.data / rec.data_off
 int gather(Workspace *ws, CblkRecords &rec, const std::vector<uint32_t> &len);
+// Packet bodies on the device: desc holds (offset in the body buffer,
+// length, source offset in the coded data of parameter slot slots[k]) per
+// chunk, all below `end`; the body buffer keeps its contents when it grows.
+// body_d2h copies n body bytes at offset off to host memory.
+int gather_bodies(Workspace *ws, const std::vector<uint64_t> &desc, const std::vector<int> &slots, uint64_t end);
+int body_d2h(Workspace *ws, uint8_t *dst, uint64_t off, uint64_t n);
 int sync(Workspace *ws);
 const char *last_error();
 int ensure_work(Workspace *ws, uint64_t n);
@@ -126,6 +134,8 @@ int record(Workspace *ws, int i);
 double elapsed(Workspace *ws, int a, int b);
 int read_norm(Workspace *ws, double mm[4]);
 int is_device_pointer(const void *p);
+void *host_alloc(size_t bytes);   // page-locked
+void host_free(void *p);
 int copy_h2d(Workspace *ws, void *dst, const void *src, size_t n);
 int copy_d2h(Workspace *ws, void *dst, const void *src, size_t n);
 int device_count();

@@ -7,11 +7,16 @@
 // :895-1254 (quality layers and the bisection); tagtree.c:310-373;
 // bitstream.c:277-363, :561-612; codestream.c:139-570, :1329-1413.
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <thread>
+#include <condition_variable>
+#include <functional>
+#include <memory>
+#include <mutex>
 #include <unordered_map>
 
 #include "codec.hpp"
@@ -19,6 +24,95 @@
 namespace bwc {
 
 namespace {
+
+// Worker threads for the rate-control probes, started once and kept for the
+// process. run(n, fn) calls fn(0) .. fn(n - 1) on the workers and the calling
+// thread and returns when all are done; every worker takes part in every run
+// (possibly with no task), so none can carry a finished run's function into
+// the next.
+class ProbePool {
+ public:
+  explicit ProbePool(int n) : nworkers_(n) {
+    for (int i = 0; i < n; ++i) th_.emplace_back([this] { worker(); });
+  }
+  ~ProbePool() {
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto &t : th_) t.join();
+  }
+  int size() const { return nworkers_; }
+  void run(size_t n, const std::function<void(size_t)> &fn) {
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      fn_ = &fn;
+      n_ = n;
+      next_ = 0;
+      finished_ = 0;
+      acked_ = 0;
+      ++gen_;
+    }
+    cv_.notify_all();
+    size_t done = 0;
+    for (size_t a; (a = next_.fetch_add(1)) < n;) {
+      fn(a);
+      ++done;
+    }
+    std::unique_lock<std::mutex> lk(m_);
+    finished_ += done;
+    done_cv_.wait(lk, [&] { return finished_ == n_ && acked_ == nworkers_; });
+  }
+
+ private:
+  void worker() {
+    uint64_t seen = 0;
+    for (;;) {
+      const std::function<void(size_t)> *f;
+      size_t n;
+      {
+        std::unique_lock<std::mutex> lk(m_);
+        cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+        if (stop_) return;
+        seen = gen_;
+        f = fn_;
+        n = n_;
+      }
+      size_t done = 0;
+      for (size_t a; (a = next_.fetch_add(1)) < n;) {
+        (*f)(a);
+        ++done;
+      }
+      {
+        std::lock_guard<std::mutex> lk(m_);
+        finished_ += done;
+        ++acked_;
+      }
+      done_cv_.notify_all();
+    }
+  }
+  int nworkers_;
+  std::vector<std::thread> th_;
+  std::mutex m_;
+  std::condition_variable cv_, done_cv_;
+  const std::function<void(size_t)> *fn_ = nullptr;
+  size_t n_ = 0;
+  std::atomic<size_t> next_{0};
+  size_t finished_ = 0;
+  int acked_ = 0;
+  uint64_t gen_ = 0;
+  bool stop_ = false;
+};
+
+// the process's pool with `workers` threads (rebuilt if another size is asked)
+ProbePool &probe_pool(int workers) {
+  static std::unique_ptr<ProbePool> pool;
+  static std::mutex m;
+  std::lock_guard<std::mutex> lk(m);
+  if (!pool || pool->size() != workers) pool.reset(new ProbePool(workers));
+  return *pool;
+}
 
 // MSB-first bit writer with JPEG 2000 style stuffing: after an emitted 0xFF
 // byte the next byte carries only 7 bits.
@@ -73,15 +167,52 @@ struct ByteSink {
   std::vector<uint8_t> out;
   void operator()(uint8_t b) { out.push_back(b); }
 };
-struct CountSink {
-  uint64_t n = 0;
-  void operator()(uint8_t) { ++n; }
-};
-
 using BitWriter = StuffedBits<ByteSink>;
-// Counting writer for the estimation modes: same bit-stuffing arithmetic,
-// no buffer.
-using BitCounter = StuffedBits<CountSink>;
+
+// Counting writer for the estimation modes: the byte count StuffedBits
+// would emit, from a 64-bit accumulator drained a byte at a time. A byte is
+// 0xFF only when it holds 8 bits (a 7-bit byte's top bit is the stuffed 0),
+// and flush() counts a last byte when bits are pending or when the byte
+// before was 0xFF (StuffedBits then holds a 7-bit byte with nothing in it).
+struct BitCounter {
+  struct {
+    uint64_t n = 0;
+  } sink;
+  uint64_t acc = 0;   // pending bits in the low `na` positions
+  int na = 0;
+  int cap = 8;        // bits of the byte being filled: 8, or 7 after 0xFF
+  void drain() {
+    while (na >= cap) {
+      const uint32_t b = (uint32_t)(acc >> (na - cap)) & ((1u << cap) - 1u);
+      na -= cap;
+      ++sink.n;
+      cap = (cap == 8 && b == 0xFF) ? 7 : 8;
+    }
+  }
+  void put(uint64_t v, int n) {   // n <= 32
+    acc = (acc << n) | (v & ((1ull << n) - 1ull));
+    na += n;
+    drain();
+  }
+  void bit(uint64_t b) { put(b ? 1 : 0, 1); }
+  void ones(int n) {
+    while (n > 0) {
+      const int k = n < 32 ? n : 32;
+      put(0xFFFFFFFFull, k);
+      n -= k;
+    }
+  }
+  void zeros(int n) {
+    while (n > 0) {
+      const int k = n < 32 ? n : 32;
+      put(0, k);
+      n -= k;
+    }
+  }
+  void flush() {
+    if (na > 0 || cap == 7) ++sink.n;
+  }
+};
 
 // Raw (unstuffed) bit string, used to cache the tag-tree bits of one
 // codeblock while the tree states stay fixed during a layer's probes.
@@ -90,17 +221,38 @@ struct BitString {
   size_t base;           // first word of this string in `words`
   uint64_t nbits = 0;
   explicit BitString(std::vector<uint64_t> *w) : words(w), base(w->size()) {}
+  // words up to and including the one holding bit `nbits - 1`
+  void cover() {
+    const size_t need = base + (size_t)((nbits + 63) >> 6);
+    if (words->size() < need) words->resize(need, 0);
+  }
   void bit(uint64_t b) { put(b ? 1 : 0, 1); }
-  void put(uint64_t v, int n) {
-    for (int i = n; i-- > 0;) {
-      const size_t w = base + (size_t)(nbits >> 6);
-      if (w >= words->size()) words->push_back(0);
-      if ((v >> i) & 1) (*words)[w] |= 1ull << (63 - (nbits & 63));
-      ++nbits;
+  void put(uint64_t v, int n) {   // n <= 32
+    if (n <= 0) return;
+    v &= (1ull << n) - 1ull;
+    const size_t w = base + (size_t)(nbits >> 6);
+    const int room = 64 - (int)(nbits & 63);
+    nbits += (uint64_t)n;
+    cover();
+    if (n <= room) {
+      (*words)[w] |= v << (room - n);
+    } else {
+      (*words)[w] |= v >> (n - room);
+      (*words)[w + 1] |= v << (64 - (n - room));
     }
   }
-  void ones(int n) { for (int i = 0; i < n; ++i) put(1, 1); }
-  void zeros(int n) { for (int i = 0; i < n; ++i) put(0, 1); }
+  void ones(int n) {
+    while (n > 0) {
+      const int k = n < 32 ? n : 32;
+      put(0xFFFFFFFFull, k);
+      n -= k;
+    }
+  }
+  void zeros(int n) {
+    if (n <= 0) return;
+    nbits += (uint64_t)n;
+    cover();
+  }
 };
 
 // Tag-tree coding of one leaf up to `threshold` (tagtree.c:310-373). At each
@@ -132,6 +284,35 @@ void tagtree_encode(TagTree &tt, W &w, uint32_t threshold, uint64_t leaf, int es
     if (!est) tt.threshold[n] = (uint16_t)th;
     if (n == (int64_t)leaf) break;
     n = branch[--nb];
+  }
+}
+
+// tagtree_encode(tt, w, t, leaf, est) for t = 1 .. T in turn (the MSB code
+// of a codeblock), with the leaf's path to the root found once
+template <class W>
+void tagtree_encode_upto(TagTree &tt, W &w, uint32_t T, uint64_t leaf, int est) {
+  int64_t path[64];   // leaf first, root last
+  int np = 0;
+  for (int64_t n = (int64_t)leaf;; n = tt.parent[n]) {
+    path[np++] = n;
+    if (tt.parent[n] < 0) break;
+  }
+  for (uint32_t t = 1; t <= T; ++t) {
+    uint32_t tmin = 0;
+    for (int a = np - 1; a >= 0; --a) {
+      const int64_t n = path[a];
+      uint32_t th = tt.threshold[n];
+      if (th < tmin) th = tmin;
+      const uint16_t v = tt.value[n];
+      if (v >= th && th < t) {
+        const uint32_t hi = std::min<uint32_t>(v, t);
+        w.zeros((int)(hi - th));
+        if (v < t) w.put(1, 1);
+        th = std::min<uint32_t>((uint32_t)v + 1, t);
+      }
+      tmin = std::min<uint32_t>(v, th);
+      if (!est) tt.threshold[n] = (uint16_t)th;
+    }
   }
 }
 
@@ -209,7 +390,7 @@ struct T2 {
           } else {
             tagtree_encode(pr.incl, w, (uint32_t)q + 1, k, est);
             if (cp[q])
-              for (uint32_t t = 1; t <= R.Kmsbs[i] + 1u; ++t) tagtree_encode(pr.msbs, w, t, k, est);
+              tagtree_encode_upto(pr.msbs, w, R.Kmsbs[i] + 1u, k, est);
           }
         } else {
           w.bit((uint64_t)dz);
@@ -263,36 +444,92 @@ struct T2 {
   // a threshold T >= 1, max{z : S[z] >= T}, is found by bisection over these
   // points instead of a scan over all passes. A codeblock whose nonzero
   // slopes are not ordered that way keeps the scan (flag `scan`).
+  // Per parameter, the passes with a nonzero slope of every codeblock
+  // (entries off[i] .. off[i+1]): pass z, slope s and body length len =
+  // L[z - 1] (0 for z = 0), kept together so a probe reads only these
+  // compact arrays; l0 = L[0] per codeblock; scan = slopes not decreasing
+  // (cuts found by a scan of S instead).
   struct HullIndex {
     std::vector<uint32_t> off;
     std::vector<uint8_t> z;
     std::vector<uint16_t> s;
+    std::vector<uint32_t> len;
+    std::vector<uint32_t> l0;
     std::vector<uint8_t> scan;
   };
   std::vector<HullIndex> hull;
+
+  // fn(a, b) over [0, n) in chunks, on the probe threads when there are any
+  void par_for(uint64_t n, const std::function<void(uint64_t, uint64_t)> &fn) {
+    const uint64_t nch = threads > 1 ? std::min<uint64_t>(n, 64) : 1;
+    if (nch <= 1) {
+      fn(0, n);
+      return;
+    }
+    const std::function<void(size_t)> task = [&](size_t k) { fn(n * k / nch, n * (k + 1) / nch); };
+    probe_pool(threads - 1).run((size_t)nch, task);
+  }
 
   void build_hull_index() {
     hull.assign(c->info.nPar, HullIndex{});
     for (int j = 0; j < c->info.nPar; ++j) {
       CblkRecords &R = rec(j);
       HullIndex &H = hull[j];
-      H.off.assign(R.n + 1, 0);
-      H.scan.assign(R.n, 0);
-      for (uint64_t i = 0; i < R.n; ++i) {
-        H.off[i] = (uint32_t)H.z.size();
-        const uint16_t *S = &R.S[i * (kMaxPasses + 1)];
-        uint16_t prev = 0xFFFF;
-        for (int z = 0; z <= R.Z[i]; ++z) {
-          if (!S[z]) continue;
-          if (S[z] > prev) H.scan[i] = 1;
-          prev = S[z];
-          H.z.push_back((uint8_t)z);
-          H.s.push_back(S[z]);
+      const uint64_t n = R.n;
+      H.off.assign(n + 1, 0);
+      H.scan.assign(n, 0);
+      H.l0.assign(n, 0);
+      // entries per codeblock, then their offsets, then the entries
+      par_for(n, [&](uint64_t a, uint64_t b) {
+        for (uint64_t i = a; i < b; ++i) {
+          const uint16_t *S = &R.S[i * (kMaxPasses + 1)];
+          uint32_t k = 0;
+          for (int z = 0; z <= R.Z[i]; ++z) k += S[z] != 0;
+          H.off[i + 1] = k;
         }
-        if (S[0] == 0) H.scan[i] = 1;
-      }
-      H.off[R.n] = (uint32_t)H.z.size();
+      });
+      for (uint64_t i = 0; i < n; ++i) H.off[i + 1] += H.off[i];
+      const uint64_t tot = H.off[n];
+      H.z.assign(tot, 0), H.s.assign(tot, 0), H.len.assign(tot, 0);
+      par_for(n, [&](uint64_t a, uint64_t b) {
+        for (uint64_t i = a; i < b; ++i) {
+          const uint16_t *S = &R.S[i * (kMaxPasses + 1)];
+          const uint32_t *L = &R.L[i * kMaxPasses];
+          H.l0[i] = L[0];
+          uint32_t k = H.off[i];
+          uint16_t prev = 0xFFFF;
+          for (int z = 0; z <= R.Z[i]; ++z) {
+            if (!S[z]) continue;
+            if (S[z] > prev) H.scan[i] = 1;
+            prev = S[z];
+            H.z[k] = (uint8_t)z;
+            H.s[k] = S[z];
+            H.len[k] = z > 0 ? L[z - 1] : 0;
+            ++k;
+          }
+          if (S[0] == 0) H.scan[i] = 1;
+        }
+      });
     }
+  }
+
+  // cut of codeblock i at threshold T (as cut_of) and its body length
+  // L[z - 1] (0 for z = 0)
+  int16_t cut_len(int j, const CblkRecords &R, uint64_t i, uint16_t T, uint32_t &len) const {
+    const HullIndex &H = hull[j];
+    if (T >= 1 && !H.scan[i]) {
+      uint32_t lo = H.off[i], hi = H.off[i + 1];
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (H.s[mid] >= T) lo = mid;
+        else hi = mid;
+      }
+      len = H.len[lo];
+      return H.z[lo];
+    }
+    const int16_t z = cut_of(j, R, i, T);
+    len = z > 0 ? R.L[i * kMaxPasses + z - 1] : 0;
+    return z;
   }
 
   int16_t cut_of(int j, const CblkRecords &R, uint64_t i, uint16_t threshold) const {
@@ -367,7 +604,7 @@ struct T2 {
               if (bi.nbits == 0) BI.words.push_back(0);
               BM.woff[i] = (uint32_t)BM.words.size();
               BitString bm{&BM.words};
-              for (uint32_t t = 1; t <= R.Kmsbs[i] + 1u; ++t) tagtree_encode(pr.msbs, bm, t, k, 2);
+              tagtree_encode_upto(pr.msbs, bm, R.Kmsbs[i] + 1u, k, 2);
               BM.nbits[i] = (uint32_t)bm.nbits;
               if (bm.nbits == 0) BM.words.push_back(0);
             }
@@ -427,16 +664,33 @@ struct T2 {
   // same as the sequential probes'.
   struct ProbeBuf {
     std::vector<std::vector<int16_t>> cq;   // per param: cut of layer q per codeblock
+    std::vector<std::vector<uint32_t>> cl;  // per param: body length L[cut - 1] where the cut is new
     std::vector<std::vector<uint8_t>> e;    // per param: contribution flag per (resolution, precinct)
     std::vector<std::vector<uint64_t>> eoff;
   };
+  // per param: L[cut - 1] of the previous layer's cut per codeblock (0 when
+  // nothing is included), set before each layer's probes
+  std::vector<std::vector<uint32_t>> plen;
+  void set_prev_lengths(int q) {
+    plen.assign(c->info.nPar, {});
+    for (int j = 0; j < c->info.nPar; ++j) {
+      const CblkRecords &R = rec(j);
+      plen[j].assign(R.n, 0);
+      for (uint64_t i = 0; i < R.n; ++i) {
+        const int16_t prev = R.cp[i * (nl + 1) + q];
+        if (prev > 0) plen[j][i] = R.L[i * kMaxPasses + prev - 1];
+      }
+    }
+  }
   void probe_init(ProbeBuf &pb) {
     const int np = c->info.nPar;
     pb.cq.assign(np, {});
+    pb.cl.assign(np, {});
     pb.e.assign(np, {});
     pb.eoff.assign(np, {});
     for (int j = 0; j < np; ++j) {
       pb.cq[j].assign(rec(j).n, 0);
+      pb.cl[j].assign(rec(j).n, 0);
       uint64_t o = 0;
       for (auto &res : tl->par[j].res) {
         pb.eoff[j].push_back(o);
@@ -446,13 +700,14 @@ struct T2 {
     }
   }
 
+  // length_code for est 2: lcur = L[cur - 1], lprev = L[prev - 1] (0 for prev 0)
   template <class W>
-  void length_code_probe(W &w, const CblkRecords &R, uint64_t i, int16_t prev, int16_t cur) const {
+  void length_code_probe(W &w, const CblkRecords &R, uint64_t i, int16_t prev, int16_t cur, uint32_t lcur,
+                         uint32_t lprev) const {
     const int16_t zc = (int16_t)(cur - 1), zp = (int16_t)(prev - 1);
     uint16_t beta = R.beta_est[i];
     uint16_t maxb = beta;
-    const uint32_t *L = &R.L[i * kMaxPasses];
-    const uint32_t len = L[zc] - ((zp >= 0) ? L[zp] : 0);
+    const uint32_t len = lcur - lprev;
     const uint16_t cpb = ilog2_u32((uint32_t)(zc - zp));
     int32_t lb = ilog2_u32(len) + 1 - cpb;
     if (maxb < lb) maxb = (uint16_t)lb;
@@ -468,7 +723,10 @@ struct T2 {
     for (int j = 0; j < c->info.nPar; ++j) {
       Param &par = tl->par[j];
       CblkRecords &R = rec(j);
+      const HullIndex &H = hull[j];
+      const uint32_t *pl = plen[j].data();
       int16_t *cq = pb.cq[j].data();
+      uint32_t *cl = pb.cl[j].data();
       uint8_t *ef = pb.e[j].data();
       std::fill(pb.e[j].begin(), pb.e[j].end(), 0);
       for (size_t r = 0; r < par.res.size(); ++r) {
@@ -480,20 +738,21 @@ struct T2 {
             for (uint64_t k = 0; k < pr.ncbs; ++k) {
               const uint64_t i = pr.first_cb + k;
               const int16_t prev = R.cp[i * (nl + 1) + q];
-              const uint32_t *L = &R.L[i * kMaxPasses];
-              const int16_t z = cut_of(j, R, i, T);
+              uint32_t zl;
+              const int16_t z = cut_len(j, R, i, T, zl);
               int16_t cur;
               if (z < prev) {
                 cur = prev;
-                if (cur == 1) ql += L[0];
+                if (cur == 1) ql += H.l0[i];
               } else {
                 cur = z;
                 if (z > 0) {
-                  ql += L[z - 1];
-                  if (z > 1 && prev > 0) ql -= L[prev - 1];
+                  ql += zl;
+                  if (z > 1 && prev > 0) ql -= pl[i];
                 }
               }
               cq[i] = cur;
+              cl[i] = zl;
               if (cur > prev) er[p] = 1;
             }
           }
@@ -517,14 +776,14 @@ struct T2 {
                 } else {   // probes leave tree thresholds alone (est 2)
                   tagtree_encode(sb.prec[p].incl, w, (uint32_t)q + 1, k, 2);
                   if (cur)
-                    for (uint32_t t = 1; t <= R.Kmsbs[i] + 1u; ++t) tagtree_encode(sb.prec[p].msbs, w, t, k, 2);
+                    tagtree_encode_upto(sb.prec[p].msbs, w, R.Kmsbs[i] + 1u, k, 2);
                 }
               } else {
                 w.bit((uint64_t)dz);
               }
-              if (dz) {
+              if (dz) {   // dz > 0: cur is this probe's cut, cl[i] its length
                 delta_z_code(w, dz);
-                length_code_probe(w, R, i, prev, cur);
+                length_code_probe(w, R, i, prev, cur, cl[i], prev > 0 ? pl[i] : 0);
               }
             }
           }
@@ -549,9 +808,8 @@ struct T2 {
     if (Ts.size() == 1 || threads <= 1) {
       for (size_t a = 0; a < Ts.size(); ++a) out[a] = probe(Ts[a], q, pbufs[0]);
     } else {
-      std::vector<std::thread> th;
-      for (size_t a = 0; a < Ts.size(); ++a) th.emplace_back([&, a] { out[a] = probe(Ts[a], q, pbufs[a]); });
-      for (auto &t : th) t.join();
+      const std::function<void(size_t)> fn = [&](size_t a) { out[a] = probe(Ts[a], q, pbufs[a]); };
+      probe_pool(threads - 1).run(Ts.size(), fn);
     }
     for (size_t a = 0; a < Ts.size(); ++a) memo[Ts[a]] = out[a];
   }
@@ -622,6 +880,7 @@ struct T2 {
     for (int l = 0; l < nl; ++l) {
       const int64_t target = (int64_t)std::ceil(c->control.bitrate[l] * tsz / 8.0f);
       uint16_t slope;
+      set_prev_lengths(l);
       if (target > 0.0f) {
         build_layer_cache(l);
         uint16_t smin = tl->slope_min, smax = tl->slope_max, tmp = tl->slope_min;
@@ -635,7 +894,7 @@ struct T2 {
           if (threads > 0) {
             if (!memo.count(slope)) {
               std::vector<uint16_t> Ts;
-              frontier(smin, smax, threads >= 7 ? 3 : (threads >= 3 ? 2 : 1), Ts);
+              frontier(smin, smax, threads >= 15 ? 4 : (threads >= 7 ? 3 : (threads >= 3 ? 2 : 1)), Ts);
               probe_many(Ts, l, memo);
             }
             f = memo[slope];
@@ -680,13 +939,15 @@ struct T2 {
 int t2_rate_control(bwc_codec *codec, Job *job, uint64_t tile, SkipCheck *chk) {
   T2 t{codec, job, &job->plan.tiles[tile], tile, codec->control.nLayers};
   // host threads for the bisection probes: BWC_AMD_RC_THREADS (0 = the
-  // sequential in-place probes, 1 = side-effect-free probes on one thread)
+  // sequential in-place probes, 1 = side-effect-free probes on one thread);
+  // by default one less than the host threads, at most 15 (the bisection's
+  // next four levels at once)
   int rc_threads;
   if (const char *e = std::getenv("BWC_AMD_RC_THREADS")) {
     rc_threads = std::max(0, std::min(16, std::atoi(e)));
   } else {
     const unsigned hc = std::thread::hardware_concurrency();
-    rc_threads = (int)std::max(1u, std::min(7u, hc > 1 ? hc - 1 : 1u));
+    rc_threads = (int)std::max(1u, std::min(15u, hc > 1 ? hc - 1 : 1u));
   }
   t.threads = rc_threads;
   if (chk) {
@@ -747,9 +1008,30 @@ struct ByteWriter {
 };
 }  // namespace
 
+void packet_body_chunks(bwc_codec *codec, Job *job, uint64_t tile, int j, int r, uint64_t p, int l,
+                        std::vector<std::pair<uint64_t, uint64_t>> &out) {
+  const int nl = codec->control.nLayers;
+  Resolution &res = job->plan.tiles[tile].par[j].res[r];
+  const CblkRecords &R = job->rec[tile * codec->info.nPar + j];
+  for (auto &sb : res.sb) {
+    const Precinct &pr = sb.prec[p];
+    for (uint64_t c = 0; c < pr.ncbs; ++c) {
+      const uint64_t i = pr.first_cb + c;
+      const int16_t *cp = &R.cp[i * (nl + 1) + 1];
+      if (cp[l] - cp[l - 1] <= 0) continue;
+      const uint32_t *L = &R.L[i * kMaxPasses];
+      const uint32_t start = (cp[l - 1] == 0) ? 0 : L[cp[l - 1] - 1];
+      const uint32_t end = (cp[l] == 0) ? 0 : L[cp[l] - 1];
+      out.emplace_back(R.src_off[i] + start, (uint64_t)(end - start));
+    }
+  }
+}
+
 // Packet sequencing and codestream assembly (codestream.c:139-570,
-// :1329-1413). Body bytes are copied straight from the packed codeblock data.
-size_t assemble_codestream(bwc_codec *codec, Job *job, const bwc_stream *st, uint8_t *out) {
+// :1329-1413). Body bytes are copied straight from the packed codeblock data,
+// or per packet through `bodies`.
+size_t assemble_codestream(bwc_codec *codec, Job *job, const bwc_stream *st, uint8_t *out,
+                           const BodySource *bodies) {
   bwc_gl_ctrl &ctl = codec->control;
   bwc_gl_inf &inf = codec->info;
   const int nl = ctl.nLayers;
@@ -852,7 +1134,16 @@ size_t assemble_codestream(bwc_codec *codec, Job *job, const bwc_stream *st, uin
             }
             w.chunk(pk.header.data(), pk.header.size());
             if (ctl.error_resilience) w.sym(0xFF92, 2);
-            if (pk.e) {
+            if (pk.e && bodies) {
+              if (pk.body_size) {
+                if (w.err || w.L + pk.body_size > w.Lmax ||
+                    bodies->copy(bodies->ctx, w.m + w.L, pk.dev_off, pk.body_size)) {
+                  w.err = true;
+                } else {
+                  w.L += pk.body_size;
+                }
+              }
+            } else if (pk.e) {
               for (auto &sb : res.sb) {
                 Precinct &pr = sb.prec[p];
                 for (uint64_t c = 0; c < pr.ncbs; ++c) {

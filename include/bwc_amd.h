@@ -25,6 +25,13 @@ int bwc_amd_version(void);
 int bwc_amd_device_count(void);
 const char *bwc_amd_last_error(void);
 
+/* Page-locked host memory (no counterpart in the reference). A codestream
+   output buffer allocated here receives the packet bodies by DMA straight
+   from the GPU; any other host buffer works too, through a staged copy.
+   bwc_amd_host_alloc returns NULL on failure. */
+void *bwc_amd_host_alloc(size_t bytes);
+void bwc_amd_host_free(void *p);
+
 /* The energy-gain table is process-global and accumulates on every codec
    configuration and bwc_set_decomp call, as in the reference (dwt.c:80,
    :808-1066). This returns it to the state of a fresh process, so a test or
@@ -76,8 +83,9 @@ size_t bwc_amd_encode_records(bwc_codec *codec, bwc_stream *stream);
 
 /* device-side stage times of the last bwc_compress on this codec, in ms:
    [0] fill+minmax [1] normalize [2] DWT [3] quantize [4] tier-1
-   [5] records D2H [6] rate control (host) [7] gather+D2H [8] packets+assembly
-   (host) [9] total; launches[0] = number of DWT kernel launches */
+   [5] records D2H [6] rate control (host) [7] packet headers (host) and the
+   device gather of packet bodies [8] codestream assembly with the bodies'
+   D2H [9] total; launches[0] = number of DWT kernel launches */
 int bwc_amd_last_timings(bwc_codec *codec, double ms[10], int launches[4]);
 
 /* Pass skipping (no counterpart in the reference; the codestream stays the
