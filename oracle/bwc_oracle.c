@@ -150,6 +150,7 @@ typedef struct
 {
   box_t b;                 /* tile-local buffer coordinates */
   uint32_t idx;
+  uint64_t aidx;           /* position in the parameter's access order */
   uint8_t K, Kmsbs, Z;
   uint64_t L[MAXP];
   uint16_t S[MAXP + 1];
@@ -608,6 +609,7 @@ static int create_geometry(oc_codec *c)
                       par->acc[cblk].pr = &res->sb[m].prec[p];
                       par->acc[cblk].cb = &res->sb[m].prec[p].cb[k];
                       par->acc[cblk].cb->idx = (uint32_t)k;
+                      par->acc[cblk].cb->aidx = cblk;
                     }
                   m++;
                 }
@@ -657,6 +659,20 @@ oc_codec *oc_create(const oc_params *p)
     set_qt(c, delta);
   }
 
+  /* header-carried settings (codestream.c:710-790): quantisation style,
+     step size, progression, kernels; the decomposition setter below
+     recomputes the step size again */
+  if (p->have_qstyle) c->qstyle = p->qstyle, c->CSsgc |= (0x01 << 1);
+  if (p->have_qt) c->qt_exp = p->qt_exp, c->qt_mant = p->qt_mant;
+  if (p->have_prog) c->prog = p->prog, c->CSsgc |= (0x01 << 3);
+  if (p->have_kern)
+  {
+    c->kern[0] = p->kern[0] ? p->kern[0] : 0;
+    c->kern[1] = p->kern[1] ? p->kern[1] : 0;
+    c->kern[2] = p->kern[2] ? p->kern[2] : 0;
+    c->kern[3] = p->kern[3] ? p->kern[3] : 2;   /* set_kernels: Haar when 0 for the temporal axis */
+    c->CSsgc |= (0x01 << 4);
+  }
   if (p->have_decomp)
   {
     uint8_t d[4];
@@ -726,8 +742,14 @@ oc_codec *oc_create(const oc_params *p)
     c->CSsgc ^= (0x01 << 0);
   }
 
-  /* rate-control string (libbwc.c:3024-3089) */
-  if (p->rate == NULL)
+  /* rate-control string (libbwc.c:3024-3089), or the layer rates as read
+     from a codestream header */
+  if (p->nrates)
+  {
+    for (int l = 0; l < p->nrates; ++l) c->bitrate[l] = p->rates[l];
+    c->nlayers = p->nrates;
+  }
+  else if (p->rate == NULL)
   {
     c->bitrate[0] = PREC_BITS + 1;
     c->nlayers = 1;

@@ -13,6 +13,11 @@
  *     reference built from source (oracle/_ref, tests/golden fixtures),
  *     plus closed-form checks (DWT DC gain and vanishing moments, MQ
  *     round trip through the decoder, tag-tree textbook cases).
+ *   decompression (oc_decompress: header and packet parsing, tier-1
+ *   decoding, reconstruction, inverse DWT, denormalisation): the output
+ *     bytes of the reference's bwc_decompress on every golden codestream
+ *     (SHA-256 and samples in tests/golden), and decoded quality growing
+ *     with the rate.
  */
 #ifndef BWC_ORACLE_H
 #define BWC_ORACLE_H
@@ -37,6 +42,15 @@ typedef struct oc_params
   uint64_t tiles[4];   /* tile sizes (bwc_tile_sizeof) */
   uint8_t  erres;
   const char *rate;    /* e.g. "2" or "1,2,4,8"; NULL = lossless-ish default */
+  /* settings a codestream header can carry beyond the above (decoder side):
+     applied in the order of codestream.c:710-790, before the decomposition */
+  uint8_t  have_qstyle, qstyle;
+  uint8_t  have_qt, qt_exp;
+  uint16_t qt_mant;
+  uint8_t  have_prog, prog;
+  uint8_t  have_kern, kern[4];
+  uint8_t  nrates;     /* > 0: these layer rates as read (no parsing of rate) */
+  const float *rates;
 } oc_params;
 
 typedef struct oc_codec oc_codec;

@@ -98,6 +98,14 @@ def main():
         ent["codestream_sha256"] = hashlib.sha256(cs).hexdigest()
         ent["codestream_size"] = len(cs)
         ent["files"] = []
+        # the reference decoder's output for this codestream (bwc_decompress)
+        dec = ol.ref_decode(cs, shape, npar=case["npar"], prec=case["prec"])
+        ent["decode_sha256"] = hashlib.sha256(dec).hexdigest()
+        ent["decode_size"] = len(dec)
+        vals = np.frombuffer(dec, np.float64 if case["prec"] == 8 else np.float32)
+        np.savez_compressed(HERE / f"{case['name']}.dec.npz", sample=vals[::53].copy(),
+                            n=np.array([vals.size], np.uint64))
+        ent["files"].append(f"{case['name']}.dec.npz")
         if case["store"]:
             (HERE / f"{case['name']}.bwc").write_bytes(cs)
             ent["files"].append(f"{case['name']}.bwc")

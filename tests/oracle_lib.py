@@ -38,6 +38,17 @@ class OcParams(C.Structure):
         ("tiles", C.c_uint64 * 4),
         ("erres", C.c_uint8),
         ("rate", C.c_char_p),
+        ("have_qstyle", C.c_uint8),
+        ("qstyle", C.c_uint8),
+        ("have_qt", C.c_uint8),
+        ("qt_exp", C.c_uint8),
+        ("qt_mant", C.c_uint16),
+        ("have_prog", C.c_uint8),
+        ("prog", C.c_uint8),
+        ("have_kern", C.c_uint8),
+        ("kern", C.c_uint8 * 4),
+        ("nrates", C.c_uint8),
+        ("rates", C.POINTER(C.c_float)),
     ]
 
 
@@ -306,6 +317,44 @@ def ref_run(mode, data, shape, npar=1, prec=8, rate="2", **kw):
             if os.path.exists(pre + ext):
                 out[ext[1:]] = Path(pre + ext).read_bytes()
         return out
+
+
+def ref_decode(cs, shape, npar=1, prec=8):
+    """Decode a codestream with the reference decoder (bwc_create_decompression
+    + bwc_decompress in the driver); returns the output bytes."""
+    n = list(shape) + [1] * (4 - len(shape))
+    with tempfile.TemporaryDirectory() as td:
+        pre = os.path.join(td, "o")
+        Path(pre + ".bwc").write_bytes(cs)
+        cmd = [str(REF_DUMP), "decode", "-"] + [str(int(x)) for x in n] + [str(npar), str(prec), "2", pre]
+        subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        return Path(pre + ".dec").read_bytes()
+
+
+def oracle_decompress(cs, layer=0):
+    """The oracle's decoder: (decoded values as a flat array, dims (nX, nY,
+    nZ, nTS, nPar, prec))."""
+    L = lib()
+    buf = np.frombuffer(cs, np.uint8).copy()
+    dims = (C.c_uint64 * 6)()
+    # the header gives the output size; decode once into a buffer large
+    # enough for any field the header can describe here
+    hdr = _header_dims(cs)
+    nout = int(np.prod(hdr[:5])) * hdr[5]
+    out = np.zeros(max(nout, 1), np.uint8)
+    rc = L.oc_decompress(buf.ctypes.data_as(C.POINTER(C.c_uint8)), buf.size, layer, out.ctypes.data, dims)
+    if rc != 0:
+        raise RuntimeError("oc_decompress failed")
+    d = tuple(int(x) for x in dims)
+    return out.view(np.float64 if d[5] == 8 else np.float32), d
+
+
+def _header_dims(cs):
+    """(nX, nY, nZ, nTS, nPar, data_prec) from the SGI marker segment."""
+    i = 10   # after SOC and the codestream length
+    assert cs[i:i + 2] == b"\xff\x51", "no SGI marker segment after SOC"
+    v = [int.from_bytes(cs[i + 4 + 8 * a:i + 12 + 8 * a], "big") for a in range(4)]
+    return v + [cs[i + 36], cs[i + 37]]
 
 
 def parse_t1(blob, ncblk_per_param):
