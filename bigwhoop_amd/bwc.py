@@ -414,3 +414,80 @@ def compress_array(data, shape, npar=1, rate="2", **kw) -> bytes:
         return c.compress(data)
     finally:
         c.close()
+
+
+class _GlInf(C.Structure):
+    _fields_ = [("nX", C.c_uint64), ("nY", C.c_uint64), ("nZ", C.c_uint64), ("nTS", C.c_uint64),
+                ("nPar", C.c_uint8), ("data_prec", C.c_int), ("codec_prec", C.c_int)]
+
+
+def header_info(cs: bytes) -> dict:
+    """bwc_open_header on a codestream: dimensions, parameters, precisions,
+    layers and their rates."""
+    L = lib()
+    buf = np.frombuffer(cs, np.uint8).copy()
+    h = L.bwc_open_header(buf.ctypes.data_as(C.c_void_p))
+    if not h:
+        raise RuntimeError("bwc_open_header failed: " + last_error())
+    try:
+        inf = _GlInf.from_address(h)
+        # bwc_header: info, then bwc_gl_ctrl; nLayers and bitrate are read
+        # through the header layout mirrored in include/bwc.h
+        ctrl = h + C.sizeof(_GlInf)
+        out = dict(shape=(inf.nX, inf.nY, inf.nZ, inf.nTS), npar=inf.nPar, data_prec=inf.data_prec,
+                   codec_prec=inf.codec_prec)
+        off = _ctrl_offsets()
+        nl = C.c_uint8.from_address(ctrl + off["nLayers"]).value
+        rp = C.c_void_p.from_address(ctrl + off["bitrate"]).value
+        out["rates"] = [C.c_float.from_address(rp + 4 * i).value for i in range(nl)] if rp else []
+        return out
+    finally:
+        L.bwc_close_header(C.c_void_p(h))
+
+
+def _ctrl_offsets():
+    class Ctrl(C.Structure):
+        _fields_ = [("CSsgc", C.c_uint16), ("error_resilience", C.c_ubyte), ("tileSizeX", C.c_uint64),
+                    ("tileSizeY", C.c_uint64), ("tileSizeZ", C.c_uint64), ("tileSizeTS", C.c_uint64),
+                    ("nTiles", C.c_uint64), ("precSizeX", C.c_uint8), ("precSizeY", C.c_uint8),
+                    ("precSizeZ", C.c_uint8), ("precSizeTS", C.c_uint8), ("cbX", C.c_uint8), ("cbY", C.c_uint8),
+                    ("cbZ", C.c_uint8), ("cbTS", C.c_uint8), ("KernelX", C.c_int), ("KernelY", C.c_int),
+                    ("KernelZ", C.c_int), ("KernelTS", C.c_int), ("decompX", C.c_uint8), ("decompY", C.c_uint8),
+                    ("decompZ", C.c_uint8), ("decompTS", C.c_uint8), ("nDecomp", C.c_uint8),
+                    ("qt_exponent", C.c_uint32), ("qt_mantissa", C.c_uint32), ("Qm", C.c_uint8),
+                    ("bitrate", C.c_void_p), ("nLayers", C.c_uint8)]
+    return {n: getattr(Ctrl, n).offset for n, _ in Ctrl._fields_}
+
+
+def decompress_bytes(cs: bytes, layer: int = 0, out=None):
+    """bwc_create_decompression + bwc_decompress: the decoded field as a flat
+    array (float64 or float32, [param][t][z][y][x]), and the header info.
+    `out` may be a device tensor (e.g. torch) of the right size to decode
+    into instead (then it is returned)."""
+    L = lib()
+    info = header_info(cs)
+    n = int(np.prod(info["shape"])) * info["npar"]
+    dt = np.float64 if info["data_prec"] == 8 else np.float32
+    buf = np.frombuffer(cs, np.uint8).copy()
+    codec = L.bwc_alloc_decoder()
+    host = None
+    if out is None:
+        host = np.zeros(n, dt)
+        optr = host.ctypes.data
+    else:
+        optr = out.data_ptr()
+    stream = L.bwc_init_stream(buf.ctypes.data_as(C.c_void_p), C.c_void_p(optr), 1)
+    try:
+        if L.bwc_create_decompression(codec, stream, layer):
+            raise RuntimeError("bwc_create_decompression failed: " + last_error())
+        if L.bwc_decompress(codec, stream):
+            raise RuntimeError("bwc_decompress failed: " + last_error())
+    finally:
+        L.bwc_free_codec(codec)
+        st = _Stream.from_address(stream)
+        for span in (st.aux, st.com):
+            if span:
+                C.CDLL(None).free(C.c_void_p(C.c_void_p.from_address(span + 24).value))
+                C.CDLL(None).free(C.c_void_p(span))
+        C.CDLL(None).free(C.c_void_p(stream))
+    return (host if out is None else out), info

@@ -612,22 +612,91 @@ fail:
   return 0;
 }
 
+// bwc_create_decompression (libbwc.c:3453-3470): the codestream's header
+// configures the codec, its packets become per-codeblock decode records
 uchar bwc_create_decompression(bwc_codec *const codec, bwc_stream *const stream, uint8 const layer) {
-  (void)codec, (void)stream, (void)layer;
-  set_err("decompression is not available in this build");
-  return 1;
+  if (!codec || !stream || !stream->inp) {
+    set_err("bwc_create_decompression: codec and codestream required");
+    return 1;
+  }
+  if (parse_codestream(codec, stream, layer)) {
+    set_err("invalid codestream");
+    return 1;
+  }
+  return 0;
 }
 
+// bwc_decompress (libbwc.c:3485-3710): per tile and parameter, tier-1
+// decoding, inverse DWT, denormalisation and output, all on the GPU; the
+// field goes to stream->out (host or device memory)
 uchar bwc_decompress(bwc_codec *const codec, bwc_stream *const stream) {
-  (void)codec, (void)stream;
-  set_err("decompression is not available in this build");
+  Job *job = job_of(codec);
+  if (!job || !job->decodable || !stream || !stream->out) {
+    set_err("bwc_decompress called before bwc_create_decompression");
+    return 1;
+  }
+  gpu::Workspace *ws = gpu::workspace_of(job);
+  if (!ws) {
+    set_err(gpu::last_error());
+    return 1;
+  }
+  const bwc_gl_inf &inf = codec->info;
+  const size_t bytes = (size_t)(inf.nX * inf.nY * inf.nZ * inf.nTS) * inf.nPar * (size_t)inf.data_prec;
+  const bool on_dev = gpu::is_device_pointer(stream->out) != 0;
+  void *dst = stream->out;
+  uint64_t maxsz = 0;
+  for (auto &tl : job->plan.tiles) maxsz = std::max<uint64_t>(maxsz, tl.box.volume());
+  if (gpu::ensure_work(ws, maxsz)) goto fail;
+  if (!on_dev && gpu::ensure_output(ws, bytes, &dst)) goto fail;
+  for (uint64_t ti = 0; ti < job->plan.tiles.size(); ++ti) {
+    Tile &tl = job->plan.tiles[ti];
+    for (int p = 0; p < inf.nPar; ++p) {
+      Param &par = tl.par[p];
+      if (gpu::t1_decode(ws, codec, par, job->drec[ti * inf.nPar + p], ws->d_work)) goto fail;
+      if (gpu::dwt_inverse(ws, codec, par, ws->d_work)) goto fail;
+      if (gpu::decode_output(ws, codec, tl, p, par, ws->d_work, dst)) goto fail;
+    }
+  }
+  if (!on_dev && gpu::copy_d2h(ws, stream->out, dst, bytes)) goto fail;
+  if (gpu::sync(ws)) goto fail;
+  return 0;
+fail:
+  set_err(gpu::last_error());
   return 1;
 }
 
+// bwc_open_header (libbwc.c:2840-2935): the main header alone
 bwc_header *bwc_open_header(void *const inpbuf) {
-  (void)inpbuf;
-  set_err("header parsing is not available in this build");
-  return nullptr;
+  if (!inpbuf) return nullptr;
+  bwc_codec *codec = bwc_alloc_decoder();
+  bwc_stream *data = bwc_init_stream(inpbuf, nullptr, decomp);
+  bwc_header *h = nullptr;
+  if (codec && data && !parse_header_only(codec, data)) {
+    h = static_cast<bwc_header *>(std::calloc(1, sizeof(bwc_header)));
+    if (h) {
+      h->info = codec->info;
+      h->control = codec->control;
+      h->control.bitrate = static_cast<float *>(std::calloc(codec->control.nLayers ? codec->control.nLayers : 1,
+                                                            sizeof(float)));
+      if (codec->control.nLayers)
+        std::memcpy(h->control.bitrate, codec->control.bitrate, codec->control.nLayers * sizeof(float));
+      if (data->codestream.aux) {
+        h->aux.memory = data->codestream.aux->memory;
+        h->aux.size = data->codestream.aux->size;
+        std::free(data->codestream.aux);
+      }
+      if (data->codestream.com) {
+        h->com.memory = data->codestream.com->memory;
+        h->com.size = data->codestream.com->size;
+        std::free(data->codestream.com);
+      }
+    }
+  } else {
+    set_err("invalid codestream header");
+  }
+  bwc_free_codec(codec);
+  std::free(data);
+  return h;
 }
 
 void bwc_close_header(bwc_header *const header) {

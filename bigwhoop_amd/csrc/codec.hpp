@@ -87,6 +87,17 @@ struct Param {
   std::vector<uint32_t> cb_idx;      // index within its precinct
 };
 
+// Decoder-side records of one (tile, param), per codeblock in access order:
+// bit-planes skipped (Kmsbs), passes included (Z), coded bytes (len) and
+// their offset in `data`.
+struct DecodeRecords {
+  uint64_t n = 0;
+  std::vector<uint8_t> Kmsbs, Z;
+  std::vector<uint32_t> len;
+  std::vector<uint64_t> off;   // n + 1
+  std::vector<uint8_t> data;
+};
+
 struct Tile {
   Box box;
   uint32_t index = 0;
@@ -138,8 +149,9 @@ struct Job {
   bool created = false;
   // per (tile, param) records, in tile-major order
   std::vector<CblkRecords> rec;
-  // decoder-side
-  std::vector<uint8_t> cs_copy;
+  // decoder-side: per (tile, param) records from the packets
+  std::vector<DecodeRecords> drec;
+  bool decodable = false;
   // device workspace (gpu::Workspace), created on first use
   void *ws = nullptr;
   void (*ws_free)(void *) = nullptr;
@@ -200,8 +212,11 @@ void packet_body_chunks(bwc_codec *codec, Job *job, uint64_t tile, int j, int r,
 uint64_t header_size(const bwc_codec *codec, const bwc_stream *st);
 
 // ---- decode (decode.cpp) ----
-int parse_codestream(bwc_codec *codec, Job *job, const uint8_t *cs, uint8_t layer);
-int decompress(bwc_codec *codec, Job *job, void *out);
+// bwc_create_decompression: main header into the codec (configuration,
+// geometry, parameter ranges), then every packet into per-codeblock decode
+// records (job->drec). bwc_open_header uses the header part alone.
+int parse_codestream(bwc_codec *codec, bwc_stream *data, uint8_t layer);
+int parse_header_only(bwc_codec *codec, bwc_stream *data);
 
 void warn(const char *msg);
 

@@ -45,7 +45,8 @@ Workspace::~Workspace() {
   void *ptrs[] = {d_input, d_work,  d_work2, d_red,  d_norm, d_cbbox, d_cbsb, d_cbmeta, d_qoff,   d_doff, d_q,
                   d_state, d_data,  d_K,    d_Kmsbs, d_Z,    d_L,    d_S,      d_snap,   d_mse,  d_flags,
                   d_packed, d_gather, d_bp, d_bpoff, d_sym, d_symoff, d_pcnt, d_pmse, d_list, d_U, d_zlim,
-                  d_feat,  d_tstat, d_skipw, d_bodies};
+                  d_feat,  d_tstat, d_skipw, d_bodies, dec.box, dec.sb, dec.meta, dec.K, dec.Z, dec.len,
+                  dec.off, dec.data, dec.soff, dec.state, dec.out};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (auto &b : bp_slots)
@@ -1333,6 +1334,7 @@ __global__ void bitplane_kernel(const double *__restrict__ work, uint64_t W, uin
 }
 
 #include "t1v2.inc"
+#include "t1dec.inc"
 
 __global__ void zero_f32_kernel(float *p, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.f;
@@ -1833,6 +1835,75 @@ int gather(Workspace *ws, CblkRecords &rec, const std::vector<uint32_t> &len) {
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpyAsync(rec.data.data(), ws->d_packed, tot, hipMemcpyDeviceToHost, ws->stream));
   HIP_OK(hipStreamSynchronize(ws->stream));
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// decompression: tier-1 decoding of one (tile, param) into the zeroed tile
+// buffer, then (after the inverse DWT) denormalisation into the output
+// ---------------------------------------------------------------------------
+int t1_decode(Workspace *ws, const bwc_codec *c, const Param &par, const DecodeRecords &rec, double *work) {
+  const uint64_t ncb = par.ncb, n = par.box.volume();
+  HIP_OK(hipMemsetAsync(work, 0, n * sizeof(double), ws->stream));
+  if (!ncb) return 0;
+  if (rec.n != ncb) {
+    std::snprintf(g_err, sizeof g_err, "decode records do not match the codec's codeblocks");
+    return 1;
+  }
+  CbTables tb;
+  build_tables(par, tb);
+  std::vector<uint8_t> K(ncb);
+  std::vector<uint64_t> soff(ncb + 1);
+  for (uint64_t i = 0; i < ncb; ++i) {
+    const Subband &s = par.res[par.cb_sbref[i] >> 8].sb[par.cb_sbref[i] & 0xFF];
+    K[i] = (uint8_t)(s.Kmax - rec.Kmsbs[i]);   // uint8 arithmetic, as tier1.c:3316
+    soff[i] = tb.qoff[i] / 4;
+  }
+  soff[ncb] = tb.qo / 4;
+  auto up = [&](void **p, size_t *cap, const void *src, size_t bytes) -> int {
+    CHK(ensure(p, cap, bytes));
+    HIP_OK(hipMemcpyAsync(*p, src, bytes, hipMemcpyHostToDevice, ws->stream));
+    return 0;
+  };
+  CHK(up((void **)&ws->dec.box, &ws->dec.cap_box, par.cb_box.data(), 8 * ncb * sizeof(uint32_t)));
+  CHK(up((void **)&ws->dec.sb, &ws->dec.cap_sb, tb.sb.data(), 3 * ncb * sizeof(double)));
+  CHK(up((void **)&ws->dec.meta, &ws->dec.cap_meta, tb.meta.data(), ncb * sizeof(uint32_t)));
+  CHK(up((void **)&ws->dec.K, &ws->dec.cap_K, K.data(), ncb));
+  CHK(up((void **)&ws->dec.Z, &ws->dec.cap_Z, rec.Z.data(), ncb));
+  CHK(up((void **)&ws->dec.len, &ws->dec.cap_len, rec.len.data(), ncb * sizeof(uint32_t)));
+  CHK(up((void **)&ws->dec.off, &ws->dec.cap_off, rec.off.data(), (ncb + 1) * sizeof(uint64_t)));
+  CHK(up((void **)&ws->dec.data, &ws->dec.cap_data, rec.data.data(), rec.data.size()));
+  CHK(up((void **)&ws->dec.soff, &ws->dec.cap_soff, soff.data(), (ncb + 1) * sizeof(uint64_t)));
+  CHK(ensure((void **)&ws->dec.state, &ws->dec.cap_state, (soff[ncb] + 1) * sizeof(uint16_t)));
+  const uint64_t W = par.box.len(0), H = par.box.len(1), D = par.box.len(2);
+  constexpr int LPW = 8;
+  hipLaunchKernelGGL(t1_decode_kernel<LPW>, dim3((uint32_t)((ncb + LPW - 1) / LPW)), dim3(64), 0, ws->stream, ncb,
+                     ws->dec.box, ws->dec.sb, ws->dec.meta, ws->dec.K, ws->dec.Z, ws->dec.len, ws->dec.off,
+                     ws->dec.data, ws->dec.soff, ws->dec.state, W, H, D, work, (int)c->control.error_resilience);
+  HIP_OK(hipGetLastError());
+  // the host records are read by the uploads above before this returns
+  HIP_OK(hipStreamSynchronize(ws->stream));
+  return 0;
+}
+
+int decode_output(Workspace *ws, const bwc_codec *c, const Tile &tl, int p, const Param &par, const double *work,
+                  void *out) {
+  const uint64_t n = par.box.volume();
+  if (!n) return 0;
+  const double alpha = (par.pmax + par.pmin) / 2.0f, beta = (par.pmax - par.pmin) / 2.0f;
+  const bwc_gl_inf &inf = c->info;
+  const uint64_t poff = inf.nX * inf.nY * inf.nZ * inf.nTS * (uint64_t)p;
+  const uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(denorm_flush_kernel, dim3((uint32_t)blocks), dim3(256), 0, ws->stream, work, par.box.len(0),
+                     par.box.len(1), par.box.len(2), n, alpha, beta, (uint64_t)c->control.Qm << 52, inf.nX, inf.nY,
+                     inf.nZ, tl.box.lo[0], tl.box.lo[1], tl.box.lo[2], tl.box.lo[3], poff, (int)inf.data_prec, out);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+int ensure_output(Workspace *ws, size_t bytes, void **out) {
+  CHK(ensure((void **)&ws->dec.out, &ws->dec.cap_out, bytes));
+  *out = ws->dec.out;
   return 0;
 }
 

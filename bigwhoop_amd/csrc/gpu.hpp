@@ -47,6 +47,18 @@ struct Workspace {
   uint64_t *d_gather = nullptr;
   uint8_t *d_bodies = nullptr;    // packet bodies of the codestream being written
   size_t cap_bodies = 0;
+  // decompression: codeblock tables and records of one (tile, param), the
+  // decoder's stripe-column state, the output staging buffer
+  struct {
+    uint32_t *box = nullptr, *meta = nullptr, *len = nullptr;
+    double *sb = nullptr;
+    uint8_t *K = nullptr, *Z = nullptr, *data = nullptr;
+    uint64_t *off = nullptr, *soff = nullptr;
+    uint16_t *state = nullptr;
+    void *out = nullptr;
+    size_t cap_box = 0, cap_meta = 0, cap_len = 0, cap_sb = 0, cap_K = 0, cap_Z = 0, cap_data = 0, cap_off = 0,
+           cap_soff = 0, cap_state = 0, cap_out = 0;
+  } dec;
   size_t cap_cb = 0, cap_q = 0, cap_data = 0, cap_packed = 0, cap_gather = 0, cap_state = 0;
   // fast tier-1 path: bitplanes (kept per parameter slot until the next
   // compression, so rate control can have codeblocks coded further),
@@ -127,6 +139,14 @@ int gather(Workspace *ws, CblkRecords &rec, const std::vector<uint32_t> &len);
 // body_d2h copies n body bytes at offset off to host memory.
 int gather_bodies(Workspace *ws, const std::vector<uint64_t> &desc, const std::vector<int> &slots, uint64_t end);
 int body_d2h(Workspace *ws, uint8_t *dst, uint64_t off, uint64_t n);
+// decompression of one (tile, param): tier-1 decoding and reconstruction
+// into work (zeroed first), and after the inverse DWT denormalisation into
+// out (device memory holding the whole field, [param][t][z][y][x]);
+// ensure_output gives a device staging buffer of `bytes`
+int t1_decode(Workspace *ws, const bwc_codec *c, const Param &par, const DecodeRecords &rec, double *work);
+int decode_output(Workspace *ws, const bwc_codec *c, const Tile &tl, int p, const Param &par, const double *work,
+                  void *out);
+int ensure_output(Workspace *ws, size_t bytes, void **out);
 int sync(Workspace *ws);
 const char *last_error();
 int ensure_work(Workspace *ws, uint64_t n);

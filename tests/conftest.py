@@ -10,3 +10,17 @@ for p in (str(ROOT), str(ROOT / "tests")):
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP) device")
     config.addinivalue_line("markers", "slow: long-running CPU test")
+
+
+def pytest_collection_modifyitems(session, config, items):
+    # Tests on the GPU use torch for device buffers next to the library. The
+    # torch wheel carries its own HIP runtime, which finds no device when it
+    # initialises after the library's: when any selected test needs the GPU,
+    # let torch initialise first.
+    if any(item.get_closest_marker("gpu") for item in items):
+        try:
+            import torch
+            if torch.cuda.is_available():
+                torch.zeros(1, device="cuda")
+        except Exception:
+            pass
