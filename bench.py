@@ -23,6 +23,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import subprocess
 import sys
@@ -188,6 +189,7 @@ def main():
         plain.compress_into(ptr)
     dt_plain, cs_plain, dwt_plain = run(plain, args.steps)
     plain.close()
+    print(f"bench: plain encoder {dt_plain / args.steps * 1e3:.1f} ms/step", file=sys.stderr, flush=True)
     # learned pass skip (the headline encoder); the codestream goes to a
     # page-locked host buffer, as a caller that wants the DMA path allocates it
     codec = bwc.Codec((n, n, n), pinned=True, **SETTINGS)
@@ -196,6 +198,7 @@ def main():
     for _ in range(args.warmup):
         codec.compress_into(ptr)
     dt, cs, dwt_ms = run(codec, args.steps)
+    print(f"bench: learned pass skip {dt / args.steps * 1e3:.1f} ms/step", file=sys.stderr, flush=True)
     ms_last, _ = codec.timings()
     st = codec.skip_stats()
     codec.close()
@@ -207,9 +210,24 @@ def main():
         pg.compress_into(ptr)
     dt_pg, cs_pg, _ = run(pg, args.steps)
     pg.close()
+    print(f"bench: pageable output {dt_pg / args.steps * 1e3:.1f} ms/step", file=sys.stderr, flush=True)
     if cs_pg != cs:
         print("pageable-output codestream differs", file=sys.stderr)
         sys.exit(1)
+    # decompression of the headline codestream into device memory
+    dec = torch.empty(n ** 3, dtype=torch.float64, device="cuda")
+    for _ in range(args.warmup):
+        bwc.decompress_bytes(cs, out=dec)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        bwc.decompress_bytes(cs, out=dec)
+    torch.cuda.synchronize()
+    dt_dec = time.perf_counter() - t0
+    print(f"bench: decompress {dt_dec / args.steps * 1e3:.1f} ms/step", file=sys.stderr, flush=True)
+    err = (dec - field.reshape(-1)).double()
+    psnr_db = 20 * math.log10(float(field.max() - field.min()) / (2 * math.sqrt(float((err * err).mean()))))
+    del dec, err
     if cs != cs_plain:
         print("learned pass skip and plain encoder codestreams differ", file=sys.stderr)
         sys.exit(1)
@@ -240,6 +258,9 @@ def main():
                    "codestream_identical_to_plain": True},
         "pageable_output": {"value": round(bytes_in * args.steps / dt_pg / 1e9, 4), "unit": "GB/s",
                             "ms_per_step": round(dt_pg / args.steps * 1e3, 3)},
+        "decompress": {"value": round(bytes_in * args.steps / dt_dec / 1e9, 4), "unit": "GB/s",
+                       "ms_per_step": round(dt_dec / args.steps * 1e3, 3), "psnr_db": round(psnr_db, 3),
+                       "output": "device memory"},
         "plain_encoder": {"value": round(plain_gbs, 4), "unit": "GB/s",
                           "ms_per_step": round(dt_plain / args.steps * 1e3, 3),
                           "speedup_learned_vs_plain": round(dt_plain / dt, 3)},
