@@ -61,7 +61,21 @@ void reset_for_encode(bwc_codec *c, Job *job) {
           }
       }
   }
-  job->rec.assign(job->plan.tiles.size() * c->info.nPar, CblkRecords());
+  // records keep their allocations from one call to the next (tier-1 fills
+  // them again); only what a call may leave unset is cleared
+  const size_t nrec = job->plan.tiles.size() * c->info.nPar;
+  if (job->rec.size() != nrec) {
+    job->rec.assign(nrec, CblkRecords());
+    return;
+  }
+  for (auto &R : job->rec) {
+    R.n = 0;
+    R.slot = 0;
+    R.data.clear();
+    R.src_off.clear();
+    R.zlim.clear();
+    R.feat.clear();
+  }
 }
 
 // tier-2 state of one tile back to "before rate control", so rate control

@@ -363,8 +363,10 @@ void CblkRecords::resize(uint64_t ncb, int nlayers) {
   Z.assign(ncb, 0);
   Zfull.assign(ncb, 0);
   U.assign(ncb, 0);
-  L.assign(ncb * kMaxPasses, 0);
-  S.assign(ncb * (kMaxPasses + 1), 0);
+  // tier-1 writes the rows of L and S that are read (L[0, Z), S[0, Z]);
+  // a record reused at the same size keeps its allocation unfilled
+  if (L.size() != ncb * kMaxPasses) L.assign(ncb * kMaxPasses, 0);
+  if (S.size() != ncb * (kMaxPasses + 1)) S.assign(ncb * (kMaxPasses + 1), 0);
   beta.assign(ncb, 3);
   beta_est.assign(ncb, 3);
   cp.assign(ncb * (nlayers + 1), 0);

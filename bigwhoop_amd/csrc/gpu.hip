@@ -46,7 +46,7 @@ Workspace::~Workspace() {
                   d_state, d_data,  d_K,    d_Kmsbs, d_Z,    d_L,    d_S,      d_snap,   d_mse,  d_flags,
                   d_packed, d_gather, d_bp, d_bpoff, d_sym, d_symoff, d_pcnt, d_pmse, d_list, d_U, d_zlim,
                   d_feat,  d_tstat, d_skipw, d_bodies, dec.box, dec.sb, dec.meta, dec.K, dec.Z, dec.len,
-                  dec.off, dec.data, dec.soff, dec.state, dec.out};
+                  dec.off, dec.data, dec.soff, dec.state, dec.out, d_poff, d_rpack};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (auto &b : bp_slots)
@@ -54,6 +54,12 @@ Workspace::~Workspace() {
   for (void *p : d_datap)
     if (p) (void)hipFree(p);
   if (h_pinned) (void)hipHostFree(h_pinned);
+  if (ev_up) {
+    (void)hipEventSynchronize(ev_up);
+    (void)hipEventDestroy(ev_up);
+  }
+  if (h_up) (void)hipHostFree(h_up);
+  if (h_down) (void)hipHostFree(h_down);
   for (auto &e : ev)
     if (e) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
@@ -66,6 +72,37 @@ static int ensure(void **p, size_t *cap, size_t bytes) {
   size_t nb = std::max<size_t>(bytes, 256);
   HIP_OK(hipMalloc(p, nb));
   *cap = nb;
+  return 0;
+}
+
+// Upload staging (page-locked): up_begin makes room for `bytes` of uploads
+// once the previous call's uploads have left the buffer; up_copy stages one
+// host array and queues its copy to dst on the workspace stream.
+// upload bytes per codeblock of one tier-1 call: box 32, steps 24, meta 4,
+// five offset tables 40, list 8, pass limit 1 (rounded up)
+constexpr size_t kUpBytesPerCb = 128;
+static int up_begin(Workspace *ws, size_t bytes) {
+  if (!ws->ev_up) HIP_OK(hipEventCreateWithFlags(&ws->ev_up, hipEventDisableTiming));
+  else HIP_OK(hipEventSynchronize(ws->ev_up));
+  ws->used_up = 0;
+  if (bytes <= ws->cap_up) return 0;
+  if (ws->h_up) HIP_OK(hipHostFree(ws->h_up));
+  ws->h_up = nullptr;
+  ws->cap_up = 0;
+  HIP_OK(hipHostMalloc((void **)&ws->h_up, bytes));
+  ws->cap_up = bytes;
+  return 0;
+}
+static int up_copy(Workspace *ws, void *dst, const void *src, size_t n) {
+  const size_t at = (ws->used_up + 255) & ~(size_t)255;
+  if (at + n > ws->cap_up) {
+    std::snprintf(g_err, sizeof g_err, "upload staging too small (%zu of %zu bytes)", at + n, ws->cap_up);
+    return 1;
+  }
+  std::memcpy(ws->h_up + at, src, n);
+  ws->used_up = at + n;
+  HIP_OK(hipMemcpyAsync(dst, ws->h_up + at, n, hipMemcpyHostToDevice, ws->stream));
+  HIP_OK(hipEventRecord(ws->ev_up, ws->stream));
   return 0;
 }
 
@@ -1540,9 +1577,8 @@ static int v2_code(Workspace *ws, const bwc_codec *c, const Param &par, const Cb
   cap = ws->cap_list;
   CHK(ensure((void **)&ws->d_list, &cap, (list.size() + 1) * sizeof(uint64_t)));
   ws->cap_list = cap;
-  HIP_OK(hipMemcpyAsync(ws->d_symoff, symoff.data(), (ncb + 1) * sizeof(uint64_t), hipMemcpyHostToDevice,
-                        ws->stream));
-  if (zlim) HIP_OK(hipMemcpyAsync(ws->d_zlim, zlim, ncb, hipMemcpyHostToDevice, ws->stream));
+  CHK(up_copy(ws, ws->d_symoff, symoff.data(), (ncb + 1) * sizeof(uint64_t)));
+  if (zlim) CHK(up_copy(ws, ws->d_zlim, zlim, ncb));
   const uint8_t *dz = zlim ? ws->d_zlim : nullptr;
   if (list.empty()) return 0;
   const int erres = (int)c->control.error_resilience;
@@ -1575,9 +1611,91 @@ static int v2_code(Workspace *ws, const bwc_codec *c, const Param &par, const Cb
     HIP_OK(hipGetLastError());
     return 0;
   };
-  HIP_OK(hipMemcpyAsync(ws->d_list, list.data(), list.size() * sizeof(uint64_t), hipMemcpyHostToDevice,
-                        ws->stream));
+  CHK(up_copy(ws, ws->d_list, list.data(), list.size() * sizeof(uint64_t)));
   return launch(ws->d_list, (uint32_t)list.size(), ws->stream, lpw);
+}
+
+// passes tier-1 codes for a codeblock with K bitplanes: all 3K - 2, or the
+// first zlim[cb] (at least one) when limits are given
+static uint32_t passes_coded(int K, const uint8_t *zlim, uint64_t cb) {
+  if (K == 0) return 0;
+  const uint32_t full = 3 * (uint32_t)K - 2;
+  return (zlim && zlim[cb] < full) ? (zlim[cb] > 0 ? zlim[cb] : 1u) : full;
+}
+
+// The records of the codeblocks in d_list, packed for one copy to the host:
+// per listed codeblock the first m entries of L and m + 1 of S (m = passes
+// coded, poff[r + 1] - poff[r]), then its Z, U and Kmsbs.
+__global__ void pack_records_kernel(const uint64_t *__restrict__ rows, uint32_t nrows,
+                                    const uint64_t *__restrict__ poff, const uint32_t *__restrict__ L,
+                                    const uint16_t *__restrict__ S, const uint8_t *__restrict__ Z,
+                                    const uint8_t *__restrict__ U, const uint8_t *__restrict__ Km,
+                                    uint32_t *__restrict__ pL, uint16_t *__restrict__ pS, uint8_t *__restrict__ pb) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  const uint64_t cb = rows[r], o = poff[r], m = poff[r + 1] - o;
+  const uint32_t *l = L + cb * kMaxPasses;
+  const uint16_t *sl = S + cb * (kMaxPasses + 1);
+  for (uint64_t j = 0; j < m; ++j) pL[o + j] = l[j];
+  for (uint64_t j = 0; j <= m; ++j) pS[o + r + j] = sl[j];
+  pb[3 * r] = Z[cb];
+  pb[3 * r + 1] = U[cb];
+  pb[3 * r + 2] = Km[cb];
+}
+
+// Bring the records of `rows` (the list v2_code launched, in d_list), coded
+// to m[r] passes, into rec, together with the flag words (hfl); ends with a
+// stream synchronisation. The offsets go up through the open upload staging.
+static int fetch_rows(Workspace *ws, CblkRecords &rec, const std::vector<uint64_t> &rows,
+                      const std::vector<uint32_t> &m, uint32_t hfl[4]) {
+  const uint64_t n = rows.size();
+  std::vector<uint64_t> poff(n + 1, 0);
+  for (uint64_t r = 0; r < n; ++r) poff[r + 1] = poff[r] + m[r];
+  const uint64_t tl = poff[n];
+  const size_t oS = (4 * tl + 255) & ~(size_t)255, oB = (oS + 2 * (tl + n) + 255) & ~(size_t)255;
+  const size_t oF = (oB + 3 * n + 255) & ~(size_t)255, total = oF + 16;
+  size_t cap = ws->cap_poff;
+  CHK(ensure((void **)&ws->d_poff, &cap, (n + 1) * sizeof(uint64_t)));
+  ws->cap_poff = cap;
+  cap = ws->cap_rpack;
+  CHK(ensure((void **)&ws->d_rpack, &cap, total));
+  ws->cap_rpack = cap;
+  if (total > ws->cap_down) {
+    if (ws->h_down) HIP_OK(hipHostFree(ws->h_down));
+    ws->h_down = nullptr;
+    ws->cap_down = 0;
+    HIP_OK(hipHostMalloc((void **)&ws->h_down, total));
+    ws->cap_down = total;
+  }
+  if (n) {
+    CHK(up_copy(ws, ws->d_poff, poff.data(), (n + 1) * sizeof(uint64_t)));
+    hipLaunchKernelGGL(pack_records_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, ws->stream, ws->d_list,
+                       (uint32_t)n, ws->d_poff, ws->d_L, ws->d_S, ws->d_Z, ws->d_U, ws->d_Kmsbs,
+                       (uint32_t *)ws->d_rpack, (uint16_t *)(ws->d_rpack + oS), ws->d_rpack + oB);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(ws->h_down, ws->d_rpack, oB + 3 * n, hipMemcpyDeviceToHost, ws->stream));
+  }
+  HIP_OK(hipMemcpyAsync(ws->h_down + oF, ws->d_flags, 16, hipMemcpyDeviceToHost, ws->stream));
+  HIP_OK(hipStreamSynchronize(ws->stream));
+  std::memcpy(hfl, ws->h_down + oF, 16);
+  if (hfl[0]) return 0;
+  const uint32_t *pL = (const uint32_t *)ws->h_down;
+  const uint16_t *pS = (const uint16_t *)(ws->h_down + oS);
+  const uint8_t *pb = ws->h_down + oB;
+  for (uint64_t r = 0; r < n; ++r) {
+    const uint64_t cb = rows[r];
+    if (pb[3 * r] != m[r]) {
+      std::snprintf(g_err, sizeof g_err, "codeblock %llu: %u passes coded, %u expected", (unsigned long long)cb,
+                    (unsigned)pb[3 * r], m[r]);
+      return 1;
+    }
+    rec.Z[cb] = pb[3 * r];
+    rec.U[cb] = pb[3 * r + 1];
+    rec.Kmsbs[cb] = pb[3 * r + 2];
+    std::memcpy(&rec.L[cb * kMaxPasses], pL + poff[r], m[r] * sizeof(uint32_t));
+    std::memcpy(&rec.S[cb * (kMaxPasses + 1)], pS + poff[r] + r, (m[r] + 1) * sizeof(uint16_t));
+  }
+  return 0;
 }
 
 int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double *work, CblkRecords &rec,
@@ -1632,12 +1750,12 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
   rec.slot = slot;
   rec.src_off = doff;
   if (!ws->d_flags) HIP_OK(hipMalloc((void **)&ws->d_flags, 16));
-  HIP_OK(hipMemcpyAsync(ws->d_cbbox, par.cb_box.data(), 8 * ncb * sizeof(uint32_t), hipMemcpyHostToDevice,
-                        ws->stream));
-  HIP_OK(hipMemcpyAsync(ws->d_cbsb, sb.data(), 3 * ncb * sizeof(double), hipMemcpyHostToDevice, ws->stream));
-  HIP_OK(hipMemcpyAsync(ws->d_cbmeta, meta.data(), ncb * sizeof(uint32_t), hipMemcpyHostToDevice, ws->stream));
-  HIP_OK(hipMemcpyAsync(ws->d_qoff, qoff.data(), (ncb + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, ws->stream));
-  HIP_OK(hipMemcpyAsync(ws->d_doff, doff.data(), (ncb + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, ws->stream));
+  CHK(up_begin(ws, kUpBytesPerCb * (ncb + 1) + kSkipModelSize * sizeof(float) + 16 * 256));
+  CHK(up_copy(ws, ws->d_cbbox, par.cb_box.data(), 8 * ncb * sizeof(uint32_t)));
+  CHK(up_copy(ws, ws->d_cbsb, sb.data(), 3 * ncb * sizeof(double)));
+  CHK(up_copy(ws, ws->d_cbmeta, meta.data(), ncb * sizeof(uint32_t)));
+  CHK(up_copy(ws, ws->d_qoff, qoff.data(), (ncb + 1) * sizeof(uint64_t)));
+  CHK(up_copy(ws, ws->d_doff, doff.data(), (ncb + 1) * sizeof(uint64_t)));
   hipLaunchKernelGGL(init_flags_kernel, dim3(1), dim3(64), 0, ws->stream, ws->d_flags);
   const uint64_t W = par.box.len(0), H = par.box.len(1), D = par.box.len(2);
 
@@ -1649,6 +1767,7 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
   const int erres = (int)c->control.error_resilience;
   rec.U.assign(ncb, 0);
   rec.Zfull.assign(ncb, 0);
+  uint32_t hfl[4] = {0, 0, 0, 0};
   if (v2) {
     if (slot >= (int)ws->bp_slots.size()) ws->bp_slots.resize(slot + 1);
     Workspace::BpSlot &bs = ws->bp_slots[slot];
@@ -1658,8 +1777,7 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
     cap = ws->cap_bpoff;
     CHK(ensure((void **)&ws->d_bpoff, &cap, (ncb + 1) * sizeof(uint64_t)));
     ws->cap_bpoff = cap;
-    HIP_OK(hipMemcpyAsync(ws->d_bpoff, tb.bpoff.data(), (ncb + 1) * sizeof(uint64_t), hipMemcpyHostToDevice,
-                          ws->stream));
+    CHK(up_copy(ws, ws->d_bpoff, tb.bpoff.data(), (ncb + 1) * sizeof(uint64_t)));
     float *d_feat = nullptr;
     if (lim.model || lim.features) {
       cap = ws->cap_feat;
@@ -1667,9 +1785,7 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
       ws->cap_feat = cap;
       if (!ws->d_tstat) HIP_OK(hipMalloc((void **)&ws->d_tstat, 4 * sizeof(float)));
       if (!ws->d_skipw) HIP_OK(hipMalloc((void **)&ws->d_skipw, kSkipModelSize * sizeof(float)));
-      if (lim.model)
-        HIP_OK(hipMemcpyAsync(ws->d_skipw, lim.model, kSkipModelSize * sizeof(float), hipMemcpyHostToDevice,
-                              ws->stream));
+      if (lim.model) CHK(up_copy(ws, ws->d_skipw, lim.model, kSkipModelSize * sizeof(float)));
       hipLaunchKernelGGL(zero_f32_kernel, dim3(1), dim3(64), 0, ws->stream, ws->d_tstat, 4);
       d_feat = ws->d_feat;
     }
@@ -1694,9 +1810,14 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
     HIP_OK(hipStreamSynchronize(ws->stream));
     if (lim.out && zlim) std::memcpy(lim.out, zlim, ncb);
     std::vector<uint64_t> list;
+    std::vector<uint32_t> m;
     for (uint64_t i = 0; i < ncb; ++i)
-      if (rec.K[i]) list.push_back(i);
+      if (rec.K[i]) {
+        list.push_back(i);
+        m.push_back(passes_coded(rec.K[i], zlim, i));
+      }
     CHK(v2_code(ws, c, par, tb, rec.K.data(), list, zlim, d_data, bs.bp, gp));
+    CHK(fetch_rows(ws, rec, list, m, hfl));
   } else {
     cap = ws->cap_q;
     CHK(ensure((void **)&ws->d_q, &cap, qo * sizeof(uint64_t)));
@@ -1712,18 +1833,16 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
                        d_data, ws->d_K, ws->d_Kmsbs, ws->d_Z, ws->d_L, ws->d_S, ws->d_snap, ws->d_mse,
                        ws->d_flags, erres);
     HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(rec.K.data(), ws->d_K, ncb, hipMemcpyDeviceToHost, ws->stream));
+    HIP_OK(hipMemcpyAsync(rec.Kmsbs.data(), ws->d_Kmsbs, ncb, hipMemcpyDeviceToHost, ws->stream));
+    HIP_OK(hipMemcpyAsync(rec.Z.data(), ws->d_Z, ncb, hipMemcpyDeviceToHost, ws->stream));
+    HIP_OK(hipMemcpyAsync(rec.L.data(), ws->d_L, ncb * kMaxPasses * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                          ws->stream));
+    HIP_OK(hipMemcpyAsync(rec.S.data(), ws->d_S, ncb * (kMaxPasses + 1) * sizeof(uint16_t), hipMemcpyDeviceToHost,
+                          ws->stream));
+    HIP_OK(hipMemcpyAsync(hfl, ws->d_flags, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, ws->stream));
+    HIP_OK(hipStreamSynchronize(ws->stream));
   }
-  HIP_OK(hipMemcpyAsync(rec.K.data(), ws->d_K, ncb, hipMemcpyDeviceToHost, ws->stream));
-  HIP_OK(hipMemcpyAsync(rec.Kmsbs.data(), ws->d_Kmsbs, ncb, hipMemcpyDeviceToHost, ws->stream));
-  HIP_OK(hipMemcpyAsync(rec.Z.data(), ws->d_Z, ncb, hipMemcpyDeviceToHost, ws->stream));
-  HIP_OK(hipMemcpyAsync(rec.L.data(), ws->d_L, ncb * kMaxPasses * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                        ws->stream));
-  HIP_OK(hipMemcpyAsync(rec.S.data(), ws->d_S, ncb * (kMaxPasses + 1) * sizeof(uint16_t), hipMemcpyDeviceToHost,
-                        ws->stream));
-  if (v2) HIP_OK(hipMemcpyAsync(rec.U.data(), ws->d_U, ncb, hipMemcpyDeviceToHost, ws->stream));
-  uint32_t hfl[4];
-  HIP_OK(hipMemcpyAsync(hfl, ws->d_flags, sizeof hfl, hipMemcpyDeviceToHost, ws->stream));
-  HIP_OK(hipStreamSynchronize(ws->stream));
   if (hfl[0]) {
     std::snprintf(g_err, sizeof g_err, "tier-1 output exceeded a per-codeblock buffer (flags %u)", hfl[0]);
     return 1;
@@ -1759,54 +1878,25 @@ int t1_extend(Workspace *ws, const bwc_codec *c, const Param &par, CblkRecords &
   CbTables tb;
   build_tables(par, tb);
   const GenParams gp = gen_params(par);
-  HIP_OK(hipMemcpyAsync(ws->d_cbbox, par.cb_box.data(), 8 * ncb * sizeof(uint32_t), hipMemcpyHostToDevice,
-                        ws->stream));
-  HIP_OK(hipMemcpyAsync(ws->d_cbsb, tb.sb.data(), 3 * ncb * sizeof(double), hipMemcpyHostToDevice, ws->stream));
-  HIP_OK(hipMemcpyAsync(ws->d_cbmeta, tb.meta.data(), ncb * sizeof(uint32_t), hipMemcpyHostToDevice, ws->stream));
-  HIP_OK(hipMemcpyAsync(ws->d_doff, tb.doff.data(), (ncb + 1) * sizeof(uint64_t), hipMemcpyHostToDevice,
-                        ws->stream));
-  HIP_OK(hipMemcpyAsync(ws->d_bpoff, tb.bpoff.data(), (ncb + 1) * sizeof(uint64_t), hipMemcpyHostToDevice,
-                        ws->stream));
-  HIP_OK(hipMemcpyAsync(ws->d_K, rec.K.data(), ncb, hipMemcpyHostToDevice, ws->stream));
+  CHK(up_begin(ws, kUpBytesPerCb * (ncb + 1) + 16 * 256));
+  CHK(up_copy(ws, ws->d_cbbox, par.cb_box.data(), 8 * ncb * sizeof(uint32_t)));
+  CHK(up_copy(ws, ws->d_cbsb, tb.sb.data(), 3 * ncb * sizeof(double)));
+  CHK(up_copy(ws, ws->d_cbmeta, tb.meta.data(), ncb * sizeof(uint32_t)));
+  CHK(up_copy(ws, ws->d_doff, tb.doff.data(), (ncb + 1) * sizeof(uint64_t)));
+  CHK(up_copy(ws, ws->d_bpoff, tb.bpoff.data(), (ncb + 1) * sizeof(uint64_t)));
+  CHK(up_copy(ws, ws->d_K, rec.K.data(), ncb));
   hipLaunchKernelGGL(init_flags_kernel, dim3(1), dim3(64), 0, ws->stream, ws->d_flags);
   CHK(v2_code(ws, c, par, tb, rec.K.data(), cbs, zlim, ws->d_datap[slot], ws->bp_slots[slot].bp, gp));
-  // the re-coded rows: Z, U, L, S. The device arrays hold other parameters'
-  // rows as well, so only the listed rows reach the records: row copies for
-  // a few codeblocks, else whole arrays through host staging.
-  const bool bulk = cbs.size() > 64;
-  std::vector<uint8_t> hz, hu;
-  std::vector<uint32_t> hl;
-  std::vector<uint16_t> hs;
-  if (bulk) {
-    hz.resize(ncb), hu.resize(ncb), hl.resize(ncb * kMaxPasses), hs.resize(ncb * (kMaxPasses + 1));
-    HIP_OK(hipMemcpyAsync(hz.data(), ws->d_Z, ncb, hipMemcpyDeviceToHost, ws->stream));
-    HIP_OK(hipMemcpyAsync(hu.data(), ws->d_U, ncb, hipMemcpyDeviceToHost, ws->stream));
-    HIP_OK(hipMemcpyAsync(hl.data(), ws->d_L, hl.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, ws->stream));
-    HIP_OK(hipMemcpyAsync(hs.data(), ws->d_S, hs.size() * sizeof(uint16_t), hipMemcpyDeviceToHost, ws->stream));
-  } else {
-    for (uint64_t i : cbs) {
-      HIP_OK(hipMemcpyAsync(&rec.Z[i], ws->d_Z + i, 1, hipMemcpyDeviceToHost, ws->stream));
-      HIP_OK(hipMemcpyAsync(&rec.U[i], ws->d_U + i, 1, hipMemcpyDeviceToHost, ws->stream));
-      HIP_OK(hipMemcpyAsync(&rec.L[i * kMaxPasses], ws->d_L + i * kMaxPasses, kMaxPasses * sizeof(uint32_t),
-                            hipMemcpyDeviceToHost, ws->stream));
-      HIP_OK(hipMemcpyAsync(&rec.S[i * (kMaxPasses + 1)], ws->d_S + i * (kMaxPasses + 1),
-                            (kMaxPasses + 1) * sizeof(uint16_t), hipMemcpyDeviceToHost, ws->stream));
-    }
-  }
+  // the re-coded rows: Z, U, Kmsbs, L, S (the device arrays hold other
+  // parameters' rows as well; only the listed rows reach the records)
+  std::vector<uint32_t> m(cbs.size());
+  for (size_t r = 0; r < cbs.size(); ++r) m[r] = passes_coded(rec.K[cbs[r]], zlim, cbs[r]);
   uint32_t hfl[4];
-  HIP_OK(hipMemcpyAsync(hfl, ws->d_flags, sizeof hfl, hipMemcpyDeviceToHost, ws->stream));
-  HIP_OK(hipStreamSynchronize(ws->stream));
+  CHK(fetch_rows(ws, rec, cbs, m, hfl));
   if (hfl[0]) {
     std::snprintf(g_err, sizeof g_err, "tier-1 output exceeded a per-codeblock buffer (flags %u)", hfl[0]);
     return 1;
   }
-  if (bulk)
-    for (uint64_t i : cbs) {
-      rec.Z[i] = hz[i];
-      rec.U[i] = hu[i];
-      std::memcpy(&rec.L[i * kMaxPasses], &hl[i * kMaxPasses], kMaxPasses * sizeof(uint32_t));
-      std::memcpy(&rec.S[i * (kMaxPasses + 1)], &hs[i * (kMaxPasses + 1)], (kMaxPasses + 1) * sizeof(uint16_t));
-    }
   return 0;
 }
 

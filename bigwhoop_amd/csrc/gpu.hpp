@@ -83,6 +83,16 @@ struct Workspace {
   uint64_t *d_list = nullptr;
   size_t cap_bp = 0, cap_bpoff = 0, cap_sym = 0, cap_symoff = 0, cap_pc = 0, cap_pm = 0, cap_list = 0;
   uint64_t *h_pinned = nullptr;   // small pinned staging for flags / min-max
+  // page-locked staging of one tier-1 call: the uploads (codeblock tables,
+  // offsets, lists) are then asynchronous DMA that the host does not wait
+  // for, and the packed records come back the same way; ev_up marks the end
+  // of the last upload from h_up
+  uint8_t *h_up = nullptr, *h_down = nullptr;
+  size_t cap_up = 0, used_up = 0, cap_down = 0;
+  hipEvent_t ev_up = nullptr;
+  uint64_t *d_poff = nullptr;     // packed-record offsets per listed codeblock
+  uint8_t *d_rpack = nullptr;     // packed records
+  size_t cap_poff = 0, cap_rpack = 0;
   int last_dwt_launches = 0;      // kernel launches of the last DWT call
   ~Workspace();
 };
