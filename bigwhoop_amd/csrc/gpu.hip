@@ -1529,10 +1529,10 @@ static uint64_t stream_offsets(const Param &par, const CbTables &t, const uint8_
 }
 
 // LDS of the symbol generator: per codeblock it packs P = 64 / pow2ceil(width)
-// slices side by side, each with its own pass-output buffer, so size for the
-// largest P times the largest per-slice bound
+// slices side by side, each with its own pass-output buffer; the buffers take
+// the largest, over the codeblocks, of P times the codeblock's per-slice bound
 struct GenParams {
-  uint32_t maxcol = 0, obuf_cap = 0, maxcw = 0, P = 1;
+  uint32_t maxcol = 0, obuf_bytes = 0, maxcw = 0;
   size_t lds = 0;
 };
 static GenParams gen_params(const Param &par) {
@@ -1543,13 +1543,13 @@ static GenParams gen_params(const Param &par) {
     const uint32_t ncol = cw * ns * (b[6] - b[2]) * (b[7] - b[3]);
     g.maxcol = std::max(g.maxcol, ncol);
     g.maxcw = std::max(g.maxcw, cw);
-    g.obuf_cap = std::max(g.obuf_cap, cw * ch * 2 + cw * ns * 3 + 16);
     uint32_t gw = 1;
     while (gw < cw) gw <<= 1;
     const uint32_t nsl = (b[6] - b[2]) * (b[7] - b[3]);
-    if (gw <= 64) g.P = std::max(g.P, std::min(64 / gw, nsl));
+    const uint32_t P = gw <= 64 ? std::max(1u, std::min(64 / gw, nsl)) : 1u;
+    g.obuf_bytes = std::max(g.obuf_bytes, P * (cw * ch * 2 + cw * ns * 3 + 16));
   }
-  g.lds = (size_t)g.maxcol * 2 + (size_t)((g.maxcol + 1) / 2) * 8 + (size_t)g.P * g.obuf_cap;
+  g.lds = (size_t)g.maxcol * 2 + (size_t)((g.maxcol + 1) / 2) * 8 + (size_t)g.obuf_bytes;
   return g;
 }
 
@@ -1590,11 +1590,7 @@ static int v2_code(Workspace *ws, const bwc_codec *c, const Param &par, const Cb
   // 1, 2, 4, 8, 16 or 32 overrides.
   const char *lpw_env = std::getenv("BWC_AMD_MQ_LPW");
   const int lpw = lpw_env ? std::atoi(lpw_env) : (zlim ? 4 : 16);
-  auto launch = [&](const uint64_t *dl, uint32_t nl, hipStream_t s, int lanes) -> int {
-    hipLaunchKernelGGL(t1_gen_kernel, dim3(nl), dim3(64), gp.lds, s, dl, ws->d_cbbox, ws->d_cbmeta, ws->d_K,
-                       ws->d_bpoff, d_bp, ws->d_symoff, ws->d_sym, ws->d_pcnt, ws->d_pmse, ws->d_flags, erres,
-                       gp.maxcol, gp.obuf_cap, dz);
-    HIP_OK(hipGetLastError());
+  auto launch_mq = [&](const uint64_t *dl, uint32_t nl, hipStream_t s, int lanes) -> int {
 #define MQ_LAUNCH(N)                                                                                           \
   hipLaunchKernelGGL(t1_mq_kernel<N>, dim3((nl + N - 1) / N), dim3(64), 0, s, dl, (uint64_t)nl, ws->d_cbmeta, \
                      ws->d_cbsb, ws->d_K, ws->d_symoff, ws->d_sym, ws->d_pcnt, ws->d_pmse, ws->d_doff, d_data,    \
@@ -1610,6 +1606,13 @@ static int v2_code(Workspace *ws, const bwc_codec *c, const Param &par, const Cb
 #undef MQ_LAUNCH
     HIP_OK(hipGetLastError());
     return 0;
+  };
+  auto launch = [&](const uint64_t *dl, uint32_t nl, hipStream_t s, int lanes) -> int {
+    hipLaunchKernelGGL(t1_gen_kernel, dim3(nl), dim3(64), gp.lds, s, dl, ws->d_cbbox, ws->d_cbmeta, ws->d_K,
+                       ws->d_bpoff, d_bp, ws->d_symoff, ws->d_sym, ws->d_pcnt, ws->d_pmse, ws->d_flags, erres,
+                       gp.maxcol, gp.obuf_bytes, dz);
+    HIP_OK(hipGetLastError());
+    return launch_mq(dl, nl, s, lanes);
   };
   CHK(up_copy(ws, ws->d_list, list.data(), list.size() * sizeof(uint64_t)));
   return launch(ws->d_list, (uint32_t)list.size(), ws->stream, lpw);
