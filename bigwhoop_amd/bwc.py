@@ -86,6 +86,9 @@ def lib():
             "bwc_amd_host_free": (None, [vp]),
             "bwc_amd_last_error": (C.c_char_p, []),
             "bwc_amd_reset_gain_lut": (None, []),
+            "bwc_amd_set_tuning": (C.c_int, [C.c_char_p, C.c_int]),
+            "bwc_amd_get_tuning": (C.c_int, [C.c_char_p, C.POINTER(C.c_int)]),
+            "bwc_amd_tuning_source": (C.c_char_p, []),
             "bwc_amd_ncblk": (u64, [vp, u64, u8]),
             "bwc_amd_param_dims": (C.c_int, [vp, u64, C.POINTER(u64)]),
             "bwc_amd_stage_dwt": (C.c_int, [vp, u64, u8, vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
@@ -190,6 +193,30 @@ def decompress(codec, stream):
 def _shape4(shape):
     s = [int(x) for x in shape] + [1] * (4 - len(shape))
     return s[:4]
+
+
+TUNING_KNOBS = ("dwt_unroll", "dwt_seg_log2", "dwt_seg_min", "mq_lanes_full", "mq_lanes_limited")
+
+
+def set_tuning(name: str, value: int):
+    """Set one kernel launch-shape knob (include/bwc_amd.h); no result changes."""
+    if lib().bwc_amd_set_tuning(name.encode(), int(value)):
+        raise ValueError(f"tuning knob {name}={value} rejected: " + last_error())
+
+
+def get_tuning(name: str = None):
+    """One knob's value, or all of them as a dict."""
+    if name is None:
+        return {k: get_tuning(k) for k in TUNING_KNOBS}
+    v = C.c_int()
+    if lib().bwc_amd_get_tuning(name.encode(), C.byref(v)):
+        raise ValueError(f"unknown tuning knob {name}")
+    return v.value
+
+
+def tuning_source() -> str:
+    """The tuning file the library read ("" when none)."""
+    return (lib().bwc_amd_tuning_source() or b"").decode()
 
 
 class Codec:

@@ -12,6 +12,7 @@
 
 #include "codec.hpp"
 #include "gpu.hpp"
+#include "tuning.hpp"
 
 using namespace bwc;
 
@@ -731,6 +732,22 @@ int bwc_amd_device_count(void) { return gpu::device_count(); }
 void *bwc_amd_host_alloc(size_t bytes) { return gpu::host_alloc(bytes); }
 void bwc_amd_host_free(void *p) { gpu::host_free(p); }
 const char *bwc_amd_last_error(void) { return g_api_err[0] ? g_api_err : gpu::last_error(); }
+
+int bwc_amd_set_tuning(const char *name, int value) {
+  if (set_tuning(name, value)) {
+    set_err("unknown tuning knob or value out of range");
+    return 1;
+  }
+  return 0;
+}
+int bwc_amd_get_tuning(const char *name, int *value) {
+  if (get_tuning(name, value)) {
+    set_err("unknown tuning knob");
+    return 1;
+  }
+  return 0;
+}
+const char *bwc_amd_tuning_source(void) { return tuning_source(); }
 
 uint64_t bwc_amd_ncblk(bwc_codec *codec, uint64_t tile, uint8_t param) {
   Job *job = job_of(codec);

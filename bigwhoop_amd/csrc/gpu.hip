@@ -17,6 +17,7 @@
 #include <cstring>
 
 #include "gpu.hpp"
+#include "tuning.hpp"
 
 namespace bwc {
 namespace gpu {
@@ -568,11 +569,8 @@ static int dwt_run(Workspace *ws, const bwc_codec *c, const Param &par, double *
   // launch shape of the streaming passes: pair iterations unrolled per load
   // batch, thread target 2^seg_log2 and minimum iterations per segment
   struct {
-    int unroll = 4, seg_log2 = 17, seg_min = 8;
-  } tune;
-  if (const char *e = std::getenv("BWC_AMD_DWT_UNROLL")) tune.unroll = std::atoi(e);
-  if (const char *e = std::getenv("BWC_AMD_DWT_SEG_LOG2")) tune.seg_log2 = std::max(8, std::min(24, std::atoi(e)));
-  if (const char *e = std::getenv("BWC_AMD_DWT_SEG_MIN")) tune.seg_min = std::max(2, std::atoi(e));
+    int unroll, seg_log2, seg_min;
+  } tune = {tuning().dwt_unroll, tuning().dwt_seg_log2, tuning().dwt_seg_min};
   for (int step = 0; step < nd; ++step) {
     const int level = fwd ? step : nd - 1 - step;
     const Box &rb = par.res[nd - level].box;
@@ -1582,14 +1580,9 @@ static int v2_code(Workspace *ws, const bwc_codec *c, const Param &par, const Cb
   const uint8_t *dz = zlim ? ws->d_zlim : nullptr;
   if (list.empty()) return 0;
   const int erres = (int)c->control.error_resilience;
-  // codeblocks per wavefront of the MQ kernel (lanes beyond LPW idle).
-  // Complete codings (~500k decisions per codeblock) are issue-bound: 16
-  // lanes share each instruction. Pass-limited codings are bound by the
-  // longest codeblocks' decision latency, which more wavefronts per SIMD
-  // hide better: 4 lanes (measured on the 512^3 field). BWC_AMD_MQ_LPW =
-  // 1, 2, 4, 8, 16 or 32 overrides.
-  const char *lpw_env = std::getenv("BWC_AMD_MQ_LPW");
-  const int lpw = lpw_env ? std::atoi(lpw_env) : (zlim ? 4 : 16);
+  // codeblocks per wavefront of the MQ kernel (lanes beyond them idle), for
+  // complete codings and for codings to a pass limit (tuning.hpp)
+  const int lpw = zlim ? tuning().mq_lanes_limited : tuning().mq_lanes_full;
   auto launch_mq = [&](const uint64_t *dl, uint32_t nl, hipStream_t s, int lanes) -> int {
 #define MQ_LAUNCH(N)                                                                                           \
   hipLaunchKernelGGL(t1_mq_kernel<N>, dim3((nl + N - 1) / N), dim3(64), 0, s, dl, (uint64_t)nl, ws->d_cbmeta, \

@@ -1,0 +1,28 @@
+// Launch-shape knobs of the kernels: defaults, then the tuning file written
+// by the autotuner (python -m bigwhoop_amd.autotune), then environment
+// variables, then bwc_amd_set_tuning. None of them changes a result, only
+// how the work is laid out on the GPU.
+#pragma once
+
+namespace bwc {
+
+struct Tuning {
+  // streaming DWT passes (strided axes): pair iterations unrolled per load
+  // batch (2, 4 or 8), thread target 2^dwt_seg_log2 per pass, minimum pair
+  // iterations per line segment
+  int dwt_unroll = 4, dwt_seg_log2 = 17, dwt_seg_min = 8;
+  // codeblocks per MQ wavefront (1, 2, 4, 8, 16 or 32): complete codings and
+  // codings to a pass limit
+  int mq_lanes_full = 16, mq_lanes_limited = 4;
+};
+
+// the current values (loaded on first use)
+const Tuning &tuning();
+// set one knob by name; 0 on success, 1 for an unknown name or a value out of
+// its range. Not synchronised with compressions running in other threads.
+int set_tuning(const char *name, int value);
+int get_tuning(const char *name, int *value);
+// the file the values were read from ("" when none)
+const char *tuning_source();
+
+}  // namespace bwc

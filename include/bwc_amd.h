@@ -39,6 +39,21 @@ void bwc_amd_host_free(void *p);
    compression per process would see. */
 void bwc_amd_reset_gain_lut(void);
 
+/* Launch-shape knobs of the kernels (no counterpart in the reference); none
+   changes a result. Names: "dwt_unroll" (2, 4, 8: pair iterations per load
+   batch of the strided DWT passes), "dwt_seg_log2" (8-24: thread target
+   2^n of those passes), "dwt_seg_min" (>= 2: minimum pair iterations per
+   line segment), "mq_lanes_full" and "mq_lanes_limited" (1, 2, 4, 8, 16, 32:
+   codeblocks per MQ wavefront for complete codings and for codings to a pass
+   limit). Values come from the defaults, then the tuning file the autotuner
+   writes (python -m bigwhoop_amd.autotune; BWC_AMD_TUNING names another
+   file), then environment variables (BWC_AMD_DWT_UNROLL, ...), then these
+   calls. bwc_amd_tuning_source gives the file read ("" when none). Setting a
+   knob while a compression runs in another thread is not supported. */
+int bwc_amd_set_tuning(const char *name, int value);
+int bwc_amd_get_tuning(const char *name, int *value);
+const char *bwc_amd_tuning_source(void);
+
 /* geometry of a codec after bwc_create_compression */
 uint64_t bwc_amd_ncblk(bwc_codec *codec, uint64_t tile, uint8_t param);
 /* per codeblock in access order (libbwc.c:1651-1683): box[8] tile-local
