@@ -119,6 +119,11 @@ struct CblkRecords {
   std::vector<uint8_t> Zfull, U;
   std::vector<uint32_t> L;      // kMaxPasses per codeblock, bytes
   std::vector<uint16_t> S;      // kMaxPasses + 1 per codeblock
+  // fast tier-1 path: distortion reduction of each coded pass (kMaxPasses
+  // per codeblock, the values the hull was built from) and, for a codeblock
+  // coded to a prefix of its passes, an upper bound on what the passes left
+  // out can add
+  std::vector<double> mse, bfut;
   // tier-2 state
   std::vector<uint16_t> beta, beta_est;
   std::vector<int16_t> cp;      // (nLayers + 1) per codeblock, cp[0] == 0 is layer -1

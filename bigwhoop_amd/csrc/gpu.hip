@@ -47,11 +47,13 @@ Workspace::~Workspace() {
                   d_state, d_data,  d_K,    d_Kmsbs, d_Z,    d_L,    d_S,      d_snap,   d_mse,  d_flags,
                   d_packed, d_gather, d_bp, d_bpoff, d_sym, d_symoff, d_pcnt, d_pmse, d_list, d_U, d_zlim,
                   d_feat,  d_tstat, d_skipw, d_bodies, dec.box, dec.sb, dec.meta, dec.K, dec.Z, dec.len,
-                  dec.off, dec.data, dec.soff, dec.state, dec.out, d_poff, d_rpack};
+                  dec.off, dec.data, dec.soff, dec.state, dec.out, d_poff, d_rpack, d_bfut};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
-  for (auto &b : bp_slots)
+  for (auto &b : bp_slots) {
     if (b.bp) (void)hipFree(b.bp);
+    if (b.hist) (void)hipFree(b.hist);
+  }
   for (void *p : d_datap)
     if (p) (void)hipFree(p);
   if (h_pinned) (void)hipHostFree(h_pinned);
@@ -577,15 +579,17 @@ static int dwt_run(Workspace *ws, const bwc_codec *c, const Param &par, double *
     uint64_t sz[4];
     for (int a = 0; a < 4; ++a) sz[a] = rb.len(a);
     auto active = [&](int a) { return level < dec[a] && sz[a] > 1; };
-    auto fast97 = [&](int a) {
-      if (!fwd || generic_only || kern[a] != bwc_dwt_9_7 || !active(a)) return false;
-      if (a == 0) return (size_t)4 * 2 * ((sz[0] + 16) / 2 + 1) * sizeof(double) <= 160 * 1024;
-      return true;
+    auto fast97 = [&](int a) { return fwd && !generic_only && kern[a] == bwc_dwt_9_7 && active(a); };
+    // X lines that fit the LDS kernel take it; longer ones stream like the
+    // strided axes (one thread per line segment)
+    auto x_lds = [&](int a) {
+      return a == 0 && fast97(0) && (size_t)4 * 2 * ((sz[0] + 16) / 2 + 1) * sizeof(double) <= 160 * 1024;
     };
-    // every strided fast pass moves the box to the other buffer; an odd count
+    auto streamed = [&](int a) { return fast97(a) && !x_lds(a); };
+    // every streamed pass moves the box to the other buffer; an odd count
     // is evened out by running the X pass out of place, else by a copy back
-    const int nstrided = (int)fast97(1) + (int)fast97(2) + (int)fast97(3);
-    const bool x_oop = (nstrided & 1) && fast97(0);
+    const int nstrided = (int)streamed(0) + (int)streamed(1) + (int)streamed(2) + (int)streamed(3);
+    const bool x_oop = (nstrided & 1) && x_lds(0);
     double *cur = work, *oth = ws->d_work2;
     if (nstrided) {
       size_t cap = ws->work2_cap;
@@ -593,7 +597,7 @@ static int dwt_run(Workspace *ws, const bwc_codec *c, const Param &par, double *
       ws->work2_cap = cap;
       oth = ws->d_work2;
     }
-    if (norm_pending && !fast97(0)) {
+    if (norm_pending && !x_lds(0)) {
       CHK(normalize(ws, work, par.box.volume(), (uint8_t)norm_qm));
       norm_pending = false;
     }
@@ -628,7 +632,7 @@ static int dwt_run(Workspace *ws, const bwc_codec *c, const Param &par, double *
       // other one along the strided axes
       const int half = par0 ? (int)(len / 2) : (int)((len + 1) / 2);
       const size_t xlds = (size_t)4 * 2 * ((len + 16) / 2 + 1) * sizeof(double);
-      if (fast97(a) && a == 0) {
+      if (x_lds(a)) {
         double *dst = x_oop ? oth : cur;
         hipLaunchKernelGGL(dwt97_x_kernel, dim3((uint32_t)((sz[o[0]] + 3) / 4), (uint32_t)gy), dim3(256), xlds,
                            ws->stream, cur, dst, (int)len, par0, half, (uint32_t)sz[o[0]], inc[o[0]],
@@ -640,7 +644,7 @@ static int dwt_run(Workspace *ws, const bwc_codec *c, const Param &par, double *
         ++ws->last_dwt_launches;
         continue;
       }
-      if (fast97(a) && a != 0) {
+      if (streamed(a)) {
         const uint64_t nlines = sz[o[0]] * sz[o[1]] * sz[o[2]];
         // split lines into segments until about 128K threads run (short
         // lines of the coarse levels), keeping >= 8 pair iterations each
@@ -1267,12 +1271,14 @@ __global__ void bitplane_kernel(const double *__restrict__ work, uint64_t W, uin
                                 const uint32_t *__restrict__ cbbox, const double *__restrict__ cbsb,
                                 const uint64_t *__restrict__ bpoff, uint8_t *__restrict__ bp,
                                 uint8_t *__restrict__ Kout, uint64_t ncb, const uint32_t *__restrict__ cbmeta,
-                                float *__restrict__ feat, float *__restrict__ tstat, float rate) {
+                                float *__restrict__ feat, float *__restrict__ tstat, float rate,
+                                uint32_t *__restrict__ hist_out) {
   __shared__ int smax[256];
   __shared__ uint32_t hist[65];
   __shared__ double s1[256], s2[256];
+  const bool want_hist = feat || hist_out;
   for (uint64_t cb = blockIdx.x; cb < ncb; cb += gridDim.x) {
-    if (feat)
+    if (want_hist)
       for (int i = threadIdx.x; i < 65; i += blockDim.x) hist[i] = 0;
     __syncthreads();
     double sq = 0, sqq = 0;
@@ -1301,8 +1307,8 @@ __global__ void bitplane_kernel(const double *__restrict__ work, uint64_t W, uin
               qq = x86_to_u64(__longlong_as_double((long long)raw) / step);
               const int nb = qq ? 64 - __clzll((long long)qq) : 0;
               kb = nb > kb ? nb : kb;
+              if (want_hist) atomicAdd(&hist[nb], 1u);
               if (feat) {
-                atomicAdd(&hist[nb], 1u);
                 const double dq = (double)qq;
                 sq += dq;
                 sqq += dq * dq;
@@ -1364,6 +1370,9 @@ __global__ void bitplane_kernel(const double *__restrict__ work, uint64_t W, uin
         atomicAdd(&tstat[1], 1.0f);
       }
     }
+    // bit-length histogram (count of values per number of magnitude bits)
+    if (hist_out)
+      for (int i = threadIdx.x; i < 65; i += blockDim.x) hist_out[65 * cb + i] = hist[i];
     __syncthreads();
   }
 }
@@ -1556,7 +1565,7 @@ static GenParams gen_params(const Param &par) {
 // zlim: per-codeblock pass limits (host array of ncb) or null for all passes.
 static int v2_code(Workspace *ws, const bwc_codec *c, const Param &par, const CbTables &tb, const uint8_t *Kh,
                    const std::vector<uint64_t> &list, const uint8_t *zlim, uint8_t *d_data, const uint8_t *d_bp,
-                   const GenParams &gp) {
+                   const uint32_t *d_hist, const GenParams &gp) {
   const uint64_t ncb = par.ncb;
   std::vector<uint64_t> symoff;
   const uint64_t so = stream_offsets(par, tb, Kh, zlim, symoff);
@@ -1587,7 +1596,8 @@ static int v2_code(Workspace *ws, const bwc_codec *c, const Param &par, const Cb
 #define MQ_LAUNCH(N)                                                                                           \
   hipLaunchKernelGGL(t1_mq_kernel<N>, dim3((nl + N - 1) / N), dim3(64), 0, s, dl, (uint64_t)nl, ws->d_cbmeta, \
                      ws->d_cbsb, ws->d_K, ws->d_symoff, ws->d_sym, ws->d_pcnt, ws->d_pmse, ws->d_doff, d_data,    \
-                     ws->d_Kmsbs, ws->d_Z, ws->d_L, ws->d_S, ws->d_snap, ws->d_mse, ws->d_flags, dz, ws->d_U)
+                     ws->d_Kmsbs, ws->d_Z, ws->d_L, ws->d_S, ws->d_snap, ws->d_mse, ws->d_flags, dz, ws->d_U,    \
+                     d_hist, ws->d_bfut)
     switch (lanes) {
       case 1: MQ_LAUNCH(1); break;
       case 2: MQ_LAUNCH(2); break;
@@ -1620,20 +1630,26 @@ static uint32_t passes_coded(int K, const uint8_t *zlim, uint64_t cb) {
 }
 
 // The records of the codeblocks in d_list, packed for one copy to the host:
-// per listed codeblock the first m entries of L and m + 1 of S (m = passes
-// coded, poff[r + 1] - poff[r]), then its Z, U and Kmsbs.
+// per listed codeblock the first m entries of L and mse and m + 1 of S (m =
+// passes coded, poff[r + 1] - poff[r]), the bound on its uncoded passes'
+// distortion, then its Z, U and Kmsbs.
 __global__ void pack_records_kernel(const uint64_t *__restrict__ rows, uint32_t nrows,
                                     const uint64_t *__restrict__ poff, const uint32_t *__restrict__ L,
-                                    const uint16_t *__restrict__ S, const uint8_t *__restrict__ Z,
+                                    const uint16_t *__restrict__ S, const double *__restrict__ mse,
+                                    const double *__restrict__ bfut, const uint8_t *__restrict__ Z,
                                     const uint8_t *__restrict__ U, const uint8_t *__restrict__ Km,
-                                    uint32_t *__restrict__ pL, uint16_t *__restrict__ pS, uint8_t *__restrict__ pb) {
+                                    uint32_t *__restrict__ pL, uint16_t *__restrict__ pS, double *__restrict__ pM,
+                                    double *__restrict__ pB, uint8_t *__restrict__ pb) {
   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrows) return;
   const uint64_t cb = rows[r], o = poff[r], m = poff[r + 1] - o;
   const uint32_t *l = L + cb * kMaxPasses;
   const uint16_t *sl = S + cb * (kMaxPasses + 1);
+  const double *ml = mse + cb * kMaxPasses;
   for (uint64_t j = 0; j < m; ++j) pL[o + j] = l[j];
+  for (uint64_t j = 0; j < m; ++j) pM[o + j] = ml[j];
   for (uint64_t j = 0; j <= m; ++j) pS[o + r + j] = sl[j];
+  pB[r] = bfut[cb];
   pb[3 * r] = Z[cb];
   pb[3 * r + 1] = U[cb];
   pb[3 * r + 2] = Km[cb];
@@ -1648,7 +1664,8 @@ static int fetch_rows(Workspace *ws, CblkRecords &rec, const std::vector<uint64_
   std::vector<uint64_t> poff(n + 1, 0);
   for (uint64_t r = 0; r < n; ++r) poff[r + 1] = poff[r] + m[r];
   const uint64_t tl = poff[n];
-  const size_t oS = (4 * tl + 255) & ~(size_t)255, oB = (oS + 2 * (tl + n) + 255) & ~(size_t)255;
+  const size_t oS = (4 * tl + 255) & ~(size_t)255, oM = (oS + 2 * (tl + n) + 255) & ~(size_t)255;
+  const size_t oD = (oM + 8 * tl + 255) & ~(size_t)255, oB = (oD + 8 * n + 255) & ~(size_t)255;
   const size_t oF = (oB + 3 * n + 255) & ~(size_t)255, total = oF + 16;
   size_t cap = ws->cap_poff;
   CHK(ensure((void **)&ws->d_poff, &cap, (n + 1) * sizeof(uint64_t)));
@@ -1666,8 +1683,9 @@ static int fetch_rows(Workspace *ws, CblkRecords &rec, const std::vector<uint64_
   if (n) {
     CHK(up_copy(ws, ws->d_poff, poff.data(), (n + 1) * sizeof(uint64_t)));
     hipLaunchKernelGGL(pack_records_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, ws->stream, ws->d_list,
-                       (uint32_t)n, ws->d_poff, ws->d_L, ws->d_S, ws->d_Z, ws->d_U, ws->d_Kmsbs,
-                       (uint32_t *)ws->d_rpack, (uint16_t *)(ws->d_rpack + oS), ws->d_rpack + oB);
+                       (uint32_t)n, ws->d_poff, ws->d_L, ws->d_S, ws->d_mse, ws->d_bfut, ws->d_Z, ws->d_U,
+                       ws->d_Kmsbs, (uint32_t *)ws->d_rpack, (uint16_t *)(ws->d_rpack + oS),
+                       (double *)(ws->d_rpack + oM), (double *)(ws->d_rpack + oD), ws->d_rpack + oB);
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(ws->h_down, ws->d_rpack, oB + 3 * n, hipMemcpyDeviceToHost, ws->stream));
   }
@@ -1677,7 +1695,10 @@ static int fetch_rows(Workspace *ws, CblkRecords &rec, const std::vector<uint64_
   if (hfl[0]) return 0;
   const uint32_t *pL = (const uint32_t *)ws->h_down;
   const uint16_t *pS = (const uint16_t *)(ws->h_down + oS);
+  const double *pM = (const double *)(ws->h_down + oM), *pD = (const double *)(ws->h_down + oD);
   const uint8_t *pb = ws->h_down + oB;
+  if (rec.mse.size() != rec.n * kMaxPasses) rec.mse.assign(rec.n * kMaxPasses, 0.0);
+  if (rec.bfut.size() != rec.n) rec.bfut.assign(rec.n, 0.0);
   for (uint64_t r = 0; r < n; ++r) {
     const uint64_t cb = rows[r];
     if (pb[3 * r] != m[r]) {
@@ -1690,6 +1711,8 @@ static int fetch_rows(Workspace *ws, CblkRecords &rec, const std::vector<uint64_
     rec.Kmsbs[cb] = pb[3 * r + 2];
     std::memcpy(&rec.L[cb * kMaxPasses], pL + poff[r], m[r] * sizeof(uint32_t));
     std::memcpy(&rec.S[cb * (kMaxPasses + 1)], pS + poff[r] + r, (m[r] + 1) * sizeof(uint16_t));
+    std::memcpy(&rec.mse[cb * kMaxPasses], pM + poff[r], m[r] * sizeof(double));
+    rec.bfut[cb] = pD[r];
   }
   return 0;
 }
@@ -1712,7 +1735,7 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
     void **tabs[] = {(void **)&ws->d_cbbox, (void **)&ws->d_cbsb, (void **)&ws->d_cbmeta, (void **)&ws->d_qoff,
                      (void **)&ws->d_doff,  (void **)&ws->d_K,    (void **)&ws->d_Kmsbs,  (void **)&ws->d_Z,
                      (void **)&ws->d_L,     (void **)&ws->d_S,    (void **)&ws->d_snap,   (void **)&ws->d_mse,
-                     (void **)&ws->d_U,     (void **)&ws->d_zlim};
+                     (void **)&ws->d_U,     (void **)&ws->d_zlim,   (void **)&ws->d_bfut};
     for (void **p : tabs)
       if (*p) {
         HIP_OK(hipFree(*p));
@@ -1720,6 +1743,7 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
       }
     const uint64_t n = ncb + 1;
     HIP_OK(hipMalloc((void **)&ws->d_U, n));
+    HIP_OK(hipMalloc((void **)&ws->d_bfut, n * sizeof(double)));
     HIP_OK(hipMalloc((void **)&ws->d_zlim, n));
     HIP_OK(hipMalloc((void **)&ws->d_cbbox, 8 * n * sizeof(uint32_t)));
     HIP_OK(hipMalloc((void **)&ws->d_cbsb, 3 * n * sizeof(double)));
@@ -1787,9 +1811,18 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
     }
     const int nl = c->control.nLayers;
     const float rate = nl > 0 && c->control.bitrate ? c->control.bitrate[nl - 1] : 1.0f;
+    // codings to a pass limit need the bit-length histogram for the bound on
+    // the distortion of the passes left out (kept with the bitplanes)
+    uint32_t *d_hist = nullptr;
+    if (lim.model || lim.fixed) {
+      cap = bs.cap_hist;
+      CHK(ensure((void **)&bs.hist, &cap, (ncb + 1) * 65 * sizeof(uint32_t)));
+      bs.cap_hist = cap;
+      d_hist = bs.hist;
+    }
     hipLaunchKernelGGL(bitplane_kernel, dim3((uint32_t)std::min<uint64_t>(ncb, 65535)), dim3(256), 0, ws->stream,
                        work, W, H, D, ws->d_cbbox, ws->d_cbsb, ws->d_bpoff, bs.bp, ws->d_K, ncb, ws->d_cbmeta,
-                       d_feat, ws->d_tstat, rate > 0 ? rate : 1.0f);
+                       d_feat, ws->d_tstat, rate > 0 ? rate : 1.0f, d_hist);
     HIP_OK(hipGetLastError());
     if (lim.model) {
       char e[200];
@@ -1812,7 +1845,7 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
         list.push_back(i);
         m.push_back(passes_coded(rec.K[i], zlim, i));
       }
-    CHK(v2_code(ws, c, par, tb, rec.K.data(), list, zlim, d_data, bs.bp, gp));
+    CHK(v2_code(ws, c, par, tb, rec.K.data(), list, zlim, d_data, bs.bp, d_hist, gp));
     CHK(fetch_rows(ws, rec, list, m, hfl));
   } else {
     cap = ws->cap_q;
@@ -1882,7 +1915,8 @@ int t1_extend(Workspace *ws, const bwc_codec *c, const Param &par, CblkRecords &
   CHK(up_copy(ws, ws->d_bpoff, tb.bpoff.data(), (ncb + 1) * sizeof(uint64_t)));
   CHK(up_copy(ws, ws->d_K, rec.K.data(), ncb));
   hipLaunchKernelGGL(init_flags_kernel, dim3(1), dim3(64), 0, ws->stream, ws->d_flags);
-  CHK(v2_code(ws, c, par, tb, rec.K.data(), cbs, zlim, ws->d_datap[slot], ws->bp_slots[slot].bp, gp));
+  CHK(v2_code(ws, c, par, tb, rec.K.data(), cbs, zlim, ws->d_datap[slot], ws->bp_slots[slot].bp,
+              ws->bp_slots[slot].hist, gp));
   // the re-coded rows: Z, U, Kmsbs, L, S (the device arrays hold other
   // parameters' rows as well; only the listed rows reach the records)
   std::vector<uint32_t> m(cbs.size());

@@ -67,6 +67,8 @@ struct Workspace {
   struct BpSlot {
     uint8_t *bp = nullptr;
     size_t cap_bp = 0;
+    uint32_t *hist = nullptr;     // bit-length histogram, 65 per codeblock
+    size_t cap_hist = 0;
   };
   std::vector<BpSlot> bp_slots;
   uint8_t *d_U = nullptr, *d_zlim = nullptr;
@@ -90,6 +92,7 @@ struct Workspace {
   uint8_t *h_up = nullptr, *h_down = nullptr;
   size_t cap_up = 0, used_up = 0, cap_down = 0;
   hipEvent_t ev_up = nullptr;
+  double *d_bfut = nullptr;       // bound on the distortion of uncoded passes, per codeblock
   uint64_t *d_poff = nullptr;     // packed-record offsets per listed codeblock
   uint8_t *d_rpack = nullptr;     // packed records
   size_t cap_poff = 0, cap_rpack = 0;

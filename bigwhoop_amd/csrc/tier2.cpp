@@ -861,15 +861,58 @@ struct T2 {
       const int64_t fm = slope > 0 ? m[(uint16_t)(slope - 1)] + base : INT64_MAX;
       if (!(f0 < target && fm > target)) chk->fallback = true;
     }
-    const uint16_t t = slope > 0 ? (uint16_t)(slope - 1) : 0;
+    // Per codeblock coded to a prefix of its passes, at thresholds t = T - 1
+    // and T, with h its cut at t in the prefix coding:
+    // (b) h lies `margin` passes before U, so the body bytes and the lengths
+    //     up to the cut are those of a complete coding;
+    // (c) no point after h can reach slope t in a complete coding, so the
+    //     hull points up to h stay (a later point pops h only with a slope
+    //     from h at least h's own, which is >= t) and none after h has slope
+    //     >= t (slopes fall along the hull). A point i after h has slope
+    //     (D[h..i]) / (L[i] - L[h - 1]) from h; D of the passes from U on
+    //     (lengths there may change) and of the uncoded ones is bounded by
+    //     their positive part and the tier-1 bound, and L[i] >= L[Zs - 1]
+    //     with Zs = min(Z, U). Slopes below 2^e map below t (slope2log,
+    //     tier1.c:2100-2130), so D < 2^e (L[Zs - 1] - L[h - 1]) suffices.
+    // Codeblocks failing either are flagged (each once) to be coded further.
     for (int j = 0; j < c->info.nPar; ++j) {
       CblkRecords &R = rec(j);
-      for (uint64_t i = 0; i < R.n; ++i) {
-        if (R.Z[i] >= R.Zfull[i]) continue;
-        const int z = cut_of(j, R, i, t);
-        if (z + chk->margin > (int)R.U[i]) chk->flagged[j].push_back(i);
+      std::vector<uint8_t> seen;
+      for (uint16_t t : {slope > 0 ? (uint16_t)(slope - 1) : (uint16_t)0, slope}) {
+        const double lam = slope_floor(t);
+        for (uint64_t i = 0; i < R.n; ++i) {
+          if (R.Z[i] >= R.Zfull[i]) continue;
+          const int h = cut_of(j, R, i, t);
+          const int Zs = std::min<int>(R.Z[i], R.U[i]);
+          bool ok = h + chk->margin <= (int)R.U[i] && h < Zs && i < R.bfut.size();
+          if (ok) {
+            const double *ms = &R.mse[i * kMaxPasses];
+            double D = R.bfut[i];
+            for (int z = h; z < Zs; ++z) D += ms[z];
+            for (int z = Zs; z < R.Z[i]; ++z) D += std::max(ms[z], 0.0);
+            const uint32_t *L = &R.L[i * kMaxPasses];
+            const double dL = (double)L[Zs - 1] - (h > 0 ? (double)L[h - 1] : 0.0);
+            ok = D < lam * dL;
+          }
+          if (!ok) {
+            if (seen.empty()) seen.assign(R.n, 0);
+            if (!seen[i]) chk->flagged[j].push_back(i);
+            seen[i] = 1;
+          }
+        }
       }
     }
+  }
+
+  // A slope that slope2log maps to t or above is at least 2^e with
+  // e = 32 + ceil((t - 65536 - 255) / 256): slope2log gives
+  // 256 (e - 32) + f + 65536 for slope 1.m * 2^e, with f in [0, 255]. Below
+  // 3 the value clamps (every positive slope reaches it): 0.
+  static double slope_floor(uint16_t t) {
+    if (t <= 2) return 0.0;
+    const int num = (int)t - 65536 - 255;
+    const int e = 32 + (num >= 0 ? (num + 255) / 256 : -((-num) / 256));
+    return std::ldexp(1.0, e);
   }
 
   void rate_control() {

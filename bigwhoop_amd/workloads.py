@@ -3,6 +3,7 @@
     python -m bigwhoop_amd.workloads learned1024   # 1024^3 f64 k^-5/3, 1 and 2 bpv
     python -m bigwhoop_amd.workloads layers512     # 512^3 f64, 4 layers at 1, 2, 4, 8 bpv
     python -m bigwhoop_amd.workloads turb4d        # 256^3 x 16 x 5 variables, 2 bpv
+    python -m bigwhoop_amd.workloads batch64       # 64 uneven fields (1-4 axes, f32/f64)
 
 Each prints one JSON line per case: input GB/s of the plain encoder (every
 pass coded) and of the learned pass-skip encoder on the same field, the
@@ -71,6 +72,85 @@ def turbulence_4d(n=256, nt=16, seed=7, rho=0.95):
     return out
 
 
+def grf_nd_device(shape, slope, seed, dtype):
+    """Gaussian random field on the GPU over the axes of `shape` (x fastest;
+    1 for absent axes): E(k) ~ k^slope in d dimensions (amplitude filter
+    k^((slope - (d - 1)) / 2)), unit variance, mean 10, cast to dtype. The
+    field is synthesised on the next power-of-two grid and cropped, so the
+    FFTs use few distinct lengths."""
+    import torch
+
+    dims = [n for n in shape if n > 1] or [shape[0]]
+    d = len(dims)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    want = tuple(dims[::-1])     # numpy order: slowest axis first
+    tshape = tuple(1 << max(3, (n - 1).bit_length()) for n in want)
+    white = torch.randn(tshape, generator=g, device="cuda", dtype=torch.float64)
+    spec = torch.fft.rfftn(white)
+    del white
+    k2 = None
+    for a, n in enumerate(tshape):
+        f = (torch.fft.rfftfreq(n, d=1.0 / n, device="cuda", dtype=torch.float64) if a == d - 1
+             else torch.fft.fftfreq(n, d=1.0 / n, device="cuda", dtype=torch.float64))
+        sh = [1] * d
+        sh[a] = -1
+        t = (f ** 2).reshape(sh)
+        k2 = t if k2 is None else k2 + t
+    kmag = torch.sqrt(k2)
+    del k2
+    spec *= torch.where(kmag > 0, kmag.clamp_min(1e-30) ** ((slope - (d - 1)) / 2.0), torch.zeros_like(kmag))
+    del kmag
+    out = torch.fft.irfftn(spec, s=tshape)
+    del spec
+    out = out[tuple(slice(0, n) for n in want)].contiguous()
+    out -= out.mean()
+    out /= out.std()
+    out += 10.0
+    return out.to(dtype).contiguous()
+
+
+def batch_spec(count=64, seed=4):
+    """The uneven batch: `count` fields with dimensionality uniform in 1-4,
+    element count log-uniform in [1e5, 1e8] split over the axes with random
+    aspect ratios, float32 or float64 (50/50), spectral slope U(-3, -1),
+    bitrate log-uniform in [0.5, 8] bits per value, DWT levels
+    min(5, floor(log2(shortest axis)) - 2) on every axis (at least 0), and
+    codeblocks of 2^6, 2^5 x 2^5, 2^5 x 2^5 x 2^3 or 2^4 x 2^4 x 2^3 x 2^2 values
+    for 1-4 axes."""
+    import numpy as np
+
+    rng = np.random.Generator(np.random.PCG64(seed))
+    cbs = {1: (6, 0, 0, 0), 2: (5, 5, 0, 0), 3: (5, 5, 3, 0), 4: (4, 4, 3, 2)}
+    out = []
+    for i in range(count):
+        d = int(rng.integers(1, 5))
+        n = float(np.exp(rng.uniform(np.log(1e5), np.log(1e8))))
+        w = rng.uniform(-1.0, 1.0, size=d)
+        w -= w.mean()
+        dims = [max(8, int(round(n ** (1.0 / d) * np.exp(x)))) for x in w]
+        shape = tuple(dims + [1] * (4 - d))
+        lv = max(0, min(5, int(np.floor(np.log2(min(dims)))) - 2))
+        out.append(dict(shape=shape, ndim=d, dtype="float32" if rng.uniform() < 0.5 else "float64",
+                        slope=float(rng.uniform(-3.0, -1.0)),
+                        rate=f"{float(np.exp(rng.uniform(np.log(0.5), np.log(8.0)))):.3f}",
+                        decomp=tuple(lv if a < d else 0 for a in range(4)), cb=cbs[d], seed=1000 + i))
+    return out
+
+
+def batch_fields(spec):
+    """The fields of batch_spec on the GPU (one tensor each)."""
+    import torch
+
+    fields = []
+    for i, f in enumerate(spec):
+        dt = torch.float32 if f["dtype"] == "float32" else torch.float64
+        fields.append(grf_nd_device(f["shape"], f["slope"], f["seed"], dt))
+        torch.cuda.synchronize()
+        print(f"batch: field {i} {f['shape']} {f['dtype']} made", file=sys.stderr, flush=True)
+    return fields
+
+
 def _time(codec, ptr, steps):
     import torch
 
@@ -115,11 +195,71 @@ def compare(name, shape, npar, ptr, nbytes, settings, steps=2, warmup=1):
     return res
 
 
+def batch64(count=64, seed=4, steps=2):
+    """The uneven batch compressed field after field (one bwc_compress per
+    field) by the plain and by the learned encoder; aggregate input GB/s over
+    the whole batch; every field's two codestreams must be identical."""
+    import torch
+    from bigwhoop_amd import bwc, skipnet
+
+    spec = batch_spec(count, seed)
+    fields = batch_fields(spec)
+    total = sum(f.numel() * f.element_size() for f in fields)
+
+    def codecs(learned):
+        cs = []
+        for f, t in zip(spec, fields):
+            c = bwc.Codec(f["shape"], prec=t.element_size(), rate=f["rate"], decomp=f["decomp"], cb=f["cb"])
+            if learned:
+                c.set_skip_model(skipnet.default_weights())
+                c.set_pass_skip(2, SKIP_MARGIN)
+            cs.append(c)
+        return cs
+
+    def run(cs):
+        for i, (c, t) in enumerate(zip(cs, fields)):
+            c.compress_into(t.data_ptr())
+            print(f"batch: field {i} compressed", file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sizes = []
+        for _ in range(steps):
+            sizes = [c.compress_into(t.data_ptr()) for c, t in zip(cs, fields)]
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        return dt, [c.out[:n].tobytes() for c, n in zip(cs, sizes)]
+
+    pl = codecs(False)
+    dt_p, cs_p = run(pl)
+    print(f"batch: plain {dt_p * 1e3:.1f} ms per pass", file=sys.stderr, flush=True)
+    for c in pl:
+        c.close()
+    le = codecs(True)
+    dt_l, cs_l = run(le)
+    stats = [c.skip_stats() for c in le]
+    for c in le:
+        c.close()
+    same = all(a == b for a, b in zip(cs_p, cs_l))
+    res = {"workload": f"batch of {count} uneven fields (seed {seed}), one bwc_compress per field",
+           "fields": count, "input_bytes": total,
+           "dims": [sum(1 for f in spec if f["ndim"] == d) for d in (1, 2, 3, 4)],
+           "float32_fields": sum(1 for f in spec if f["dtype"] == "float32"),
+           "plain_gbs": round(total / dt_p / 1e9, 3), "plain_ms": round(dt_p * 1e3, 2),
+           "learned_gbs": round(total / dt_l / 1e9, 3), "learned_ms": round(dt_l * 1e3, 2),
+           "speedup_learned_vs_plain": round(dt_p / dt_l, 3), "codestream_bytes": sum(len(c) for c in cs_l),
+           "identical": same,
+           "extra_rounds": sum(s["extra_rounds"] for s in stats), "fallbacks": sum(s["fallbacks"] for s in stats)}
+    print(json.dumps(res), flush=True)
+    if not same:
+        raise SystemExit("batch: learned and plain codestreams differ")
+    return res
+
+
 def main(argv):
     import torch
     from bigwhoop_amd.fields import grf_device
 
-    which = argv[1:] or ["learned1024", "layers512", "turb4d"]
+    which = argv[1:] or ["learned1024", "layers512", "turb4d", "batch64"]
     for w in which:
         if w == "learned1024":
             n = 1024
@@ -134,6 +274,8 @@ def main(argv):
             compare("512^3 k^-5/3, layers 1,2,4,8 bpv", (n, n, n), 1, f.data_ptr(), n ** 3 * 8,
                     dict(rate="1,2,4,8", decomp=(5, 5, 5, 0), cb=(5, 5, 3, 0)))
             del f
+        elif w == "batch64":
+            batch64()
         elif w == "turb4d":
             f = turbulence_4d()
             compare("256^3 x 16, 5 variables, 2 bpv", (256, 256, 256, 16), 5, f.data_ptr(), f.numel() * 8,
