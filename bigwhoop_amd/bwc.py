@@ -86,6 +86,8 @@ def lib():
             "bwc_amd_host_free": (None, [vp]),
             "bwc_amd_last_error": (C.c_char_p, []),
             "bwc_amd_reset_gain_lut": (None, []),
+            "bwc_amd_compress_batch": (C.c_int, [C.POINTER(vp), C.POINTER(vp), C.c_int, C.POINTER(C.c_size_t),
+                                                 C.c_int]),
             "bwc_amd_set_tuning": (C.c_int, [C.c_char_p, C.c_int]),
             "bwc_amd_get_tuning": (C.c_int, [C.c_char_p, C.POINTER(C.c_int)]),
             "bwc_amd_tuning_source": (C.c_char_p, []),
@@ -212,6 +214,23 @@ def get_tuning(name: str = None):
     if lib().bwc_amd_get_tuning(name.encode(), C.byref(v)):
         raise ValueError(f"unknown tuning knob {name}")
     return v.value
+
+
+def compress_batch(codecs, ptrs, workers: int = 0):
+    """Compress each Codec from its host or device address, concurrently
+    (bwc_amd_compress_batch); returns the codestream sizes (each codec's
+    .out holds its codestream until its next call)."""
+    n = len(codecs)
+    if len(ptrs) != n:
+        raise ValueError("one input address per codec")
+    for c, p in zip(codecs, ptrs):
+        _Stream.from_address(c.stream).inp = p
+    hs = (C.c_void_p * n)(*[c.h for c in codecs])
+    ss = (C.c_void_p * n)(*[c.stream for c in codecs])
+    sizes = (C.c_size_t * n)()
+    if lib().bwc_amd_compress_batch(hs, ss, n, sizes, int(workers)):
+        raise RuntimeError("bwc_amd_compress_batch failed: " + last_error())
+    return [int(x) for x in sizes]
 
 
 def tuning_source() -> str:

@@ -9,6 +9,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
+#include <mutex>
+#include <string>
+#include <thread>
 
 #include "codec.hpp"
 #include "gpu.hpp"
@@ -458,7 +462,11 @@ uchar create_codec(bwc_codec *const codec) {
   // accumulated into it by then (libbwc.c:450 via create_codec)
   std::memcpy(job->lut, gain_lut(), sizeof job->lut);
   job->plan = Plan();
-  if (build_plan(codec, job)) return 1;
+  const int rc = build_plan(codec, job);
+  if (rc == 2)
+    set_err("a tile bound above 2^24 rounds in single precision so that the resolution boxes "
+            "(libbwc.c:1539-1546) would reach past the tile's samples; such extents are not supported");
+  if (rc) return 1;
   job->created = true;
   return 0;
 }
@@ -732,6 +740,54 @@ int bwc_amd_device_count(void) { return gpu::device_count(); }
 void *bwc_amd_host_alloc(size_t bytes) { return gpu::host_alloc(bytes); }
 void bwc_amd_host_free(void *p) { gpu::host_free(p); }
 const char *bwc_amd_last_error(void) { return g_api_err[0] ? g_api_err : gpu::last_error(); }
+
+int bwc_amd_compress_batch(bwc_codec *const *codecs, bwc_stream *const *streams, int n, size_t *sizes,
+                           int workers) {
+  if (n <= 0) return 0;
+  if (!codecs || !streams || !sizes) {
+    set_err("bwc_amd_compress_batch: null argument");
+    return 1;
+  }
+  // workspaces (device state, streams) made here, on the calling thread
+  for (int i = 0; i < n; ++i) {
+    sizes[i] = 0;
+    Job *job = codecs[i] ? job_of(codecs[i]) : nullptr;
+    if (!job || !job->created || !streams[i]) {
+      set_err("bwc_amd_compress_batch: a codec is not ready for compression");
+      return 1;
+    }
+    if (!gpu::workspace_of(job)) {
+      set_err(gpu::last_error());
+      return 1;
+    }
+  }
+  if (workers <= 0) workers = 8;
+  workers = std::min(workers, n);
+  int device = 0;
+  (void)hipGetDevice(&device);
+  std::atomic<int> next{0};
+  std::mutex em;
+  std::string first_err;
+  auto work = [&] {
+    (void)hipSetDevice(device);
+    for (int i; (i = next.fetch_add(1)) < n;) {
+      sizes[i] = bwc_compress(codecs[i], streams[i]);
+      if (!sizes[i]) {
+        std::lock_guard<std::mutex> lk(em);
+        if (first_err.empty()) first_err = bwc_amd_last_error();
+      }
+    }
+  };
+  std::vector<std::thread> th;
+  for (int w = 1; w < workers; ++w) th.emplace_back(work);
+  work();
+  for (auto &t : th) t.join();
+  if (!first_err.empty()) {
+    std::snprintf(g_api_err, sizeof g_api_err, "%s", first_err.c_str());
+    return 1;
+  }
+  return 0;
+}
 
 int bwc_amd_set_tuning(const char *name, int value) {
   if (set_tuning(name, value)) {

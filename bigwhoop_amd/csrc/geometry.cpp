@@ -293,6 +293,17 @@ int build_plan(bwc_codec *codec, Job *job) {
   const uint64_t ts[4] = {ctl.tileSizeX, ctl.tileSizeY, ctl.tileSizeZ, ctl.tileSizeTS};
   uint32_t nt[4];
   for (int a = 0; a < 4; ++a) nt[a] = (uint32_t)(int)std::ceil(((float)n[a] / ts[a]));
+  // Resolution boxes are computed in single precision (libbwc.c:1539-1546),
+  // so tile bounds above 2^24 round to the nearest float. Where that makes a
+  // tile's finest resolution box longer than the tile, the transform would
+  // run past the tile's samples (the reference reads and writes beyond its
+  // tile buffer there): such extents are refused. Shorter boxes are kept,
+  // as in the reference.
+  for (int a = 0; a < 4; ++a)
+    for (uint32_t t = 0; t < nt[a]; ++t) {
+      const uint64_t lo = (uint64_t)t * ts[a], hi = std::min<uint64_t>(n[a], (uint64_t)(t + 1) * ts[a]);
+      if ((uint64_t)std::ceil((float)hi) - (uint64_t)std::ceil((float)lo) > hi - lo) return 2;
+    }
   Plan &plan = job->plan;
   plan.codec = codec;
   plan.tiles.assign(ctl.nTiles, Tile());

@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 
 #include "gpu.hpp"
 #include "tuning.hpp"
@@ -2105,7 +2106,11 @@ int init_device(Workspace *ws) {
   ws->device = dev;
   HIP_OK(hipStreamCreateWithFlags(&ws->stream, hipStreamNonBlocking));
   for (auto &e : ws->ev) HIP_OK(hipEventCreate(&e));
+  // constant tables and kernel attributes, once per process (workspaces may
+  // be created from several host threads)
+  static std::mutex m;
   static bool tables = false;
+  std::lock_guard<std::mutex> lk(m);
   if (!tables) {
     if (upload_tables()) return 1;
     const int lds_max = 160 * 1024;

@@ -20,6 +20,8 @@
 // costs further coding, too many only a little extra work.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <cstdio>
 #include <utility>
 #include <vector>
@@ -412,11 +414,11 @@ size_t skipnet_train_lds() { return sizeof(TrainLds); }
 int skipnet_train(hipStream_t s, const float *d_feat, const float *d_tgt, const uint32_t *d_order, uint64_t n,
                   uint64_t start, int steps, float lr, int t0, float *d_w, float *d_m, float *d_v, float *d_g,
                   float *d_loss, int mode, char *err, size_t errlen) {
-  static bool attr = false;
-  if (!attr) {
+  static std::atomic<bool> attr{false};
+  if (!attr.load()) {
     SN_OK(hipFuncSetAttribute((const void *)skipnet_train_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)sizeof(TrainLds)));
-    attr = true;
+    attr.store(true);
   }
   hipLaunchKernelGGL(skipnet_train_kernel, dim3(1), dim3(256), sizeof(TrainLds), s, d_feat, d_tgt, d_order, n,
                      start, steps, lr, t0, d_w, d_m, d_v, d_g, d_loss, mode);

@@ -105,13 +105,15 @@ class ProbePool {
   bool stop_ = false;
 };
 
-// the process's pool with `workers` threads (rebuilt if another size is asked)
-ProbePool &probe_pool(int workers) {
+// Runs fn(0 .. n-1) on the process's pool of `workers` threads (rebuilt
+// when another size is asked). Runs from different host threads (concurrent
+// compressions) take the pool one after the other.
+void pool_run(int workers, size_t n, const std::function<void(size_t)> &fn) {
   static std::unique_ptr<ProbePool> pool;
   static std::mutex m;
   std::lock_guard<std::mutex> lk(m);
   if (!pool || pool->size() != workers) pool.reset(new ProbePool(workers));
-  return *pool;
+  pool->run(n, fn);
 }
 
 // MSB-first bit writer with JPEG 2000 style stuffing: after an emitted 0xFF
@@ -467,7 +469,7 @@ struct T2 {
       return;
     }
     const std::function<void(size_t)> task = [&](size_t k) { fn(n * k / nch, n * (k + 1) / nch); };
-    probe_pool(threads - 1).run((size_t)nch, task);
+    pool_run(threads - 1, (size_t)nch, task);
   }
 
   void build_hull_index() {
@@ -809,7 +811,7 @@ struct T2 {
       for (size_t a = 0; a < Ts.size(); ++a) out[a] = probe(Ts[a], q, pbufs[0]);
     } else {
       const std::function<void(size_t)> fn = [&](size_t a) { out[a] = probe(Ts[a], q, pbufs[a]); };
-      probe_pool(threads - 1).run(Ts.size(), fn);
+      pool_run(threads - 1, Ts.size(), fn);
     }
     for (size_t a = 0; a < Ts.size(); ++a) memo[Ts[a]] = out[a];
   }

@@ -116,12 +116,13 @@ def batch_spec(count=64, seed=4):
     aspect ratios, float32 or float64 (50/50), spectral slope U(-3, -1),
     bitrate log-uniform in [0.5, 8] bits per value, DWT levels
     min(5, floor(log2(shortest axis)) - 2) on every axis (at least 0), and
-    codeblocks of 2^6, 2^5 x 2^5, 2^5 x 2^5 x 2^3 or 2^4 x 2^4 x 2^3 x 2^2 values
-    for 1-4 axes."""
+    codeblocks of 2^10, 2^5 x 2^5, 2^5 x 2^5 x 2^3 or 2^4 x 2^4 x 2^3 x 2^2 values
+    for 1-4 axes. An extent above 2^24 that would round up to a float is
+    lowered to the float below it (the codec refuses the former)."""
     import numpy as np
 
     rng = np.random.Generator(np.random.PCG64(seed))
-    cbs = {1: (6, 0, 0, 0), 2: (5, 5, 0, 0), 3: (5, 5, 3, 0), 4: (4, 4, 3, 2)}
+    cbs = {1: (10, 0, 0, 0), 2: (5, 5, 0, 0), 3: (5, 5, 3, 0), 4: (4, 4, 3, 2)}
     out = []
     for i in range(count):
         d = int(rng.integers(1, 5))
@@ -129,6 +130,9 @@ def batch_spec(count=64, seed=4):
         w = rng.uniform(-1.0, 1.0, size=d)
         w -= w.mean()
         dims = [max(8, int(round(n ** (1.0 / d) * np.exp(x)))) for x in w]
+        # extents above 2^24 that round up to a float are refused by the codec
+        # (single-precision resolution boxes): take the float just below
+        dims = [m if int(np.float32(m)) <= m else int(np.nextafter(np.float32(m), np.float32(0))) for m in dims]
         shape = tuple(dims + [1] * (4 - d))
         lv = max(0, min(5, int(np.floor(np.log2(min(dims)))) - 2))
         out.append(dict(shape=shape, ndim=d, dtype="float32" if rng.uniform() < 0.5 else "float64",
@@ -217,9 +221,12 @@ def batch64(count=64, seed=4, steps=2):
         return cs
 
     def run(cs):
+        per = []
         for i, (c, t) in enumerate(zip(cs, fields)):
+            t0 = time.perf_counter()
             c.compress_into(t.data_ptr())
-            print(f"batch: field {i} compressed", file=sys.stderr, flush=True)
+            per.append(time.perf_counter() - t0)
+            print(f"batch: field {i} compressed in {per[-1] * 1e3:.1f} ms", file=sys.stderr, flush=True)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         sizes = []
@@ -227,28 +234,53 @@ def batch64(count=64, seed=4, steps=2):
             sizes = [c.compress_into(t.data_ptr()) for c, t in zip(cs, fields)]
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / steps
-        return dt, [c.out[:n].tobytes() for c, n in zip(cs, sizes)]
+        return dt, [c.out[:n].tobytes() for c, n in zip(cs, sizes)], per
+
+    def run_batch(cs, ref):
+        ptrs = [t.data_ptr() for t in fields]
+        bwc.compress_batch(cs, ptrs)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            sizes = bwc.compress_batch(cs, ptrs)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        if any(c.out[:n].tobytes() != r for c, n, r in zip(cs, sizes, ref)):
+            raise SystemExit("batch: concurrent compression differs from one field at a time")
+        return dt
 
     pl = codecs(False)
-    dt_p, cs_p = run(pl)
+    dt_p, cs_p, per_p = run(pl)
     print(f"batch: plain {dt_p * 1e3:.1f} ms per pass", file=sys.stderr, flush=True)
+    dtc_p = run_batch(pl, cs_p)
+    print(f"batch: plain, concurrent {dtc_p * 1e3:.1f} ms per pass", file=sys.stderr, flush=True)
     for c in pl:
         c.close()
     le = codecs(True)
-    dt_l, cs_l = run(le)
+    dt_l, cs_l, per_l = run(le)
+    print(f"batch: learned {dt_l * 1e3:.1f} ms per pass", file=sys.stderr, flush=True)
+    dtc_l = run_batch(le, cs_l)
+    print(f"batch: learned, concurrent {dtc_l * 1e3:.1f} ms per pass", file=sys.stderr, flush=True)
     stats = [c.skip_stats() for c in le]
     for c in le:
         c.close()
     same = all(a == b for a, b in zip(cs_p, cs_l))
-    res = {"workload": f"batch of {count} uneven fields (seed {seed}), one bwc_compress per field",
+    res = {"workload": f"batch of {count} uneven fields (seed {seed})",
            "fields": count, "input_bytes": total,
            "dims": [sum(1 for f in spec if f["ndim"] == d) for d in (1, 2, 3, 4)],
            "float32_fields": sum(1 for f in spec if f["dtype"] == "float32"),
            "plain_gbs": round(total / dt_p / 1e9, 3), "plain_ms": round(dt_p * 1e3, 2),
            "learned_gbs": round(total / dt_l / 1e9, 3), "learned_ms": round(dt_l * 1e3, 2),
            "speedup_learned_vs_plain": round(dt_p / dt_l, 3), "codestream_bytes": sum(len(c) for c in cs_l),
+           "concurrent": {"plain_gbs": round(total / dtc_p / 1e9, 3), "plain_ms": round(dtc_p * 1e3, 2),
+                          "learned_gbs": round(total / dtc_l / 1e9, 3), "learned_ms": round(dtc_l * 1e3, 2),
+                          "workers": 8, "identical_to_one_at_a_time": True},
            "identical": same,
-           "extra_rounds": sum(s["extra_rounds"] for s in stats), "fallbacks": sum(s["fallbacks"] for s in stats)}
+           "extra_rounds": sum(s["extra_rounds"] for s in stats), "fallbacks": sum(s["fallbacks"] for s in stats),
+           "slowest_fields": [dict(shape=spec[i]["shape"], dtype=spec[i]["dtype"], rate=spec[i]["rate"],
+                                   plain_ms=round(per_p[i] * 1e3, 1), learned_ms=round(per_l[i] * 1e3, 1),
+                                   extra_rounds=stats[i]["extra_rounds"])
+                              for i in sorted(range(count), key=lambda k: -per_l[k])[:12]]}
     print(json.dumps(res), flush=True)
     if not same:
         raise SystemExit("batch: learned and plain codestreams differ")

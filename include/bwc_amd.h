@@ -39,6 +39,18 @@ void bwc_amd_host_free(void *p);
    compression per process would see. */
 void bwc_amd_reset_gain_lut(void);
 
+/* Compress n independently configured codecs (each after
+   bwc_create_compression, each with its own stream) concurrently: `workers`
+   host threads (<= 0: 8) take the codecs in turn, each codec's kernels run
+   on its own HIP stream, so the GPU works on several fields at once and rate
+   control of one field overlaps the GPU work of others. No counterpart in
+   the reference; each codestream is byte-identical to what bwc_compress
+   writes for that codec alone. sizes[i] receives the bytes of codestream i
+   (0 on failure). Returns 0 when every codec succeeded. A codec must not
+   appear twice. */
+int bwc_amd_compress_batch(bwc_codec *const *codecs, bwc_stream *const *streams, int n, size_t *sizes,
+                           int workers);
+
 /* Launch-shape knobs of the kernels (no counterpart in the reference); none
    changes a result. Names: "dwt_unroll" (2, 4, 8: pair iterations per load
    batch of the strided DWT passes), "dwt_seg_log2" (8-24: thread target

@@ -542,6 +542,16 @@ static int create_geometry(oc_codec *c)
   uint32_t nty = (int)ceil(((float)c->n[1] / c->ts[1]));
   uint32_t ntz = (int)ceil(((float)c->n[2] / c->ts[2]));
   uint32_t ntt = (int)ceil(((float)c->n[3] / c->ts[3]));
+  /* the float resolution boxes below round tile bounds above 2^24; a finest
+     box longer than its tile would take the transform past the samples
+     (undefined in the reference), so such extents are refused */
+  const uint32_t nt[4] = {ntx, nty, ntz, ntt};
+  for (int a = 0; a < 4; ++a)
+    for (uint32_t t = 0; t < nt[a]; ++t)
+    {
+      const uint64_t lo = (uint64_t)t * c->ts[a], hi = MN(c->n[a], (uint64_t)(t + 1) * c->ts[a]);
+      if ((uint64_t)ceil((float)hi) - (uint64_t)ceil((float)lo) > hi - lo) return 1;
+    }
   c->tile = calloc(c->ntiles, sizeof(tile_t));
   uint32_t i = 0;
   for (uint32_t t = 0; t < ntt; ++t)
@@ -788,7 +798,11 @@ oc_codec *oc_create(const oc_params *p)
     }
     free(rc);
   }
-  create_geometry(c);
+  if (create_geometry(c))
+  {
+    oc_destroy(c);
+    return NULL;
+  }
   c->header_size = 111 + (uint64_t)c->nPar * c->ntiles * 2 * PREC_BYTES + (uint64_t)c->nlayers * 4;
   return c;
 }

@@ -111,3 +111,23 @@ def test_rate_control_modes_agree_with_oracle_multilayer(threads, monkeypatch):
     finally:
         c.close()
         o.close()
+
+
+@pytest.mark.parametrize("n, ok", [(33554434, True), (48412990, False), (48412992, True), (16777217, True),
+                                   (16777219, False)])
+def test_float_rounded_extents(n, ok):
+    """Resolution boxes are computed in single precision (libbwc.c:1539-1546).
+    A field extent that rounds down to a float (2^25 + 2, 2^24 + 1) leaves a
+    shorter box and is coded as the reference codes it; one that rounds up
+    (48412990 -> 48412992, 2^24 + 3 -> 2^24 + 4) would take the transform past
+    the samples and is refused by the library and the oracle alike."""
+    kw = dict(rate="2", decomp=(1, 0, 0, 0), cb=(6, 0, 0, 0))
+    assert (int(np.float32(n)) <= n) == ok
+    if ok:
+        bwc.Codec((n, 1, 1, 1), **kw).close()
+        ol.Oracle((n, 1, 1, 1), **kw).close()
+    else:
+        with pytest.raises(RuntimeError, match="single precision"):
+            bwc.Codec((n, 1, 1, 1), **kw)
+        with pytest.raises(ValueError, match="rejected"):
+            ol.Oracle((n, 1, 1, 1), **kw)
