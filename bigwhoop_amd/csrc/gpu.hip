@@ -1540,7 +1540,7 @@ static uint64_t stream_offsets(const Param &par, const CbTables &t, const uint8_
 // slices side by side, each with its own pass-output buffer; the buffers take
 // the largest, over the codeblocks, of P times the codeblock's per-slice bound
 struct GenParams {
-  uint32_t maxcol = 0, obuf_bytes = 0, maxcw = 0;
+  uint32_t maxcol = 0, obuf_bytes = 0;
   size_t lds = 0;
 };
 static GenParams gen_params(const Param &par) {
@@ -1550,7 +1550,6 @@ static GenParams gen_params(const Param &par) {
     const uint32_t cw = b[4] - b[0], ch = b[5] - b[1], ns = (ch + 3) / 4;
     const uint32_t ncol = cw * ns * (b[6] - b[2]) * (b[7] - b[3]);
     g.maxcol = std::max(g.maxcol, ncol);
-    g.maxcw = std::max(g.maxcw, cw);
     uint32_t gw = 1;
     while (gw < cw) gw <<= 1;
     const uint32_t nsl = (b[6] - b[2]) * (b[7] - b[3]);
@@ -1781,10 +1780,11 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
   const uint64_t W = par.box.len(0), H = par.box.len(1), D = par.box.len(2);
 
   // fast path: wavefront-per-codeblock symbol generation + few-lane MQ, when
-  // every codeblock row fits one wavefront and its state fits LDS
+  // every codeblock's state fits LDS (rows wider than a wavefront are coded
+  // in 64-column chunks)
   const GenParams gp = gen_params(par);
   const char *force = std::getenv("BWC_AMD_T1");
-  const bool v2 = gp.maxcw <= 64 && gp.lds <= 64 * 1024 && !(force && force[0] == '1');
+  const bool v2 = gp.lds <= 64 * 1024 && !(force && force[0] == '1');
   const int erres = (int)c->control.error_resilience;
   rec.U.assign(ncb, 0);
   rec.Zfull.assign(ncb, 0);
