@@ -1997,7 +1997,11 @@ int t1_decode(Workspace *ws, const bwc_codec *c, const Param &par, const DecodeR
   CHK(up((void **)&ws->dec.soff, &ws->dec.cap_soff, soff.data(), (ncb + 1) * sizeof(uint64_t)));
   CHK(ensure((void **)&ws->dec.state, &ws->dec.cap_state, (soff[ncb] + 1) * sizeof(uint16_t)));
   const uint64_t W = par.box.len(0), H = par.box.len(1), D = par.box.len(2);
-  constexpr int LPW = 8;
+  // codeblocks per wavefront: each lane is a chain of dependent decisions,
+  // so more wavefronts with fewer lanes hide more of the latency (512^3,
+  // 2 bpv: 1 -> 137 ms, 2 -> 134, 4 -> 155, 8 -> 187, 16 -> 205 ms per
+  // decompression)
+  constexpr int LPW = 2;
   hipLaunchKernelGGL(t1_decode_kernel<LPW>, dim3((uint32_t)((ncb + LPW - 1) / LPW)), dim3(64), 0, ws->stream, ncb,
                      ws->dec.box, ws->dec.sb, ws->dec.meta, ws->dec.K, ws->dec.Z, ws->dec.len, ws->dec.off,
                      ws->dec.data, ws->dec.soff, ws->dec.state, W, H, D, work, (int)c->control.error_resilience);
