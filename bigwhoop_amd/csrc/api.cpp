@@ -4,6 +4,7 @@
 // codestream control bits (libbwc.c:1828-2828); bwc_compress runs the hot
 // stages on one GPU (gpu.hip) and tier-2 / codestream assembly on the host
 // (tier2.cpp).
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -119,11 +120,6 @@ void reset_tile_t2(bwc_codec *c, Job *job, uint64_t ti) {
   }
 }
 
-// Rate control of one tile whose codeblocks may be coded to a prefix of
-// their passes: run it, and while its outcome is not provably that of
-// complete codings, code the flagged codeblocks further (limits grow 9, 18,
-// 36 passes per round; complete coding from the fourth round or for the
-// whole tile on a fallback) and run it again.
 // Gather the packet bodies of tile ti on the device, from [body_end] on:
 // each packet's chunks back to back (Packet::dev_off, its body_size bytes).
 int gather_tile_bodies(bwc_codec *c, Job *job, gpu::Workspace *ws, uint64_t ti, uint64_t &body_end) {
@@ -166,6 +162,11 @@ int gather_tile_bodies(bwc_codec *c, Job *job, gpu::Workspace *ws, uint64_t ti, 
   return 0;
 }
 
+// Rate control of one tile whose codeblocks may be coded to a prefix of
+// their passes: run it, and while its outcome is not provably that of
+// complete codings, code the flagged codeblocks further (limits grow 9, 18,
+// 36 passes per round; complete coding from the fourth round or for the
+// whole tile on a fallback) and run it again.
 int rate_control_prefix(bwc_codec *c, Job *job, gpu::Workspace *ws, uint64_t ti) {
   Tile &tl = job->plan.tiles[ti];
   const int np = c->info.nPar;
@@ -748,6 +749,14 @@ int bwc_amd_compress_batch(bwc_codec *const *codecs, bwc_stream *const *streams,
     set_err("bwc_amd_compress_batch: null argument");
     return 1;
   }
+  {  // one codec's workspace and records serve one compression at a time
+    std::vector<bwc_codec *> sorted(codecs, codecs + n);
+    std::sort(sorted.begin(), sorted.end());
+    if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end()) {
+      set_err("bwc_amd_compress_batch: a codec appears twice");
+      return 1;
+    }
+  }
   // workspaces (device state, streams) made here, on the calling thread
   for (int i = 0; i < n; ++i) {
     sizes[i] = 0;
@@ -766,13 +775,15 @@ int bwc_amd_compress_batch(bwc_codec *const *codecs, bwc_stream *const *streams,
   int device = 0;
   (void)hipGetDevice(&device);
   std::atomic<int> next{0};
+  std::atomic<bool> failed{false};
   std::mutex em;
   std::string first_err;
   auto work = [&] {
     (void)hipSetDevice(device);
-    for (int i; (i = next.fetch_add(1)) < n;) {
+    for (int i; !failed.load() && (i = next.fetch_add(1)) < n;) {
       sizes[i] = bwc_compress(codecs[i], streams[i]);
       if (!sizes[i]) {
+        failed.store(true);   // no further codec is started
         std::lock_guard<std::mutex> lk(em);
         if (first_err.empty()) first_err = bwc_amd_last_error();
       }

@@ -46,8 +46,10 @@ void bwc_amd_reset_gain_lut(void);
    control of one field overlaps the GPU work of others. No counterpart in
    the reference; each codestream is byte-identical to what bwc_compress
    writes for that codec alone. sizes[i] receives the bytes of codestream i
-   (0 on failure). Returns 0 when every codec succeeded. A codec must not
-   appear twice. */
+   (0 on failure, or when the codec was not started because another one
+   failed first: no further codec starts after a failure). Returns 0 when
+   every codec succeeded, else 1 with the first error in
+   bwc_amd_last_error(). A codec that appears twice is an error. */
 int bwc_amd_compress_batch(bwc_codec *const *codecs, bwc_stream *const *streams, int n, size_t *sizes,
                            int workers);
 

@@ -321,7 +321,7 @@ def ref_run(mode, data, shape, npar=1, prec=8, rate="2", **kw):
 
 def ref_decode(cs, shape, npar=1, prec=8):
     """Decode a codestream with the reference decoder (bwc_create_decompression
-    + bwc_decompress in the driver); returns the output bytes."""
+    + bwc_decompress in oracle/ref_dump.c); returns the output bytes."""
     n = list(shape) + [1] * (4 - len(shape))
     with tempfile.TemporaryDirectory() as td:
         pre = os.path.join(td, "o")
@@ -358,7 +358,7 @@ def _header_dims(cs):
 
 
 def parse_t1(blob, ncblk_per_param):
-    """Parse the driver's per-codeblock tier-1 records (see oracle/ref_dump.c)."""
+    """Parse the per-codeblock tier-1 records oracle/ref_dump.c writes."""
     recs = []
     slopes = []
     off = 0

@@ -131,3 +131,18 @@ def test_float_rounded_extents(n, ok):
             bwc.Codec((n, 1, 1, 1), **kw)
         with pytest.raises(ValueError, match="rejected"):
             ol.Oracle((n, 1, 1, 1), **kw)
+
+
+def test_batch_rejects_a_codec_twice():
+    """bwc_amd_compress_batch refuses a codec that appears twice (its
+    workspace and records serve one compression at a time), before any GPU
+    work."""
+    kw = dict(rate="2", decomp=(1, 1, 0, 0), cb=(4, 4, 0, 0))
+    a, b = bwc.Codec((32, 32), **kw), bwc.Codec((32, 32), **kw)
+    x = np.zeros(32 * 32)
+    try:
+        with pytest.raises(RuntimeError, match="appears twice"):
+            bwc.compress_batch([a, b, a], [x.ctypes.data] * 3)
+    finally:
+        a.close()
+        b.close()
