@@ -97,6 +97,15 @@ struct Workspace {
   uint8_t *d_rpack = nullptr;     // packed records
   size_t cap_poff = 0, cap_rpack = 0;
   int last_dwt_launches = 0;      // kernel launches of the last DWT call
+  // forward DWT launch sequences as HIP graphs, one per (parameter geometry,
+  // buffers, normalisation, launch knobs): recorded the second time a key is
+  // seen, replayed from then on (exec null: seen once, not recorded yet)
+  struct DwtGraph {
+    uint64_t key;
+    hipGraphExec_t exec;
+    int launches;
+  };
+  std::vector<DwtGraph> dwt_graphs;
   ~Workspace();
 };
 
