@@ -1657,8 +1657,11 @@ static int v2_code(Workspace *ws, const bwc_codec *c, const Param &par, const Cb
   if (list.empty()) return 0;
   const int erres = (int)c->control.error_resilience;
   // codeblocks per wavefront of the MQ kernel (lanes beyond them idle), for
-  // complete codings and for codings to a pass limit (tuning.hpp)
-  const int lpw = zlim ? tuning().mq_lanes_limited : tuning().mq_lanes_full;
+  // complete codings and for codings to a pass limit (tuning.hpp); halved
+  // while fewer than 256 wavefronts (one per CU) would run, since lanes of
+  // one wavefront share its issue slots and a short list leaves CUs idle
+  int lpw = zlim ? tuning().mq_lanes_limited : tuning().mq_lanes_full;
+  while (lpw > 1 && (list.size() + lpw - 1) / lpw < 256) lpw >>= 1;
   auto launch_mq = [&](const uint64_t *dl, uint32_t nl, hipStream_t s, int lanes) -> int {
 #define MQ_LAUNCH(N)                                                                                           \
   hipLaunchKernelGGL(t1_mq_kernel<N>, dim3((nl + N - 1) / N), dim3(64), 0, s, dl, (uint64_t)nl, ws->d_cbmeta, \
