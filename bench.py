@@ -3,15 +3,19 @@
 Workload (one step = one bwc_compress call): a 512^3 float64 Gaussian random
 field with isotropic energy spectrum E(k) ~ k^-5/3 (seeded FFT synthesis,
 unit variance, mean 10), 5 CDF 9/7 levels per axis, codeblocks 32x32x8, one
-quality layer at 2 bits/value. The field is resident in HBM before timing
-starts (the C API reads a device pointer in place); the codestream is written
-to host memory inside the timed region.
+quality layer at 2 bits/value.
 
-`value` is the learned pass-skip encoder (tier-1 codes each codeblock to the
-pass-limit network's prediction plus a margin; rate control proves the cuts
-are those of complete codings). The plain encoder (every pass coded) is timed
-on the same field, and the two codestreams must be byte-identical or the run
-fails.
+`value` is input bytes over the wall time of bwc_compress with the field in
+page-locked host memory, as the C API takes it: the copy to HBM (PCIe) is
+inside the timed call, and the codestream is written to page-locked host
+memory. The encoder is the learned pass-skip encoder (tier-1 codes each
+codeblock to the pass-limit network's prediction plus a margin; rate control
+proves the cuts are those of complete codings). Beside it: the same field in
+ordinary (pageable) host memory, the same field already resident in HBM (a
+device pointer), the plain encoder (every pass coded) and decompression.
+Every codestream must be byte-identical to the others, and at 512^3 to the
+SHA-256 of the codestream the reference's own bwc_compress wrote for this
+field (tests/golden), or the run fails.
 
 Prints one JSON line. Also reports the DWT's achieved HBM bandwidth against
 the 8 TB/s peak (algorithmic bytes: 16 B per value per axis pass, summed over
@@ -52,31 +56,26 @@ def dwt_traffic():
     return None
 
 
-def make_field_gpu(n: int, seed: int = 1, slope: float = -5.0 / 3.0):
-    """Seeded FFT synthesis of the GRF directly in HBM (torch on the GPU)."""
-    import torch
+def make_field_host(n: int):
+    """The headline field on the host: the seeded numpy synthesis the
+    reference's golden codestream was written from (bigwhoop_amd.fields.grf,
+    seed 1), with its committed input digest when one exists for n."""
+    import hashlib
 
-    g = torch.Generator(device="cuda")
-    g.manual_seed(seed)
-    white = torch.randn((n, n, n), generator=g, device="cuda", dtype=torch.float64)
-    spec = torch.fft.rfftn(white)
-    del white
-    f = torch.fft.fftfreq(n, d=1.0 / n, device="cuda", dtype=torch.float64)
-    fr = torch.fft.rfftfreq(n, d=1.0 / n, device="cuda", dtype=torch.float64)
-    k2 = f[:, None, None] ** 2 + f[None, :, None] ** 2 + fr[None, None, :] ** 2
-    kmag = torch.sqrt(k2)
-    del k2
-    filt = torch.where(kmag > 0, kmag.clamp_min(1e-30) ** ((slope - 2.0) / 2.0), torch.zeros_like(kmag))
-    del kmag
-    spec *= filt
-    del filt
-    out = torch.fft.irfftn(spec, s=(n, n, n))
-    del spec
-    out -= out.mean()
-    out *= 1.0 / out.std()
-    out += 10.0
-    torch.cuda.synchronize()
-    return out.contiguous()
+    from bigwhoop_amd.fields import grf
+
+    x = grf((n, n, n), slope=-5.0 / 3.0, seed=1)
+    golden = None
+    try:
+        for c in json.loads((ROOT / "tests" / "golden" / "cases.json").read_text()):
+            if c.get("large") and c["gen"]["shape"] == [n, n, n] and c["rate"] == SETTINGS["rate"]:
+                golden = c
+    except OSError:
+        pass
+    if golden is not None and hashlib.sha256(x.tobytes()).hexdigest() != golden["input_sha256"]:
+        print("bench: the field differs from the one the reference compressed", file=sys.stderr)
+        sys.exit(1)
+    return x, golden
 
 
 def dwt_algorithmic_bytes(n: int, levels: int) -> int:
@@ -155,11 +154,19 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    import hashlib
+
     import torch
-    from bigwhoop_amd import bwc
+    from bigwhoop_amd import bwc, skipnet
 
     n = args.size
-    field = make_field_gpu(n)
+    field_np, golden = make_field_host(n)
+    print(f"bench: field made ({n}^3 float64)", file=sys.stderr, flush=True)
+    # page-locked host copy (the headline input) and a device copy
+    host = torch.empty(n ** 3, dtype=torch.float64, pin_memory=True)
+    host.copy_(torch.from_numpy(field_np.reshape(-1)))
+    field = torch.from_numpy(field_np.reshape(-1)).to("cuda")
+    torch.cuda.synchronize()
     if args.copy_calibration:
         # one device copy of exactly n^3 * 8 bytes: the reference reading for
         # the FETCH_SIZE / WRITE_SIZE counters of the DWT kernels
@@ -167,11 +174,8 @@ def main():
         scratch.copy_(field)
         torch.cuda.synchronize()
         del scratch
-    from bigwhoop_amd import skipnet
 
-    ptr = field.data_ptr()
-
-    def run(codec, steps, src=ptr):
+    def run(codec, steps, src):
         """`steps` timed bwc_compress calls into the codec's own output buffer;
         wall time, the last codestream, the DWT stage ms of each call."""
         torch.cuda.synchronize()
@@ -183,53 +187,50 @@ def main():
         torch.cuda.synchronize()
         return time.perf_counter() - t0, codec.out[:size].tobytes(), dwt_ms
 
-    # plain encoder (every pass of every codeblock coded)
-    plain = bwc.Codec((n, n, n), pinned=True, **SETTINGS)
-    for _ in range(args.warmup):
-        plain.compress_into(ptr)
-    dt_plain, cs_plain, dwt_plain = run(plain, args.steps)
-    plain.close()
-    print(f"bench: plain encoder {dt_plain / args.steps * 1e3:.1f} ms/step", file=sys.stderr, flush=True)
-    # learned pass skip (the headline encoder); the codestream goes to a
-    # page-locked host buffer, as a caller that wants the DMA path allocates it
-    codec = bwc.Codec((n, n, n), pinned=True, **SETTINGS)
-    codec.set_skip_model(skipnet.default_weights())
-    codec.set_pass_skip(2, SKIP_MARGIN)
-    for _ in range(args.warmup):
-        codec.compress_into(ptr)
-    dt, cs, dwt_ms = run(codec, args.steps)
-    print(f"bench: learned pass skip {dt / args.steps * 1e3:.1f} ms/step", file=sys.stderr, flush=True)
+    def learned_codec():
+        c = bwc.Codec((n, n, n), pinned=True, **SETTINGS)
+        c.set_skip_model(skipnet.default_weights())
+        c.set_pass_skip(2, SKIP_MARGIN)
+        return c
+
+    def leg(name, codec, src):
+        for _ in range(args.warmup):
+            codec.compress_into(src)
+        dt, cs, dwt = run(codec, args.steps, src)
+        print(f"bench: {name} {dt / args.steps * 1e3:.1f} ms/step", file=sys.stderr, flush=True)
+        return dt, cs, dwt
+
+    # headline: learned pass skip, field in page-locked host memory
+    codec = learned_codec()
+    dt, cs, dwt_ms = leg("learned pass skip, page-locked host input", codec, host.data_ptr())
     ms_last, _ = codec.timings()
     st = codec.skip_stats()
     codec.close()
-    # the same into ordinary (pageable) host memory
-    pg = bwc.Codec((n, n, n), **SETTINGS)
-    pg.set_skip_model(skipnet.default_weights())
-    pg.set_pass_skip(2, SKIP_MARGIN)
-    for _ in range(args.warmup):
-        pg.compress_into(ptr)
-    dt_pg, cs_pg, _ = run(pg, args.steps)
+    # the same field in ordinary (pageable) host memory
+    pg = learned_codec()
+    dt_pg, cs_pg, _ = leg("learned pass skip, pageable host input", pg, field_np.ctypes.data)
     pg.close()
-    print(f"bench: pageable output {dt_pg / args.steps * 1e3:.1f} ms/step", file=sys.stderr, flush=True)
-    if cs_pg != cs:
-        print("pageable-output codestream differs", file=sys.stderr)
-        sys.exit(1)
-    # the field handed over in page-locked host memory: bwc_compress copies it
-    # to HBM over PCIe inside the call (not the headline value)
-    host = torch.empty(n ** 3, dtype=torch.float64, pin_memory=True)
-    host.copy_(field.reshape(-1))
-    hi = bwc.Codec((n, n, n), pinned=True, **SETTINGS)
-    hi.set_skip_model(skipnet.default_weights())
-    hi.set_pass_skip(2, SKIP_MARGIN)
-    for _ in range(args.warmup):
-        hi.compress_into(host.data_ptr())
-    dt_hi, cs_hi, _ = run(hi, args.steps, host.data_ptr())
-    hi.close()
+    # the same field resident in HBM (device pointer)
+    dv = learned_codec()
+    dt_dv, cs_dv, dwt_dv = leg("learned pass skip, device input", dv, field.data_ptr())
+    ms_dev, _ = dv.timings()
+    dv.close()
+    # plain encoder (every pass of every codeblock coded), page-locked host input
+    plain = bwc.Codec((n, n, n), pinned=True, **SETTINGS)
+    dt_plain, cs_plain, dwt_plain = leg("plain encoder, page-locked host input", plain, host.data_ptr())
+    plain.close()
+    for name, other in (("pageable-input", cs_pg), ("device-input", cs_dv), ("plain-encoder", cs_plain)):
+        if other != cs:
+            print(f"bench: {name} codestream differs from the headline one", file=sys.stderr)
+            sys.exit(1)
+    same_as_ref = None
+    if golden is not None:
+        same_as_ref = (len(cs) == golden["codestream_size"]
+                       and hashlib.sha256(cs).hexdigest() == golden["codestream_sha256"])
+        if not same_as_ref:
+            print("bench: codestream differs from the reference's", file=sys.stderr)
+            sys.exit(1)
     del host
-    print(f"bench: host input {dt_hi / args.steps * 1e3:.1f} ms/step", file=sys.stderr, flush=True)
-    if cs_hi != cs:
-        print("host-input codestream differs", file=sys.stderr)
-        sys.exit(1)
     # decompression of the headline codestream into device memory
     dec = torch.empty(n ** 3, dtype=torch.float64, device="cuda")
     for _ in range(args.warmup):
@@ -241,21 +242,23 @@ def main():
     torch.cuda.synchronize()
     dt_dec = time.perf_counter() - t0
     print(f"bench: decompress {dt_dec / args.steps * 1e3:.1f} ms/step", file=sys.stderr, flush=True)
-    err = (dec - field.reshape(-1)).double()
+    dec_ok = None
+    if golden is not None:
+        dec_ok = hashlib.sha256(dec.cpu().numpy().tobytes()).hexdigest() == golden["decode_sha256"]
+    err = (dec - field).double()
     psnr_db = 20 * math.log10(float(field.max() - field.min()) / (2 * math.sqrt(float((err * err).mean()))))
     del dec, err
-    if cs != cs_plain:
-        print("learned pass skip and plain encoder codestreams differ", file=sys.stderr)
-        sys.exit(1)
 
     bytes_in = n ** 3 * 8
     value = bytes_in * args.steps / dt / 1e9
     plain_gbs = bytes_in * args.steps / dt_plain / 1e9
     alg = dwt_algorithmic_bytes(n, 5)
-    dwt_all = dwt_ms + dwt_plain
+    dwt_all = dwt_ms + dwt_dv + dwt_plain
     dwt_avg = sum(dwt_all) / len(dwt_all)
     achieved = alg / (dwt_avg / 1e3) / 1e9
     cpu = None if args.no_cpu_baseline else cpu_baseline()
+    stage_names = ["fill_minmax", "normalize", "dwt", "quantize", "tier1", "records_d2h", "rate_control",
+                   "headers_gather", "assembly_d2h", "total"]
     line = {
         "metric": "compress GB/s at 2 bpv, bit-exact codestream; learned-skip vs plain GPU",
         "value": round(value, 4),
@@ -270,16 +273,17 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"{n}^3 float64 GRF k^-5/3, 5 levels, cb 32x32x8, 1 layer at 2 bpv",
+                   "input": "page-locked host memory, copied to HBM inside bwc_compress (PCIe)",
                    "encoder": f"learned pass skip (margin {SKIP_MARGIN} passes)", "codestream_bytes": len(cs),
-                   "codestream_identical_to_plain": True},
-        "pageable_output": {"value": round(bytes_in * args.steps / dt_pg / 1e9, 4), "unit": "GB/s",
-                            "ms_per_step": round(dt_pg / args.steps * 1e3, 3)},
-        "host_input": {"value": round(bytes_in * args.steps / dt_hi / 1e9, 4), "unit": "GB/s",
-                       "ms_per_step": round(dt_hi / args.steps * 1e3, 3),
-                       "input": "page-locked host memory, copied to HBM inside bwc_compress (PCIe)"},
+                   "codestream_identical_to_plain": True, "codestream_identical_to_reference": same_as_ref},
+        "pageable_input": {"value": round(bytes_in * args.steps / dt_pg / 1e9, 4), "unit": "GB/s",
+                           "ms_per_step": round(dt_pg / args.steps * 1e3, 3)},
+        "device_input": {"value": round(bytes_in * args.steps / dt_dv / 1e9, 4), "unit": "GB/s",
+                         "ms_per_step": round(dt_dv / args.steps * 1e3, 3),
+                         "stages_ms": {k: round(v, 3) for k, v in zip(stage_names, ms_dev)}},
         "decompress": {"value": round(bytes_in * args.steps / dt_dec / 1e9, 4), "unit": "GB/s",
                        "ms_per_step": round(dt_dec / args.steps * 1e3, 3), "psnr_db": round(psnr_db, 3),
-                       "output": "device memory"},
+                       "output": "device memory", "identical_to_reference": dec_ok},
         "plain_encoder": {"value": round(plain_gbs, 4), "unit": "GB/s",
                           "ms_per_step": round(dt_plain / args.steps * 1e3, 3),
                           "speedup_learned_vs_plain": round(dt_plain / dt, 3)},
@@ -290,9 +294,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": dwt_traffic(),
                      "kernel": "DWT stage (15 axis passes, normalisation fused)", "dwt_ms": round(dwt_avg, 4),
                      "algorithmic_bytes": alg},
-        "stages_ms": {k: round(v, 3) for k, v in zip(
-            ["fill_minmax", "normalize", "dwt", "quantize", "tier1", "records_d2h", "rate_control", "headers_gather",
-             "assembly_d2h", "total"], ms_last)},
+        "stages_ms": {k: round(v, 3) for k, v in zip(stage_names, ms_last)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -242,12 +242,15 @@ class Codec:
     """One configured compression job: owns the bwc_codec and its stream.
 
     ``shape`` is (nX, nY, nZ, nTS) with x fastest; settings follow the C
-    setters and are applied in the order decomp, precincts, codeblocks, qm,
-    tiles, error resilience.
+    setters and are applied in the order kernels (set_kernels), progression
+    (set_progression), quantisation style (set_quant_style), decomp,
+    precincts, codeblocks, qm, tiles, error resilience, quantisation step
+    size (set_quant_step_size), then the stream's AUX and COM segments.
     """
 
     def __init__(self, shape, npar=1, prec=8, rate="2", decomp=None, precincts=None, cb=None, qm=None,
-                 tiles=None, erres=False, fresh_gain_table=True, pinned=False):
+                 tiles=None, erres=False, kernels=None, prog=None, qstyle=None, qstep=None, com=None, aux=None,
+                 fresh_gain_table=True, pinned=False):
         """``pinned``: the codestream buffer is page-locked host memory
         (bwc_amd_host_alloc), so packet bodies reach it by DMA."""
         L = lib()
@@ -261,6 +264,12 @@ class Codec:
         self.h = L.bwc_alloc_coder(n[0], n[1], n[2], n[3], npar, prec)
         if not self.h:
             raise RuntimeError("bwc_alloc_coder failed")
+        if kernels is not None:
+            L.set_kernels(self.h, *[int(x) for x in kernels])
+        if prog is not None:
+            L.set_progression(self.h, int(prog))
+        if qstyle is not None:
+            L.set_quant_style(self.h, int(qstyle))
         if decomp is not None:
             L.bwc_set_decomp(self.h, *[int(x) for x in decomp])
         if precincts is not None:
@@ -273,6 +282,8 @@ class Codec:
             L.bwc_set_tiles(self.h, *[int(x) for x in tiles], 0)
         if erres:
             L.bwc_set_error_resilience(self.h)
+        if qstep is not None:
+            L.set_quant_step_size(self.h, float(qstep))
         nout = self.nbytes * 2 + (1 << 20)
         self._pinned = None
         if pinned:
@@ -283,6 +294,11 @@ class Codec:
         else:
             self.out = np.zeros(nout, np.uint8)
         self.stream = L.bwc_init_stream(None, self.out.ctypes.data_as(C.c_void_p), 0)
+        for text, setter in ((aux, L.bwc_set_aux), (com, L.bwc_set_com)):
+            if text is not None:
+                b = text.encode() if isinstance(text, str) else bytes(text)
+                if setter(self.stream, b, len(b)):
+                    raise RuntimeError("bwc_set_aux / bwc_set_com failed")
         self._rate = C.create_string_buffer(rate.encode())
         if L.bwc_create_compression(self.h, self.stream, self._rate):
             raise RuntimeError("bwc_create_compression failed: " + last_error())
@@ -292,6 +308,12 @@ class Codec:
             lib().bwc_free_codec(self.h)
             self.h = None
         if getattr(self, "stream", None):
+            # the library never frees AUX / COM spans (the caller owns them)
+            st = _Stream.from_address(self.stream)
+            for span in (st.aux, st.com):
+                if span:
+                    C.CDLL(None).free(C.c_void_p(C.c_void_p.from_address(span + 24).value))
+                    C.CDLL(None).free(C.c_void_p(span))
             C.CDLL(None).free(C.c_void_p(self.stream))
             self.stream = None
         if getattr(self, "_pinned", None):

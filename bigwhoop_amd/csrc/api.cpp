@@ -681,7 +681,13 @@ uchar bwc_decompress(bwc_codec *const codec, bwc_stream *const stream) {
       if (gpu::decode_output(ws, codec, tl, p, par, ws->d_work, dst)) goto fail;
     }
   }
-  if (!on_dev && gpu::copy_d2h(ws, stream->out, dst, bytes)) goto fail;
+  // host output: only the samples a tile wrote cross back (the others keep
+  // the caller's contents, as the reference leaves them)
+  if (!on_dev)
+    for (uint64_t ti = 0; ti < job->plan.tiles.size(); ++ti)
+      for (int p = 0; p < inf.nPar; ++p)
+        if (gpu::output_box_d2h(ws, codec, job->plan.tiles[ti], p, job->plan.tiles[ti].par[p], dst, stream->out))
+          goto fail;
   if (gpu::sync(ws)) goto fail;
   return 0;
 fail:
@@ -749,12 +755,22 @@ int bwc_amd_compress_batch(bwc_codec *const *codecs, bwc_stream *const *streams,
     set_err("bwc_amd_compress_batch: null argument");
     return 1;
   }
-  {  // one codec's workspace and records serve one compression at a time
-    std::vector<bwc_codec *> sorted(codecs, codecs + n);
-    std::sort(sorted.begin(), sorted.end());
-    if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end()) {
-      set_err("bwc_amd_compress_batch: a codec appears twice");
-      return 1;
+  {  // one codec's workspace and records serve one compression at a time;
+     // two workers must not write one stream or one output buffer
+    std::vector<const void *> seen[3];
+    for (int i = 0; i < n; ++i) {
+      seen[0].push_back(codecs[i]);
+      seen[1].push_back(streams[i]);
+      if (streams[i]) seen[2].push_back(streams[i]->out);
+    }
+    static const char *what[3] = {"a codec", "a stream", "an output buffer"};
+    for (int k = 0; k < 3; ++k) {
+      std::sort(seen[k].begin(), seen[k].end());
+      if (std::adjacent_find(seen[k].begin(), seen[k].end()) != seen[k].end()) {
+        std::snprintf(g_api_err, sizeof g_api_err, "bwc_amd_compress_batch: %s appears twice", what[k]);
+        std::fprintf(stderr, "bwc: error: %s\n", g_api_err);
+        return 1;
+      }
     }
   }
   // workspaces (device state, streams) made here, on the calling thread

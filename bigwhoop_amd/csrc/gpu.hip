@@ -25,6 +25,7 @@ namespace gpu {
 
 static thread_local char g_err[256];
 const char *last_error() { return g_err; }
+static int process_setup();
 
 #define HIP_OK(x)                                                                                      \
   do {                                                                                                 \
@@ -66,6 +67,10 @@ Workspace::~Workspace() {
   if (h_down) (void)hipHostFree(h_down);
   for (auto &e : ev)
     if (e) (void)hipEventDestroy(e);
+  if (copy_stream) (void)hipStreamSynchronize(copy_stream);
+  for (auto &e : ev_copy) (void)hipEventDestroy(e);
+  if (ev_fill) (void)hipEventDestroy(ev_fill);
+  if (copy_stream) (void)hipStreamDestroy(copy_stream);
   if (stream) (void)hipStreamSynchronize(stream);
   for (auto &g : dwt_graphs)
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
@@ -116,16 +121,23 @@ static int up_copy(Workspace *ws, void *dst, const void *src, size_t n) {
 // ===========================================================================
 // fill + min/max, normalize
 // ===========================================================================
-__global__ void fill_minmax_kernel(const void *__restrict__ in, int prec, uint64_t nX, uint64_t nY, uint64_t nZ,
-                                   uint64_t poff, uint64_t x0, uint64_t y0, uint64_t z0, uint64_t t0, uint64_t w,
-                                   uint64_t h, uint64_t d, uint64_t n, double *__restrict__ out,
+// Copy to f64 + per-block min/max over the tile-local indices [i0, i1) of one
+// (tile, param). The source is the caller's field (strides sX, sY, sZ, tile
+// offsets x0, y0, z0) or a tile-shaped staging copy of it (strides w, h, d,
+// zero offsets). The time index is not offset by the tile's start, as in the
+// reference's fill_buffer (libbwc.c:727-729: nZ * t). Partials go to
+// red[2 * blockIdx.x], red[2 * blockIdx.x + 1].
+__global__ void fill_minmax_kernel(const void *__restrict__ in, int prec, uint64_t sX, uint64_t sY, uint64_t sZ,
+                                   uint64_t poff, uint64_t x0, uint64_t y0, uint64_t z0, uint64_t w, uint64_t h,
+                                   uint64_t d, uint64_t i0, uint64_t i1, double *__restrict__ out,
                                    double *__restrict__ red) {
   double mx = -1.7976931348623157e+308, mn = 1.7976931348623157e+308;
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+  for (uint64_t i = i0 + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < i1;
+       i += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t x = i % w, r = i / w;
     const uint64_t y = r % h, r2 = r / h;
     const uint64_t z = r2 % d, t = r2 / d;
-    const uint64_t src = poff + x0 + x + nX * (y0 + y + nY * (z0 + z + nZ * (t0 + t)));
+    const uint64_t src = poff + x0 + x + sX * (y0 + y + sY * (z0 + z + sZ * t));
     const double v = prec == 8 ? ((const double *)in)[src] : (double)((const float *)in)[src];
     out[i] = v;
     mx = (mx < v) ? v : mx;
@@ -179,6 +191,15 @@ __global__ void minmax_final_kernel(const double *red, int nblk, double *norm) {
 __device__ __forceinline__ double norm_value(double x, double alpha, double beta, uint64_t shift) {
   const double v = (x - alpha) * beta;
   uint64_t raw = __double_as_longlong(v);
+  // A NaN here has the bits x86-64 SSE arithmetic gives it: a NaN sample
+  // propagates quieted, an invalid operation (0 * inf for a constant field,
+  // whose beta is 2/0) gives the default NaN with its sign bit set. The
+  // exponent shift below turns those bits into a finite value that the
+  // transform then carries.
+  if (v != v) {
+    const uint64_t xr = (uint64_t)__double_as_longlong(x);
+    raw = (x != x) ? (xr | 0x0008000000000000ull) : 0xFFF8000000000000ull;
+  }
   raw += ((raw & 0x7FF0000000000000ull) == 0 && (raw & 0x000FFFFFFFFFFFFFull) == 0) ? 0 : shift;
   return __longlong_as_double((long long)raw);
 }
@@ -190,6 +211,12 @@ __global__ void normalize_kernel(double *__restrict__ data, uint64_t n, const do
     data[i] = norm_value(data[i], alpha, beta, shift);
 }
 
+// partial min/max slots of one fill (all chunks together)
+constexpr int kRedSlots = 8192;
+// bytes per staged chunk of a host field: copies of later chunks overlap the
+// min/max pass over earlier ones
+constexpr uint64_t kFillChunkBytes = 64ull << 20;
+
 int fill_minmax(Workspace *ws, const bwc_codec *c, const Tile &tl, int param, const void *input, bool on_dev,
                 double *work) {
   const Param &par = tl.par[param];
@@ -197,40 +224,107 @@ int fill_minmax(Workspace *ws, const bwc_codec *c, const Tile &tl, int param, co
   const uint64_t nX = c->info.nX, nY = c->info.nY, nZ = c->info.nZ, nTS = c->info.nTS;
   const int prec = c->info.data_prec == bwc_precision_single ? 4 : 8;
   const uint64_t poff = nX * nY * nZ * nTS * (uint64_t)param;
-  const void *src = input;
-  uint64_t srcoff = poff;
-  uint64_t sx = nX, sy = nY, sz = nZ;
-  uint64_t ox = tl.box.lo[0], oy = tl.box.lo[1], oz = tl.box.lo[2], ot = tl.box.lo[3];
-  if (!on_dev) {
-    // copy just this tile's parameter slab (whole rows of the global field)
-    const uint64_t slab = nX * nY * nZ * nTS;
-    CHK(ensure(&ws->d_input, &ws->input_cap, slab * prec));
-    HIP_OK(hipMemcpyAsync(ws->d_input, (const char *)input + poff * prec, slab * prec, hipMemcpyHostToDevice,
-                          ws->stream));
-    src = ws->d_input;
-    srcoff = 0;
-  }
-  (void)sx, (void)sy, (void)sz;
-  const int threads = 256;
-  const int blocks = (int)std::min<uint64_t>((n + threads - 1) / threads, 4096);
-  size_t redcap = 0;
-  if (!ws->d_red) HIP_OK(hipMalloc((void **)&ws->d_red, 2 * 4096 * sizeof(double)));
+  const uint64_t w = par.box.len(0), h = par.box.len(1), d = par.box.len(2), dt = par.box.len(3);
+  const uint64_t x0 = tl.box.lo[0], y0 = tl.box.lo[1], z0 = tl.box.lo[2];
+  if (!ws->d_red) HIP_OK(hipMalloc((void **)&ws->d_red, 2 * kRedSlots * sizeof(double)));
   if (!ws->d_norm) HIP_OK(hipMalloc((void **)&ws->d_norm, 4 * sizeof(double)));
-  (void)redcap;
-  hipLaunchKernelGGL(fill_minmax_kernel, dim3(blocks), dim3(threads), 0, ws->stream, src, prec, nX, nY, nZ, srcoff, ox,
-                     oy, oz, ot, par.box.len(0), par.box.len(1), par.box.len(2), n, work, ws->d_red);
-  hipLaunchKernelGGL(minmax_final_kernel, dim3(1), dim3(256), 0, ws->stream, ws->d_red, blocks, ws->d_norm);
+  const int threads = 256;
+  int slots = 0;
+  if (on_dev || !n) {
+    const int blocks = (int)std::max<uint64_t>(1, std::min<uint64_t>((n + threads - 1) / threads, 4096));
+    hipLaunchKernelGGL(fill_minmax_kernel, dim3(blocks), dim3(threads), 0, ws->stream, input, prec, nX, nY, nZ, poff,
+                       x0, y0, z0, w, h, d, (uint64_t)0, n, work, ws->d_red);
+    HIP_OK(hipGetLastError());
+    slots = blocks;
+  } else {
+    // Host field: only this tile's box crosses PCIe, into a tile-shaped
+    // staging buffer, in chunks on a copy stream; the min/max pass over a
+    // chunk starts as soon as it has landed. Contiguous source runs: whole
+    // (z, t) volumes when the tile spans full X and Y rows and planes, whole
+    // planes when it spans full rows, else rows (one 3-D copy per time step).
+    CHK(ensure(&ws->d_input, &ws->input_cap, n * prec));
+    if (!ws->copy_stream) HIP_OK(hipStreamCreateWithFlags(&ws->copy_stream, hipStreamNonBlocking));
+    if (!ws->ev_fill) HIP_OK(hipEventCreateWithFlags(&ws->ev_fill, hipEventDisableTiming));
+    // the staging buffer may still be read by the previous fill
+    HIP_OK(hipEventRecord(ws->ev_fill, ws->stream));
+    HIP_OK(hipStreamWaitEvent(ws->copy_stream, ws->ev_fill, 0));
+    const char *src = static_cast<const char *>(input);
+    char *dst = static_cast<char *>(ws->d_input);
+    // (tile-local start, length) of each chunk, in values
+    std::vector<std::pair<uint64_t, uint64_t>> chunks;
+    const uint64_t cvals = std::max<uint64_t>(1, kFillChunkBytes / prec);
+    if (w == nX && h == nY) {
+      // per time step one contiguous range of w*h*d values (all of them
+      // contiguous when d == nZ)
+      const uint64_t run = w * h * d;
+      const bool whole = d == nZ;
+      for (uint64_t t = 0; t < (whole ? 1 : dt); ++t) {
+        const uint64_t len = whole ? n : run;
+        const uint64_t sbase = poff + nX * nY * (z0 + nZ * t);
+        for (uint64_t o = 0; o < len; o += cvals) {
+          const uint64_t m = std::min(cvals, len - o);
+          HIP_OK(hipMemcpyAsync(dst + (t * run + o) * prec, src + (sbase + o) * prec, m * prec,
+                                hipMemcpyHostToDevice, ws->copy_stream));
+          chunks.push_back({t * run + o, m});
+          if (ws->ev_copy.size() < chunks.size()) {
+            hipEvent_t e;
+            HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            ws->ev_copy.push_back(e);
+          }
+          HIP_OK(hipEventRecord(ws->ev_copy[chunks.size() - 1], ws->copy_stream));
+        }
+      }
+    } else {
+      // one strided 3-D copy per time step: rows of w values at pitch nX,
+      // h rows per plane, d planes
+      for (uint64_t t = 0; t < dt; ++t) {
+        hipMemcpy3DParms cp = {};
+        cp.srcPtr = make_hipPitchedPtr(const_cast<char *>(src) + (poff + nX * nY * nZ * t) * prec, nX * prec, nX, nY);
+        cp.srcPos = make_hipPos(x0 * prec, y0, z0);
+        cp.dstPtr = make_hipPitchedPtr(dst + w * h * d * t * prec, w * prec, w, h);
+        cp.dstPos = make_hipPos(0, 0, 0);
+        cp.extent = make_hipExtent(w * prec, h, d);
+        cp.kind = hipMemcpyHostToDevice;
+        HIP_OK(hipMemcpy3DAsync(&cp, ws->copy_stream));
+        chunks.push_back({w * h * d * t, w * h * d});
+        if (ws->ev_copy.size() < chunks.size()) {
+          hipEvent_t e;
+          HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+          ws->ev_copy.push_back(e);
+        }
+        HIP_OK(hipEventRecord(ws->ev_copy[chunks.size() - 1], ws->copy_stream));
+      }
+    }
+    const int per = (int)std::max<size_t>(1, std::min<size_t>(512, kRedSlots / std::max<size_t>(1, chunks.size())));
+    if ((size_t)per * chunks.size() > (size_t)kRedSlots) {
+      std::snprintf(g_err, sizeof g_err, "field staged in too many chunks (%zu)", chunks.size());
+      return 1;
+    }
+    for (size_t k = 0; k < chunks.size(); ++k) {
+      HIP_OK(hipStreamWaitEvent(ws->stream, ws->ev_copy[k], 0));
+      const int blocks = (int)std::max<uint64_t>(1, std::min<uint64_t>((chunks[k].second + threads - 1) / threads, per));
+      hipLaunchKernelGGL(fill_minmax_kernel, dim3(blocks), dim3(threads), 0, ws->stream, ws->d_input, prec, w, h, d,
+                         (uint64_t)0, (uint64_t)0, (uint64_t)0, (uint64_t)0, w, h, d, chunks[k].first,
+                         chunks[k].first + chunks[k].second, work, ws->d_red + 2 * slots);
+      HIP_OK(hipGetLastError());
+      slots += blocks;
+    }
+  }
+  hipLaunchKernelGGL(minmax_final_kernel, dim3(1), dim3(256), 0, ws->stream, ws->d_red, slots, ws->d_norm);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+static int normalize_on(hipStream_t stream, double *work, uint64_t n, const double *norm, uint8_t Qm) {
+  const int threads = 256;
+  const int blocks = (int)std::max<uint64_t>(1, std::min<uint64_t>((n + threads - 1) / threads, 8192));
+  hipLaunchKernelGGL(normalize_kernel, dim3(blocks), dim3(threads), 0, stream, work, n, norm, (uint64_t)Qm << 52);
   HIP_OK(hipGetLastError());
   return 0;
 }
 
 int normalize(Workspace *ws, double *work, uint64_t n, uint8_t Qm) {
-  const int threads = 256;
-  const int blocks = (int)std::min<uint64_t>((n + threads - 1) / threads, 8192);
-  hipLaunchKernelGGL(normalize_kernel, dim3(blocks), dim3(threads), 0, ws->stream, work, n, ws->d_norm,
-                     (uint64_t)Qm << 52);
-  HIP_OK(hipGetLastError());
-  return 0;
+  return normalize_on(ws->stream, work, n, ws->d_norm, Qm);
 }
 
 // ===========================================================================
@@ -558,7 +652,20 @@ __global__ void box_copy_kernel(const double *__restrict__ src, double *__restri
   }
 }
 
-static int dwt_run(Workspace *ws, const bwc_codec *c, const Param &par, double *work, bool fwd, int norm_qm) {
+// Where one DWT call runs: its stream, the second buffer of the out-of-place
+// passes (grown from ws when the call may allocate, else of fixed capacity),
+// the {min, max, alpha, beta} the fused normalisation reads, and the number
+// of kernels it launched.
+struct DwtCtx {
+  hipStream_t stream = nullptr;
+  double *work2 = nullptr;
+  size_t work2_cap = 0;     // bytes
+  const double *norm = nullptr;
+  Workspace *grow = nullptr;
+  int launches = 0;
+};
+
+static int dwt_passes(DwtCtx &x, const bwc_codec *c, const Param &par, double *work, bool fwd, int norm_qm) {
   const bwc_gl_ctrl &ctl = c->control;
   // normalisation still to apply (forward only): fused into the first X pass
   // when that pass runs on the fast kernel, else a separate in-place pass
@@ -568,7 +675,7 @@ static int dwt_run(Workspace *ws, const bwc_codec *c, const Param &par, double *
   const uint8_t dec[4] = {ctl.decompX, ctl.decompY, ctl.decompZ, ctl.decompTS};
   const int kern[4] = {ctl.KernelX, ctl.KernelY, ctl.KernelZ, ctl.KernelTS};
   const int nd = ctl.nDecomp;
-  ws->last_dwt_launches = 0;
+  x.launches = 0;
   // BWC_AMD_DWT=0 keeps every pass on the generic in-place kernel
   const char *dwt_env = std::getenv("BWC_AMD_DWT");
   const bool generic_only = dwt_env && dwt_env[0] == '0';
@@ -594,15 +701,22 @@ static int dwt_run(Workspace *ws, const bwc_codec *c, const Param &par, double *
     // is evened out by running the X pass out of place, else by a copy back
     const int nstrided = (int)streamed(0) + (int)streamed(1) + (int)streamed(2) + (int)streamed(3);
     const bool x_oop = (nstrided & 1) && x_lds(0);
-    double *cur = work, *oth = ws->d_work2;
-    if (nstrided) {
-      size_t cap = ws->work2_cap;
-      CHK(ensure((void **)&ws->d_work2, &cap, par.box.volume() * sizeof(double)));
-      ws->work2_cap = cap;
-      oth = ws->d_work2;
+    double *cur = work, *oth = x.work2;
+    if (nstrided && x.work2_cap < par.box.volume() * sizeof(double)) {
+      if (!x.grow) {
+        std::snprintf(g_err, sizeof g_err, "DWT second buffer of %zu bytes is smaller than the %llu the box needs",
+                      x.work2_cap, (unsigned long long)(par.box.volume() * sizeof(double)));
+        return 1;
+      }
+      size_t cap = x.grow->work2_cap;
+      CHK(ensure((void **)&x.grow->d_work2, &cap, par.box.volume() * sizeof(double)));
+      x.grow->work2_cap = cap;
+      x.work2 = x.grow->d_work2;
+      x.work2_cap = cap;
+      oth = x.work2;
     }
     if (norm_pending && !x_lds(0)) {
-      CHK(normalize(ws, work, par.box.volume(), (uint8_t)norm_qm));
+      CHK(normalize_on(x.stream, work, par.box.volume(), x.norm, (uint8_t)norm_qm));
       norm_pending = false;
     }
     for (int oi = 0; oi < 4; ++oi) {
@@ -639,13 +753,13 @@ static int dwt_run(Workspace *ws, const bwc_codec *c, const Param &par, double *
       if (x_lds(a)) {
         double *dst = x_oop ? oth : cur;
         hipLaunchKernelGGL(dwt97_x_kernel, dim3((uint32_t)((sz[o[0]] + 3) / 4), (uint32_t)gy), dim3(256), xlds,
-                           ws->stream, cur, dst, (int)len, par0, half, (uint32_t)sz[o[0]], inc[o[0]],
-                           (uint32_t)sz[o[1]], inc[o[1]], inc[o[2]], norm_pending ? ws->d_norm : nullptr,
+                           x.stream, cur, dst, (int)len, par0, half, (uint32_t)sz[o[0]], inc[o[0]],
+                           (uint32_t)sz[o[1]], inc[o[1]], inc[o[2]], norm_pending ? x.norm : nullptr,
                            (uint64_t)(norm_pending ? norm_qm : 0) << 52);
         HIP_OK(hipGetLastError());
         norm_pending = false;
         if (x_oop) std::swap(cur, oth);
-        ++ws->last_dwt_launches;
+        ++x.launches;
         continue;
       }
       if (streamed(a)) {
@@ -659,7 +773,7 @@ static int dwt_run(Workspace *ws, const bwc_codec *c, const Param &par, double *
         const uint64_t nthreads = nlines * (uint64_t)nseg;
         const uint32_t nblk = (uint32_t)((nthreads + 255) / 256);
 #define STREAM_LAUNCH(UU)                                                                                      \
-  hipLaunchKernelGGL(dwt97_stream_kernel<UU>, dim3(nblk), dim3(256), 0, ws->stream, cur, oth, (int)len, par0, \
+  hipLaunchKernelGGL(dwt97_stream_kernel<UU>, dim3(nblk), dim3(256), 0, x.stream, cur, oth, (int)len, par0, \
                      half, inc[a], (uint32_t)sz[o[0]], inc[o[0]], (uint32_t)sz[o[1]], inc[o[1]], inc[o[2]],    \
                      (uint32_t)nlines, nseg, ksz)
         if (tune.unroll == 2) STREAM_LAUNCH(2);
@@ -668,7 +782,7 @@ static int dwt_run(Workspace *ws, const bwc_codec *c, const Param &par, double *
 #undef STREAM_LAUNCH
         HIP_OK(hipGetLastError());
         std::swap(cur, oth);
-        ++ws->last_dwt_launches;
+        ++x.launches;
         continue;
       }
       if (lds > 160 * 1024) {
@@ -676,7 +790,7 @@ static int dwt_run(Workspace *ws, const bwc_codec *c, const Param &par, double *
         return 1;
       }
       auto launch = [&](auto kfn) {
-        hipLaunchKernelGGL(kfn, dim3((uint32_t)gx, (uint32_t)gy), dim3(threads), lds, ws->stream, cur, len, par0,
+        hipLaunchKernelGGL(kfn, dim3((uint32_t)gx, (uint32_t)gy), dim3(threads), lds, x.stream, cur, len, par0,
                            inc[a], sz[o[0]], inc[o[0]], sz[o[1]], inc[o[1]], inc[o[2]], G, contiguous ? 1 : 0);
       };
       if (kern[a] == bwc_dwt_9_7) {
@@ -687,20 +801,34 @@ static int dwt_run(Workspace *ws, const bwc_codec *c, const Param &par, double *
         else launch(dwt_axis_kernel<1, false>);
       }
       HIP_OK(hipGetLastError());
-      ++ws->last_dwt_launches;
+      ++x.launches;
     }
     if (cur != work) {
       const uint64_t total = sz[0] * sz[1] * sz[2] * sz[3];
       hipLaunchKernelGGL(box_copy_kernel, dim3((uint32_t)std::min<uint64_t>((total + 255) / 256, 16384)), dim3(256),
-                         0, ws->stream, cur, work, (uint32_t)sz[0], (uint32_t)sz[1], (uint32_t)sz[2], inc[1], inc[2],
+                         0, x.stream, cur, work, (uint32_t)sz[0], (uint32_t)sz[1], (uint32_t)sz[2], inc[1], inc[2],
                          inc[3], total);
       HIP_OK(hipGetLastError());
-      ++ws->last_dwt_launches;
+      ++x.launches;
     }
   }
-  if (norm_pending) CHK(normalize(ws, work, par.box.volume(), (uint8_t)norm_qm));
+  if (norm_pending) CHK(normalize_on(x.stream, work, par.box.volume(), x.norm, (uint8_t)norm_qm));
   return 0;
 }
+
+static int dwt_run(Workspace *ws, const bwc_codec *c, const Param &par, double *work, bool fwd, int norm_qm) {
+  DwtCtx x;
+  x.stream = ws->stream;
+  x.work2 = ws->d_work2;
+  x.work2_cap = ws->work2_cap;
+  x.norm = ws->d_norm;
+  x.grow = ws;
+  const int rc = dwt_passes(x, c, par, work, fwd, norm_qm);
+  ws->last_dwt_launches = x.launches;
+  return rc;
+}
+
+
 
 // Everything the forward DWT's launch sequence depends on: the parameter's
 // geometry, the buffers, the normalisation and the launch knobs.
@@ -772,6 +900,7 @@ int dwt_forward(Workspace *ws, const bwc_codec *c, const Param &par, double *wor
 int dwt_inverse(Workspace *ws, const bwc_codec *c, const Param &par, double *work) {
   return dwt_run(ws, c, par, work, false, -1);
 }
+
 
 // ===========================================================================
 // quantisation into stripe columns (tier1.c:526-729)
@@ -2091,8 +2220,47 @@ int decode_output(Workspace *ws, const bwc_codec *c, const Tile &tl, int p, cons
   const uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 65536);
   hipLaunchKernelGGL(denorm_flush_kernel, dim3((uint32_t)blocks), dim3(256), 0, ws->stream, work, par.box.len(0),
                      par.box.len(1), par.box.len(2), n, alpha, beta, (uint64_t)c->control.Qm << 52, inf.nX, inf.nY,
-                     inf.nZ, tl.box.lo[0], tl.box.lo[1], tl.box.lo[2], tl.box.lo[3], poff, (int)inf.data_prec, out);
+                     inf.nZ, tl.box.lo[0], tl.box.lo[1], tl.box.lo[2], poff, (int)inf.data_prec, out);
   HIP_OK(hipGetLastError());
+  return 0;
+}
+
+// Copy the samples the decoder wrote for (tile, param) from the device
+// staging buffer `stage` (the whole field's layout) to the caller's host
+// buffer; samples no tile writes keep what the caller had there, as with the
+// reference's flush_buffer (libbwc.c:827-978). The time range starts at 0, as
+// flush_buffer indexes it.
+int output_box_d2h(Workspace *ws, const bwc_codec *c, const Tile &tl, int p, const Param &par, const void *stage,
+                   void *out) {
+  const bwc_gl_inf &inf = c->info;
+  const uint64_t nX = inf.nX, nY = inf.nY, nZ = inf.nZ;
+  const uint64_t prec = (uint64_t)inf.data_prec;
+  const uint64_t w = par.box.len(0), h = par.box.len(1), d = par.box.len(2), dt = par.box.len(3);
+  if (!w || !h || !d || !dt) return 0;
+  const uint64_t poff = nX * nY * nZ * inf.nTS * (uint64_t)p;
+  const uint64_t x0 = tl.box.lo[0], y0 = tl.box.lo[1], z0 = tl.box.lo[2];
+  const char *src = static_cast<const char *>(stage);
+  char *dst = static_cast<char *>(out);
+  if (w == nX && h == nY) {
+    const bool whole = d == nZ;
+    for (uint64_t t = 0; t < (whole ? 1 : dt); ++t) {
+      const uint64_t o = (poff + nX * nY * (z0 + nZ * t)) * prec;
+      HIP_OK(hipMemcpyAsync(dst + o, src + o, (whole ? w * h * d * dt : w * h * d) * prec, hipMemcpyDeviceToHost,
+                            ws->stream));
+    }
+    return 0;
+  }
+  for (uint64_t t = 0; t < dt; ++t) {
+    const uint64_t base = (poff + nX * nY * nZ * t) * prec;
+    hipMemcpy3DParms cp = {};
+    cp.srcPtr = make_hipPitchedPtr(const_cast<char *>(src) + base, nX * prec, nX, nY);
+    cp.srcPos = make_hipPos(x0 * prec, y0, z0);
+    cp.dstPtr = make_hipPitchedPtr(dst + base, nX * prec, nX, nY);
+    cp.dstPos = make_hipPos(x0 * prec, y0, z0);
+    cp.extent = make_hipExtent(w * prec, h, d);
+    cp.kind = hipMemcpyDeviceToHost;
+    HIP_OK(hipMemcpy3DAsync(&cp, ws->stream));
+  }
   return 0;
 }
 
@@ -2180,8 +2348,12 @@ int init_device(Workspace *ws) {
   ws->device = dev;
   HIP_OK(hipStreamCreateWithFlags(&ws->stream, hipStreamNonBlocking));
   for (auto &e : ws->ev) HIP_OK(hipEventCreate(&e));
-  // constant tables and kernel attributes, once per process (workspaces may
-  // be created from several host threads)
+  return process_setup();
+}
+
+// constant tables and kernel attributes, once per process (workspaces may be
+// created from several host threads)
+static int process_setup() {
   static std::mutex m;
   static bool tables = false;
   std::lock_guard<std::mutex> lk(m);

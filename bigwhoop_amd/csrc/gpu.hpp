@@ -16,6 +16,12 @@ struct Workspace {
   int device = -1;
   hipStream_t stream = nullptr;
   hipEvent_t ev[8] = {};
+  // host fields: the tile's box is staged into d_input in chunks on
+  // copy_stream, one event per chunk; ev_fill orders a fill after the last
+  // reader of the staging buffer
+  hipStream_t copy_stream = nullptr;
+  std::vector<hipEvent_t> ev_copy;
+  hipEvent_t ev_fill = nullptr;
   // field staging (host input copied here) and the tile working buffer (f64)
   void *d_input = nullptr;
   size_t input_cap = 0;
@@ -169,6 +175,10 @@ int t1_decode(Workspace *ws, const bwc_codec *c, const Param &par, const DecodeR
 int decode_output(Workspace *ws, const bwc_codec *c, const Tile &tl, int p, const Param &par, const double *work,
                   void *out);
 int ensure_output(Workspace *ws, size_t bytes, void **out);
+// the samples decode_output wrote for (tile, param) into the staging buffer,
+// copied to the same places of a host output buffer
+int output_box_d2h(Workspace *ws, const bwc_codec *c, const Tile &tl, int p, const Param &par, const void *stage,
+                   void *out);
 int sync(Workspace *ws);
 const char *last_error();
 int ensure_work(Workspace *ws, uint64_t n);

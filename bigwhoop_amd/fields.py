@@ -114,3 +114,45 @@ def grf_device(n: int, slope=-5.0 / 3.0, seed=1, mean=10.0, std=1.0):
     out += mean
     torch.cuda.synchronize()
     return out.contiguous()
+
+
+def turbulence_4d(shape, nvar=5, seed=7, rho=0.95):
+    """(x, y, z, t) float64 fields of ``nvar`` variables, numpy layout
+    [variable][t][z][y][x]: variables 0-2 (u, v, w) from independent
+    E(k) ~ k^-5/3 Gaussian random fields, the others (p, rho) from k^-7/3
+    fields plus a 5% Gaussian pulse background (centred, width nx/6). In
+    time the complex spectral coefficients follow a first-order AR(rho)
+    process, so the phases evolve smoothly; every time step is rescaled to
+    unit variance and mean 10. Seeded and numpy-only, like :func:`grf`."""
+    nx, ny, nz, nt = (list(int(s) for s in shape) + [1, 1, 1])[:4]
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sshape = (nz, ny, nx)
+    kz = np.fft.fftfreq(nz) * nz
+    ky = np.fft.fftfreq(ny) * ny
+    kx = np.fft.rfftfreq(nx) * nx
+    kmag = np.sqrt(kz[:, None, None] ** 2 + ky[None, :, None] ** 2 + kx[None, None, :] ** 2)
+    cshape = kmag.shape
+    az = np.arange(nz, dtype=np.float64) - nz / 2
+    ay = np.arange(ny, dtype=np.float64) - ny / 2
+    ax = np.arange(nx, dtype=np.float64) - nx / 2
+    r2 = az[:, None, None] ** 2 + ay[None, :, None] ** 2 + ax[None, None, :] ** 2
+    pulse = np.exp(-r2 / (2 * (nx / 6.0) ** 2))
+    out = np.empty((nvar, nt) + sshape, np.float64)
+    s = (1.0 - rho * rho) ** 0.5
+    for v in range(nvar):
+        slope = -5.0 / 3.0 if v < 3 else -7.0 / 3.0
+        with np.errstate(divide="ignore"):
+            filt = np.where(kmag > 0, kmag ** ((slope - 2.0) / 2.0), 0.0)
+        w = rng.standard_normal(cshape) + 1j * rng.standard_normal(cshape)
+        for t in range(nt):
+            if t:
+                w = rho * w + s * (rng.standard_normal(cshape) + 1j * rng.standard_normal(cshape))
+            x = np.fft.irfftn(w * filt, s=sshape, axes=(0, 1, 2))
+            x -= x.mean()
+            sd = x.std()
+            if sd > 0:
+                x /= sd
+            if v >= 3:
+                x += 0.05 * pulse * np.abs(x).max()
+            out[v, t] = x + 10.0
+    return out

@@ -117,9 +117,9 @@ def grf_nd_device(shape, slope, seed, dtype):
     return out.to(dtype).contiguous()
 
 
-def batch_spec(count=64, seed=4):
+def batch_spec(count=64, seed=4, nmin=1e5, nmax=1e8):
     """The uneven batch: `count` fields with dimensionality uniform in 1-4,
-    element count log-uniform in [1e5, 1e8] split over the axes with random
+    element count log-uniform in [nmin, nmax] split over the axes with random
     aspect ratios, float32 or float64 (50/50), spectral slope U(-3, -1),
     bitrate log-uniform in [0.5, 8] bits per value, DWT levels
     min(5, floor(log2(shortest axis)) - 2) on every axis (at least 0), and
@@ -133,7 +133,7 @@ def batch_spec(count=64, seed=4):
     out = []
     for i in range(count):
         d = int(rng.integers(1, 5))
-        n = float(np.exp(rng.uniform(np.log(1e5), np.log(1e8))))
+        n = float(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))
         w = rng.uniform(-1.0, 1.0, size=d)
         w -= w.mean()
         dims = [max(8, int(round(n ** (1.0 / d) * np.exp(x)))) for x in w]

@@ -244,6 +244,9 @@ struct oc_codec
   uint8_t nlayers;
   uint64_t header_size, cs_size;
   uint8_t qstyle, prog;
+  uint8_t *aux, *com;       /* SAX / COM payloads (codestream.c:239-258) */
+  uint32_t naux;
+  uint16_t ncom;
   tile_t *tile;
   double lut[3][10];   /* snapshot of the global LUT at geometry creation */
 };
@@ -751,6 +754,25 @@ oc_codec *oc_create(const oc_params *p)
     c->erres ^= 0x01;
     c->CSsgc ^= (0x01 << 0);
   }
+  /* set_quant_step_size (libbwc.c:2134-2184): 0 < delta < 2, else a warning
+     and no change */
+  if (p->have_qstep && p->qstep > 0 && p->qstep < 2)
+  {
+    set_qt(c, p->qstep);
+    c->CSsgc |= (0x01 << 2);
+  }
+  if (p->naux)
+  {
+    c->aux = malloc(p->naux);
+    memcpy(c->aux, p->aux, p->naux);
+    c->naux = p->naux;
+  }
+  if (p->ncom)
+  {
+    c->com = malloc(p->ncom);
+    memcpy(c->com, p->com, p->ncom);
+    c->ncom = p->ncom;
+  }
 
   /* rate-control string (libbwc.c:3024-3089), or the layer rates as read
      from a codestream header */
@@ -803,13 +825,19 @@ oc_codec *oc_create(const oc_params *p)
     oc_destroy(c);
     return NULL;
   }
+  /* headerSize (libbwc.c:3096-3107): 6 bytes plus the payload for each of
+     AUX and COM, although a COM segment takes 4 bytes plus its payload */
   c->header_size = 111 + (uint64_t)c->nPar * c->ntiles * 2 * PREC_BYTES + (uint64_t)c->nlayers * 4;
+  if (c->naux) c->header_size += 6 + (uint64_t)c->naux;
+  if (c->ncom) c->header_size += 6 + (uint64_t)c->ncom;
   return c;
 }
 
 void oc_destroy(oc_codec *c)
 {
   if (!c) return;
+  free(c->aux);
+  free(c->com);
   for (uint64_t i = 0; c->tile && i < c->ntiles; ++i)
   {
     tile_t *tl = &c->tile[i];

@@ -51,6 +51,15 @@ typedef struct oc_params
   uint8_t  have_kern, kern[4];
   uint8_t  nrates;     /* > 0: these layer rates as read (no parsing of rate) */
   const float *rates;
+  /* encoder-side setters applied after error resilience, in this order:
+     set_quant_step_size (libbwc.c:2120-2184), then the stream's AUX and COM
+     segments (bwc_set_aux / bwc_set_com, libbwc.c:1264-1340) */
+  uint8_t  have_qstep;
+  double   qstep;
+  const uint8_t *aux;
+  uint32_t naux;
+  const uint8_t *com;
+  uint16_t ncom;
 } oc_params;
 
 typedef struct oc_codec oc_codec;
@@ -95,6 +104,17 @@ int       oc_t1_encode_block(const double *buf, uint64_t width, uint64_t height,
                              const uint64_t box[8], double step_eff, double step, double gain,
                              uint8_t highband, uint16_t Kmax, uint8_t erres,
                              uint8_t kkz[3], uint16_t *S, uint64_t *L, uint8_t *data, uint64_t cap);
+/* one codeblock coded to a pass limit (zlim > 0; 0 = every pass), for the
+   pass-skip tests: records as oc_t1_encode_block, plus the integer
+   distortion sum of each pass (dm), the read extent of each pass's
+   truncation-length search (reach: one past the last byte index read), the
+   bytes emitted before termination (nfinal) and the histogram of the
+   quantised magnitudes' bit lengths (hist[65]) */
+int       oc_t1_probe_block(const double *buf, uint64_t width, uint64_t height, uint64_t depth,
+                            const uint64_t box[8], double step_eff, double step, double gain,
+                            uint8_t highband, uint16_t Kmax, uint8_t erres, int zlim,
+                            uint8_t kkz[3], uint16_t *S, uint64_t *L, int64_t *dm, int64_t *reach,
+                            int64_t *nfinal, uint32_t hist[65]);
 /* MQ round trip helper: encode n (bit,ctx) symbols; returns bytes */
 uint64_t  oc_mq_encode_symbols(const uint8_t *bits, const uint8_t *ctx, uint64_t n,
                                uint8_t *out, uint64_t cap);

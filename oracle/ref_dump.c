@@ -9,7 +9,9 @@
  *
  * One compression per process (the reference's energy-gain LUT accumulates in
  * a process-global, dwt.c:80 / dwt.c:864-1045), with the fixed setter order
- * alloc -> decomp -> precincts -> codeblocks -> qm -> tiles -> erres.
+ * alloc -> kernels -> progression -> quant style -> decomp -> precincts ->
+ * codeblocks -> qm -> tiles -> erres -> quant step size, then COM / AUX on
+ * the stream before bwc_create_compression (the header size counts them).
  *
  * Modes
  *   full   : bwc_create_compression + bwc_compress (libbwc.c:2970, :3145);
@@ -72,7 +74,8 @@ int main(int argc, char **argv)
 {
   if (argc < 10)
     die("usage: ref_dump full|stages|decode input nX nY nZ nTS nPar prec rate prefix [--decomp a,b,c,d]"
-        " [--prec a,b,c,d] [--cb a,b,c,d] [--qm N] [--tiles a,b,c,d] [--erres] [--layer L]");
+        " [--prec a,b,c,d] [--cb a,b,c,d] [--qm N] [--tiles a,b,c,d] [--erres] [--layer L]"
+        " [--kernels a,b,c,d] [--prog N] [--qstyle N] [--qstep D] [--com TEXT] [--aux TEXT]");
   const char *mode = argv[1];
   const char *input = argv[2];
   uint64 nX = strtoull(argv[3], 0, 10), nY = strtoull(argv[4], 0, 10);
@@ -83,9 +86,12 @@ int main(int argc, char **argv)
   snprintf(rate, sizeof rate, "%s", argv[9]);
   const char *prefix = argv[10];
 
-  unsigned long long dec[4], pr[4], cb[4], tl[4];
+  unsigned long long dec[4], pr[4], cb[4], tl[4], kn[4];
   int have_dec = 0, have_pr = 0, have_cb = 0, have_qm = 0, have_tl = 0, erres = 0, layer = 0;
-  int qm = 0;
+  int have_kn = 0, have_prog = 0, have_qstyle = 0, have_qstep = 0;
+  int qm = 0, prog = 0, qstyle = 0;
+  double qstep = 0;
+  const char *com = NULL, *aux = NULL;
   for (int i = 11; i < argc; ++i)
   {
     if (!strcmp(argv[i], "--decomp") && i + 1 < argc) have_dec = parse4(argv[++i], dec);
@@ -95,6 +101,12 @@ int main(int argc, char **argv)
     else if (!strcmp(argv[i], "--tiles") && i + 1 < argc) have_tl = parse4(argv[++i], tl);
     else if (!strcmp(argv[i], "--erres")) erres = 1;
     else if (!strcmp(argv[i], "--layer") && i + 1 < argc) layer = atoi(argv[++i]);
+    else if (!strcmp(argv[i], "--kernels") && i + 1 < argc) have_kn = parse4(argv[++i], kn);
+    else if (!strcmp(argv[i], "--prog") && i + 1 < argc) { have_prog = 1; prog = atoi(argv[++i]); }
+    else if (!strcmp(argv[i], "--qstyle") && i + 1 < argc) { have_qstyle = 1; qstyle = atoi(argv[++i]); }
+    else if (!strcmp(argv[i], "--qstep") && i + 1 < argc) { have_qstep = 1; qstep = strtod(argv[++i], 0); }
+    else if (!strcmp(argv[i], "--com") && i + 1 < argc) com = argv[++i];
+    else if (!strcmp(argv[i], "--aux") && i + 1 < argc) aux = argv[++i];
     else die("bad option");
   }
 
@@ -123,14 +135,21 @@ int main(int argc, char **argv)
 
   bwc_codec *codec = bwc_alloc_coder(nX, nY, nZ, nTS, nPar, (bwc_precision)prec);
   if (!codec) die("alloc_coder failed");
+  if (have_kn)
+    set_kernels(codec, (bwc_dwt_filter)kn[0], (bwc_dwt_filter)kn[1], (bwc_dwt_filter)kn[2], (bwc_dwt_filter)kn[3]);
+  if (have_prog) set_progression(codec, (bwc_prog_ord)prog);
+  if (have_qstyle) set_quant_style(codec, (bwc_quant_st)qstyle);
   if (have_dec) bwc_set_decomp(codec, (uint8)dec[0], (uint8)dec[1], (uint8)dec[2], (uint8)dec[3]);
   if (have_pr) bwc_set_precincts(codec, (uint8)pr[0], (uint8)pr[1], (uint8)pr[2], (uint8)pr[3]);
   if (have_cb) bwc_set_codeblocks(codec, (uint8)cb[0], (uint8)cb[1], (uint8)cb[2], (uint8)cb[3]);
   if (have_qm) bwc_set_qm(codec, (uint8)qm);
   if (have_tl) bwc_set_tiles(codec, tl[0], tl[1], tl[2], tl[3], bwc_tile_sizeof);
   if (erres) bwc_set_error_resilience(codec);
+  if (have_qstep) set_quant_step_size(codec, qstep);
 
   bwc_stream *stream = bwc_init_stream(inp, out, comp);
+  if (com && bwc_set_com(stream, com, (uint16)strlen(com))) die("set_com failed");
+  if (aux && bwc_set_aux(stream, aux, (uint32)strlen(aux))) die("set_aux failed");
   if (bwc_create_compression(codec, stream, rate)) die("create_compression failed");
 
   if (!strcmp(mode, "full"))

@@ -30,8 +30,8 @@ ROOT = HERE.parents[1]
 sys.path.insert(0, str(ROOT))
 sys.path.insert(0, str(ROOT / "tests"))
 
-from bigwhoop_amd.fields import grf, modes_field  # noqa: E402
 import oracle_lib as ol  # noqa: E402
+from golden_cases import CODEC_KEYS, generate  # noqa: E402
 
 CASES = [
     # the CPU-runnable workload: 64^3 f64, 4 modes + noise, 3 levels, cb 16^3, 1 layer at 4 bpv
@@ -59,36 +59,67 @@ CASES = [
          store=True, stages=True),
     dict(name="grf2d_qm40", gen=dict(kind="grf", shape=[96, 80], seed=8, slope=-2.5), npar=1, prec=8,
          rate="2,5", qm=40, cb=[5, 4, 0, 0], store=True, stages=True),
+    # the 4-D turbulence workload's settings on a 32^3 x 8 field: five variables (u, v, w from k^-5/3,
+    # p and rho from k^-7/3 plus a pulse background, AR(0.95) phases in time), decomposition (5, 5, 5, 2),
+    # codeblocks 16x16x8x4, 2 bpv
+    dict(name="turb4d_5var", gen=dict(kind="turb4d", shape=[32, 32, 32, 8], seed=7, npar=5), npar=5, prec=8,
+         rate="2", decomp=[5, 5, 5, 2], cb=[4, 4, 3, 2], store=True, stages=True),
+    # tiles along every axis including time (fill_buffer and flush_buffer leave out the tile's time offset)
+    dict(name="grf4d_ttiles", gen=dict(kind="grf", shape=[24, 20, 16, 8], seed=13, slope=-2.0), npar=1, prec=8,
+         rate="3", decomp=[2, 2, 2, 1], cb=[3, 3, 3, 2], tiles=[16, 16, 16, 4], store=True, stages=True),
+    # exported options: LeGall 5/3 kernels, RLCP and CPRL progression, quantisation step size and style,
+    # AUX and COM segments
+    dict(name="grf3d_legall", gen=dict(kind="grf", shape=[40, 36, 28], seed=17, slope=-2.0), npar=1, prec=8,
+         rate="1,4", kernels=[1, 1, 0, 1], decomp=[3, 3, 2, 0], cb=[4, 4, 3, 0], store=True, stages=True),
+    dict(name="grf4d_legall_t", gen=dict(kind="grf", shape=[24, 20, 16, 8], seed=19, slope=-5 / 3), npar=1,
+         prec=8, rate="2", kernels=[0, 0, 0, 1], decomp=[2, 2, 2, 2], cb=[3, 3, 3, 2], store=True, stages=True),
+    dict(name="grf_rlcp_2var", gen=dict(kind="grf", shape=[48, 40, 24], seed=23, slope=-2.0, npar=2), npar=2,
+         prec=8, rate="0.5,2,6", prog=1, decomp=[3, 3, 2, 0], cb=[4, 4, 3, 0], precincts=[3, 3, 2, 0],
+         store=True, stages=False),
+    dict(name="grf_cprl", gen=dict(kind="grf", shape=[48, 40, 24], seed=29, slope=-2.0), npar=1, prec=8,
+         rate="1,5", prog=4, decomp=[3, 3, 2, 0], cb=[4, 4, 3, 0], precincts=[3, 3, 2, 0], store=True,
+         stages=False),
+    dict(name="grf_qstep", gen=dict(kind="grf", shape=[40, 36, 28], seed=31, slope=-5 / 3), npar=1, prec=8,
+         rate="3", qstep=1.6e-19, store=True, stages=True),
+    # no quantisation (step size 1): with Qm 20, so that no pass's slope reaches 2^64 (the reference's convex
+    # hull then pops past the start of its array, tier1.c:2198-2212)
+    dict(name="grf_qnone", gen=dict(kind="grf", shape=[40, 36, 28], seed=37, slope=-5 / 3), npar=1, prec=4,
+         rate="2", qstyle=0, qm=20, decomp=[2, 2, 2, 0], store=True, stages=True),
+    dict(name="grf_com_aux", gen=dict(kind="grf", shape=[48, 40, 24], seed=41, slope=-2.0), npar=1, prec=8,
+         rate="2", com="MI355X parity case", aux="auxiliary block: 0123456789abcdef", erres=True, store=True,
+         stages=False),
+    # constant variables (max = min: beta = 2/0, the normalised values are 0 * inf)
+    dict(name="grf_constvars", gen=dict(kind="mixed", shape=[40, 36, 28], seed=43,
+                                        vars=[dict(kind="grf", slope=-5 / 3), dict(kind="const", value=1.0),
+                                              dict(kind="const", value=0.0)]),
+         npar=3, prec=8, rate="0.5,2", qm=32, erres=True, store=True, stages=True),
+    # the headline field and settings at full size: 512^3 f64 k^-5/3, 5 levels, cb 32x32x8, one layer at
+    # 2 bpv and four layers at 1, 2, 4, 8 bpv (digests only)
+    dict(name="grf512_l5_r2", gen=dict(kind="grf", shape=[512, 512, 512], seed=1, slope=-5 / 3), npar=1, prec=8,
+         rate="2", decomp=[5, 5, 5, 0], cb=[5, 5, 3, 0], store=False, stages=False, large=True),
+    dict(name="grf512_l5_r1248", gen=dict(kind="grf", shape=[512, 512, 512], seed=1, slope=-5 / 3), npar=1,
+         prec=8, rate="1,2,4,8", decomp=[5, 5, 5, 0], cb=[5, 5, 3, 0], store=False, stages=False, large=True),
 ]
 
 
-def make_input(gen):
-    npar = gen.get("npar", 1)
-    parts = []
-    for p in range(npar):
-        if gen["kind"] == "modes":
-            parts.append(modes_field(gen["shape"], seed=gen["seed"] + 1000 * p))
-        else:
-            parts.append(grf(gen["shape"], slope=gen["slope"], seed=gen["seed"] + 1000 * p))
-    return np.concatenate([a.reshape(-1) for a in parts])
-
-
 def codec_kw(case):
-    kw = {}
-    for k in ("decomp", "cb", "precincts", "qm", "tiles", "erres"):
-        if k in case:
-            kw[k] = case[k]
-    return kw
+    return {k: case[k] for k in CODEC_KEYS if k in case}
 
 
-def main():
+def main(only=None):
+    """Write every case (or the named ones, keeping the others' entries)."""
     if not ol.have_ref():
         raise SystemExit("oracle/_ref/ref_dump missing: run `make -C oracle ref` where the reference exists")
+    old = {}
+    if only and (HERE / "cases.json").exists():
+        old = {c["name"]: c for c in json.loads((HERE / "cases.json").read_text())}
     meta = []
     for case in CASES:
-        data = make_input(case["gen"])
-        if case["prec"] == 4:
-            data = data.astype(np.float32)
+        if only and case["name"] not in only:
+            if case["name"] in old:
+                meta.append(old[case["name"]])
+            continue
+        data = generate(case["gen"], case["prec"])
         shape = case["gen"]["shape"]
         kw = codec_kw(case)
         out = ol.ref_run("full", data, shape, npar=case["npar"], prec=case["prec"], rate=case["rate"], **kw)
@@ -102,10 +133,11 @@ def main():
         dec = ol.ref_decode(cs, shape, npar=case["npar"], prec=case["prec"])
         ent["decode_sha256"] = hashlib.sha256(dec).hexdigest()
         ent["decode_size"] = len(dec)
-        vals = np.frombuffer(dec, np.float64 if case["prec"] == 8 else np.float32)
-        np.savez_compressed(HERE / f"{case['name']}.dec.npz", sample=vals[::53].copy(),
-                            n=np.array([vals.size], np.uint64))
-        ent["files"].append(f"{case['name']}.dec.npz")
+        if not case.get("large"):   # full-size cases keep the digest only
+            vals = np.frombuffer(dec, np.float64 if case["prec"] == 8 else np.float32)
+            np.savez_compressed(HERE / f"{case['name']}.dec.npz", sample=vals[::53].copy(),
+                                n=np.array([vals.size], np.uint64))
+            ent["files"].append(f"{case['name']}.dec.npz")
         if case["store"]:
             (HERE / f"{case['name']}.bwc").write_bytes(cs)
             ent["files"].append(f"{case['name']}.bwc")
@@ -146,4 +178,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(set(sys.argv[1:]) or None)

@@ -49,6 +49,12 @@ class OcParams(C.Structure):
         ("kern", C.c_uint8 * 4),
         ("nrates", C.c_uint8),
         ("rates", C.POINTER(C.c_float)),
+        ("have_qstep", C.c_uint8),
+        ("qstep", C.c_double),
+        ("aux", C.c_char_p),
+        ("naux", C.c_uint32),
+        ("com", C.c_char_p),
+        ("ncom", C.c_uint16),
     ]
 
 
@@ -91,6 +97,12 @@ def lib():
         L.oc_t2_encode.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(C.c_int16)]
         L.oc_assemble.restype = C.c_size_t
         L.oc_assemble.argtypes = [C.c_void_p, u8p, C.c_size_t]
+        L.oc_t1_probe_block.restype = C.c_int
+        L.oc_t1_probe_block.argtypes = [C.POINTER(C.c_double), C.c_uint64, C.c_uint64, C.c_uint64,
+                                        C.POINTER(C.c_uint64), C.c_double, C.c_double, C.c_double, C.c_uint8,
+                                        C.c_uint16, C.c_uint8, C.c_int, C.POINTER(C.c_uint8),
+                                        C.POINTER(C.c_uint16), C.POINTER(C.c_uint64), C.POINTER(C.c_int64),
+                                        C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_uint32)]
         L.oc_t1_encode_block.restype = C.c_int
         L.oc_t1_encode_block.argtypes = [C.POINTER(C.c_double), C.c_uint64, C.c_uint64, C.c_uint64,
                                          C.POINTER(C.c_uint64), C.c_double, C.c_double, C.c_double, C.c_uint8,
@@ -113,7 +125,10 @@ def ptr(a, t):
 
 
 def make_params(shape, npar=1, prec=8, rate="2", decomp=None, cb=None, precincts=None, qm=None,
-                tiles=None, erres=False):
+                tiles=None, erres=False, kernels=None, prog=None, qstyle=None, qstep=None, com=None, aux=None):
+    """Oracle settings; the setters apply in the order kernels, progression,
+    quantisation style, decomposition, precincts, codeblocks, Qm, tiles,
+    error resilience, quantisation step size, then AUX and COM."""
     p = OcParams()
     n = list(shape) + [1] * (4 - len(shape))
     for i in range(4):
@@ -141,6 +156,25 @@ def make_params(shape, npar=1, prec=8, rate="2", decomp=None, cb=None, precincts
             p.tiles[i] = tiles[i]
     p.erres = 1 if erres else 0
     p.rate = rate.encode() if rate is not None else None
+    if kernels is not None:
+        p.have_kern = 1
+        for i in range(4):
+            p.kern[i] = kernels[i]
+    if prog is not None:
+        p.have_prog = 1
+        p.prog = prog
+    if qstyle is not None:
+        p.have_qstyle = 1
+        p.qstyle = qstyle
+    if qstep is not None:
+        p.have_qstep = 1
+        p.qstep = float(qstep)
+    if aux is not None:
+        p.aux = aux.encode() if isinstance(aux, str) else bytes(aux)
+        p.naux = len(p.aux)
+    if com is not None:
+        p.com = com.encode() if isinstance(com, str) else bytes(com)
+        p.ncom = len(p.com)
     return p
 
 
@@ -279,6 +313,28 @@ def t1_encode_block(buf, dims, box, step_eff, step, gain, hb, kmax, erres=False)
     return dict(K=K, Kmsbs=Kmsbs, Z=Z, S=S[: Z + 1].copy(), L=L[:Z].copy(), data=data[:nb].tobytes())
 
 
+def t1_probe_block(buf, dims, box, step_eff, step, gain, hb, kmax, zlim=0, erres=False):
+    """One codeblock coded to `zlim` passes (0: all), with the pass-limit
+    probes: per-pass integer distortion sums `dm`, truncation-search read
+    extents `reach`, bytes emitted before termination `nfinal`, and the
+    bit-length histogram `hist` of the quantised magnitudes."""
+    buf = np.ascontiguousarray(buf, dtype=np.float64)
+    b = (C.c_uint64 * 8)(*[int(x) for x in box])
+    kkz = (C.c_uint8 * 3)()
+    S = np.zeros(191, np.uint16)
+    L = np.zeros(190, np.uint64)
+    dm = np.zeros(190, np.int64)
+    reach = np.zeros(190, np.int64)
+    nfinal = C.c_int64(0)
+    hist = np.zeros(65, np.uint32)
+    lib().oc_t1_probe_block(ptr(buf, C.c_double), dims[0], dims[1], dims[2], b, step_eff, step, gain, hb, kmax,
+                            1 if erres else 0, int(zlim), kkz, ptr(S, C.c_uint16), ptr(L, C.c_uint64),
+                            ptr(dm, C.c_int64), ptr(reach, C.c_int64), C.byref(nfinal), ptr(hist, C.c_uint32))
+    K, Kmsbs, Z = kkz
+    return dict(K=K, Z=Z, S=S[: Z + 1].copy(), L=L[:Z].copy(), dm=dm[:Z].copy(), reach=reach[:Z].copy(),
+                nfinal=int(nfinal.value), hist=hist.copy())
+
+
 # ---------------------------------------------------------------------------
 # reference driver (only where oracle/_ref was built from the reference source)
 # ---------------------------------------------------------------------------
@@ -286,8 +342,21 @@ def have_ref():
     return REF_DUMP.exists()
 
 
-def _opts(decomp=None, cb=None, precincts=None, qm=None, tiles=None, erres=False):
+def _opts(decomp=None, cb=None, precincts=None, qm=None, tiles=None, erres=False, kernels=None, prog=None,
+          qstyle=None, qstep=None, com=None, aux=None):
     o = []
+    if kernels is not None:
+        o += ["--kernels", ",".join(str(int(x)) for x in kernels)]
+    if prog is not None:
+        o += ["--prog", str(int(prog))]
+    if qstyle is not None:
+        o += ["--qstyle", str(int(qstyle))]
+    if qstep is not None:
+        o += ["--qstep", repr(float(qstep))]
+    if com is not None:
+        o += ["--com", com]
+    if aux is not None:
+        o += ["--aux", aux]
     if decomp is not None:
         o += ["--decomp", ",".join(str(int(x)) for x in decomp)]
     if precincts is not None:
