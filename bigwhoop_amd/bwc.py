@@ -94,6 +94,7 @@ def lib():
             "bwc_amd_ncblk": (u64, [vp, u64, u8]),
             "bwc_amd_param_dims": (C.c_int, [vp, u64, C.POINTER(u64)]),
             "bwc_amd_stage_dwt": (C.c_int, [vp, u64, u8, vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+            "bwc_amd_stage_dwt_noalloc": (C.c_int, [vp, u64, u8, vp, vp, vp, vp]),
             "bwc_amd_stage_idwt": (C.c_int, [vp, u64, u8, C.POINTER(C.c_double)]),
             "bwc_amd_stage_t1": (C.c_int, [vp, u64, u8, C.POINTER(C.c_double), C.POINTER(u8), C.POINTER(u8),
                                            C.POINTER(u8), C.POINTER(u32), C.POINTER(u16), C.POINTER(u8), u64,

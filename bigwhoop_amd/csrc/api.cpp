@@ -879,6 +879,24 @@ int bwc_amd_stage_dwt(bwc_codec *codec, uint64_t tile, uint8_t param, const void
   return gpu::copy_d2h(ws, out, ws->d_work, n * sizeof(double));
 }
 
+int bwc_amd_stage_dwt_noalloc(bwc_codec *codec, uint64_t tile, uint8_t param, const void *input, double *out,
+                              double *norm, void *stream) {
+  Job *job = job_of(codec);
+  if (!job || !job->created || tile >= job->plan.tiles.size() || param >= codec->info.nPar) {
+    set_err("bwc_amd_stage_dwt_noalloc: codec not created or tile / parameter out of range");
+    return 1;
+  }
+  if (!input || !out || !norm) {
+    set_err("bwc_amd_stage_dwt_noalloc: null buffer");
+    return 1;
+  }
+  if (gpu::dwt_noalloc(codec, job->plan.tiles[tile], param, input, out, norm, static_cast<hipStream_t>(stream))) {
+    set_err(gpu::last_error());
+    return 1;
+  }
+  return 0;
+}
+
 int bwc_amd_stage_idwt(bwc_codec *codec, uint64_t tile, uint8_t param, double *coeffs) {
   Job *job = job_of(codec);
   if (!job || !job->created || tile >= job->plan.tiles.size() || param >= codec->info.nPar) return 1;

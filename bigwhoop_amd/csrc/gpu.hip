@@ -901,6 +901,72 @@ int dwt_inverse(Workspace *ws, const bwc_codec *c, const Param &par, double *wor
   return dwt_run(ws, c, par, work, false, -1);
 }
 
+// ===========================================================================
+// Fill + forward DWT with no allocation: the min/max partials and the second
+// buffer of the out-of-place passes live in `scratch`, a static device array
+// (each GPU the library runs on holds its own copy until the process ends).
+// Its device address is looked up once per device with hipGetSymbolAddress;
+// host code uses only that address, never the array's name. Calls that
+// overlap on one GPU share the array and race.
+// ===========================================================================
+constexpr size_t kScratchRed = 2 * (size_t)kRedSlots * sizeof(double);   // min/max partials
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[64];
+
+static int scratch_of_device(void **out) {
+  int dev = 0;
+  HIP_OK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) {
+    std::snprintf(g_err, sizeof g_err, "device %d out of range for the scratch table", dev);
+    return 1;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+int dwt_noalloc(const bwc_codec *c, const Tile &tl, int param, const void *input, double *out, double *norm,
+                hipStream_t stream) {
+  const Param &par = tl.par[param];
+  const uint64_t n = par.box.volume();
+  if (n * sizeof(double) > kNoAllocScratchBytes - kScratchRed) {
+    std::snprintf(g_err, sizeof g_err,
+                  "tile parameter of %llu values exceeds the allocation-free limit of %llu values",
+                  (unsigned long long)n, (unsigned long long)((kNoAllocScratchBytes - kScratchRed) / sizeof(double)));
+    return 1;
+  }
+  if (!is_device_pointer(input) || !is_device_pointer(out) || !is_device_pointer(norm)) {
+    std::snprintf(g_err, sizeof g_err, "the allocation-free DWT takes device pointers only");
+    return 1;
+  }
+  CHK(process_setup());
+  void *sp = nullptr;
+  CHK(scratch_of_device(&sp));
+  double *red = static_cast<double *>(sp);
+  double *work2 = reinterpret_cast<double *>(static_cast<unsigned char *>(sp) + kScratchRed);
+  const uint64_t nX = c->info.nX, nY = c->info.nY, nZ = c->info.nZ, nTS = c->info.nTS;
+  const int prec = c->info.data_prec == bwc_precision_single ? 4 : 8;
+  const int threads = 256;
+  const int blocks = (int)std::max<uint64_t>(1, std::min<uint64_t>((n + threads - 1) / threads, 4096));
+  hipLaunchKernelGGL(fill_minmax_kernel, dim3(blocks), dim3(threads), 0, stream, input, prec, nX, nY, nZ,
+                     nX * nY * nZ * nTS * (uint64_t)param, (uint64_t)tl.box.lo[0], (uint64_t)tl.box.lo[1],
+                     (uint64_t)tl.box.lo[2], par.box.len(0), par.box.len(1), par.box.len(2), (uint64_t)0, n, out,
+                     red);
+  hipLaunchKernelGGL(minmax_final_kernel, dim3(1), dim3(256), 0, stream, red, blocks, norm);
+  HIP_OK(hipGetLastError());
+  DwtCtx x;
+  x.stream = stream;
+  x.work2 = work2;
+  x.work2_cap = kNoAllocScratchBytes - kScratchRed;
+  x.norm = norm;
+  return dwt_passes(x, c, par, out, true, c->control.Qm);
+}
 
 // ===========================================================================
 // quantisation into stripe columns (tier1.c:526-729)

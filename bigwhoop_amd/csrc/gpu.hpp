@@ -125,6 +125,16 @@ int normalize(Workspace *ws, double *work, uint64_t n, uint8_t Qm);
 // it (fused into its first pass where it can)
 int dwt_forward(Workspace *ws, const bwc_codec *c, const Param &par, double *work, int norm_qm = -1);
 int dwt_inverse(Workspace *ws, const bwc_codec *c, const Param &par, double *work);
+// Fill + min/max + normalised forward DWT of (tile, param) from the caller's
+// device field into `out` (device, the parameter's tile-local layout), with
+// {min, max, alpha, beta} into `norm` (device, 4 doubles), asynchronously on
+// `stream`, without allocating: the partials and the second DWT buffer use a
+// static device scratch array of kNoAllocScratchBytes, so the parameter may
+// hold at most (kNoAllocScratchBytes - 128 KiB) / 8 values. Concurrent calls
+// on one GPU race on that array.
+constexpr size_t kNoAllocScratchBytes = (64ull << 20) + (128ull << 10);
+int dwt_noalloc(const bwc_codec *c, const Tile &tl, int param, const void *input, double *out, double *norm,
+                hipStream_t stream);
 // quantise + tier-1 for all codeblocks of (tile, param); results land in rec
 // Pass limits for t1_encode: none, given per codeblock (host, par.ncb
 // entries), or predicted on the device by the pass-limit network from the

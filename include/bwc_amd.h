@@ -83,6 +83,22 @@ int bwc_amd_param_dims(bwc_codec *codec, uint64_t tile, uint64_t dims[4]);
    coefficients; mm receives {min, max, alpha, beta} */
 int bwc_amd_stage_dwt(bwc_codec *codec, uint64_t tile, uint8_t param, const void *input, double *out, double mm[4]);
 
+/* Allocation-free variant of bwc_amd_stage_dwt. No counterpart in the
+   reference: it is for callers that cannot allocate device memory (code
+   under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc). input: the field in device memory ([param][t][z][y][x], as
+   bwc_compress reads it); out: device buffer of the parameter's tile-local
+   size (bwc_amd_param_dims) receiving the float64 coefficients; norm: device
+   buffer of 4 doubles receiving {min, max, alpha, beta}; stream: a
+   hipStream_t (NULL: the default stream). Asynchronous on that stream, no
+   host synchronisation. Nothing is allocated: the min/max partials and the
+   second buffer of the out-of-place DWT passes live in a static device
+   array of 64 MiB + 128 KiB, so a tile parameter may hold at most 8,388,608
+   values (larger ones return 1 with an error). Every call on a GPU shares
+   that GPU's array: calls that overlap on one GPU race. */
+int bwc_amd_stage_dwt_noalloc(bwc_codec *codec, uint64_t tile, uint8_t param, const void *input, double *out,
+                              double *norm, void *stream);
+
 /* inverse DWT of tile-local float64 coefficients in place on the GPU
    (dwt.c:1738-2246) */
 int bwc_amd_stage_idwt(bwc_codec *codec, uint64_t tile, uint8_t param, double *coeffs);
