@@ -112,6 +112,7 @@ def lib():
             "bwc_amd_cblk_cuts": (C.c_int, [vp, u64, u8, C.POINTER(C.c_int16)]),
             "bwc_amd_collect_features": (C.c_int, [vp, C.c_int]),
             "bwc_amd_cblk_features": (C.c_int, [vp, u64, u8, C.POINTER(C.c_float)]),
+            "bwc_amd_cblk_decisions": (C.c_int, [vp, u64, u8, C.POINTER(u32)]),
             "bwc_amd_skipnet_train": (C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float), u64,
                                                 C.POINTER(C.c_float), C.c_int, C.c_float, u64,
                                                 C.POINTER(C.c_float)]),
@@ -389,6 +390,13 @@ class Codec:
         return out
 
     # ---- single stages (bwc_amd.h) ----
+    def decisions(self, tile=0, param=0):
+        """MQ decisions per codeblock of the last compress (collect_features on)."""
+        out = np.zeros(self.ncblk(tile, param), np.uint32)
+        if lib().bwc_amd_cblk_decisions(self.h, tile, param, _p(out, C.c_uint32)):
+            raise RuntimeError("bwc_amd_cblk_decisions failed: " + last_error())
+        return out
+
     def ncblk(self, tile=0, param=0):
         return lib().bwc_amd_ncblk(self.h, tile, param)
 

@@ -81,6 +81,7 @@ void reset_for_encode(bwc_codec *c, Job *job) {
     R.src_off.clear();
     R.zlim.clear();
     R.feat.clear();
+    R.decisions.clear();
   }
 }
 
@@ -1052,6 +1053,18 @@ int bwc_amd_cblk_features(bwc_codec *codec, uint64_t tile, uint8_t param, float 
     return 1;
   }
   std::memcpy(feat, job->rec[r].feat.data(), job->rec[r].feat.size() * sizeof(float));
+  return 0;
+}
+
+int bwc_amd_cblk_decisions(bwc_codec *codec, uint64_t tile, uint8_t param, uint32_t *out) {
+  Job *job = job_of(codec);
+  if (!job || tile >= job->plan.tiles.size() || param >= codec->info.nPar) return 1;
+  const uint64_t r = tile * codec->info.nPar + param;
+  if (r >= job->rec.size() || job->rec[r].decisions.size() != job->rec[r].n) {
+    set_err("no decision counts collected for this tile and parameter");
+    return 1;
+  }
+  std::memcpy(out, job->rec[r].decisions.data(), job->rec[r].decisions.size() * sizeof(uint32_t));
   return 0;
 }
 

@@ -138,6 +138,9 @@ struct CblkRecords {
   std::vector<uint8_t> zlim;
   // pass-limit network features (16 per codeblock) when collected
   std::vector<float> feat;
+  // MQ decisions of each codeblock's last coding (fast tier-1 path), when
+  // collected with the features
+  std::vector<uint32_t> decisions;
   void resize(uint64_t ncb, int nlayers);
 };
 

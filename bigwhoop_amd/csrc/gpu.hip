@@ -2113,6 +2113,15 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
       }
     CHK(v2_code(ws, c, par, tb, rec.K.data(), list, zlim, d_data, bs.bp, d_hist, gp));
     CHK(fetch_rows(ws, rec, list, m, hfl));
+    if (lim.features) {
+      // decisions per codeblock: the symbol position where its last coded
+      // pass ends (measurement only)
+      std::vector<uint32_t> pc((size_t)ncb * kMaxPasses);
+      HIP_OK(hipMemcpy(pc.data(), ws->d_pcnt, pc.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+      rec.decisions.assign(ncb, 0);
+      for (size_t r = 0; r < list.size(); ++r)
+        if (m[r]) rec.decisions[list[r]] = pc[list[r] * kMaxPasses + m[r] - 1];
+    }
   } else {
     cap = ws->cap_q;
     CHK(ensure((void **)&ws->d_q, &cap, qo * sizeof(uint64_t)));

@@ -159,6 +159,10 @@ int bwc_amd_cblk_cuts(bwc_codec *codec, uint64_t tile, uint8_t param, int16_t *c
    (ncblk x 16 floats; feature 15 relative to the tile mean). */
 int bwc_amd_collect_features(bwc_codec *codec, int on);
 int bwc_amd_cblk_features(bwc_codec *codec, uint64_t tile, uint8_t param, float *feat);
+/* With features collected: the MQ decisions of each codeblock's last coding
+   in the last bwc_compress (ncblk entries; fast tier-1 path, 0 for
+   codeblocks without significant bits), for measurement. */
+int bwc_amd_cblk_decisions(bwc_codec *codec, uint64_t tile, uint8_t param, uint32_t *out);
 /* Pass-limit network on the GPU (BF16 matrix cores, fp32 accumulation).
    Targets are cuts in units of 8 passes; weights as for
    bwc_amd_set_skip_model, feature means and scales set by the caller.
