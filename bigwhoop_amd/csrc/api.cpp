@@ -173,6 +173,9 @@ int rate_control_prefix(bwc_codec *c, Job *job, gpu::Workspace *ws, uint64_t ti)
   const int np = c->info.nPar;
   SkipCheck chk;
   chk.margin = 3;
+  // BWC_AMD_SKIP_SETTLE: passes a cut must lie before the first unsettled
+  // pass (measurement of the proof's slack; at least 1)
+  if (const char *e = std::getenv("BWC_AMD_SKIP_SETTLE")) chk.margin = std::max(1, std::atoi(e));
   for (int round = 0;; ++round) {
     reset_tile_t2(c, job, ti);
     // every codeblock coded completely: the outcome is exact as it stands

@@ -27,14 +27,8 @@ import json
 import sys
 import time
 
-SKIP_MARGIN = 6
-# Margins for the uneven batch, by field dimensionality. The network was
-# trained on 3-D fields at 1 and 2 bpv; on the batch's other shapes and rates
-# it under-predicts more, and a wider margin saves re-coding rounds. Measured
-# on the batch (ms per dimensionality group at margins 6 / 24 / 36 / 48):
-# 1-D 1336 / 1130 / 1003 / 933, 2-D 731 / 440 / 451 / 484,
-# 3-D 366 / 308 / 403 / 491, 4-D 538 / 403 / 519 / 636.
-BATCH_MARGIN = {1: 48, 2: 24, 3: 24, 4: 24}
+def _top_rate(rate: str) -> float:
+    return max(float(r) for r in rate.replace("-", ",").replace(" ", ",").split(",") if r)
 
 
 def _spectral_filter(n, slope, device):
@@ -187,8 +181,10 @@ def compare(name, shape, npar, ptr, nbytes, settings, steps=2, warmup=1):
     dt_p, cs_p = _time(plain, ptr, steps)
     plain.close()
     learned = bwc.Codec(shape, npar=npar, **settings)
-    learned.set_skip_model(skipnet.default_weights())
-    learned.set_pass_skip(2, SKIP_MARGIN)
+    w, margin = skipnet.choose(sum(1 for n in shape if n > 1), _top_rate(settings.get("rate", "2")))
+    learned.set_skip_model(w)
+    learned.set_pass_skip(2, margin)
+    res["margin"] = margin
     for _ in range(warmup):
         learned.compress_into(ptr)
     dt_l, cs_l = _time(learned, ptr, steps)
@@ -208,8 +204,8 @@ def compare(name, shape, npar, ptr, nbytes, settings, steps=2, warmup=1):
 
 def batch64(count=64, seed=4, steps=2):
     """The uneven batch compressed field after field (one bwc_compress per
-    field) by the plain and by the learned encoder (margin by field
-    dimensionality, BATCH_MARGIN), then all fields concurrently; aggregate
+    field) by the plain and by the learned encoder (network and margin from
+    skipnet.choose), then all fields concurrently; aggregate
     input GB/s over the whole batch; every field's codestreams must be
     identical across the four runs."""
     import torch
@@ -224,8 +220,9 @@ def batch64(count=64, seed=4, steps=2):
         for f, t in zip(spec, fields):
             c = bwc.Codec(f["shape"], prec=t.element_size(), rate=f["rate"], decomp=f["decomp"], cb=f["cb"])
             if learned:
-                c.set_skip_model(skipnet.default_weights())
-                c.set_pass_skip(2, BATCH_MARGIN[f["ndim"]])
+                w, margin = skipnet.choose(f["ndim"], _top_rate(f["rate"]))
+                c.set_skip_model(w)
+                c.set_pass_skip(2, margin)
             cs.append(c)
         return cs
 
@@ -281,7 +278,6 @@ def batch64(count=64, seed=4, steps=2):
            "plain_gbs": round(total / dt_p / 1e9, 3), "plain_ms": round(dt_p * 1e3, 2),
            "learned_gbs": round(total / dt_l / 1e9, 3), "learned_ms": round(dt_l * 1e3, 2),
            "speedup_learned_vs_plain": round(dt_p / dt_l, 3), "codestream_bytes": sum(len(c) for c in cs_l),
-           "learned_margin_by_ndim": BATCH_MARGIN,
            "concurrent": {"plain_gbs": round(total / dtc_p / 1e9, 3), "plain_ms": round(dtc_p * 1e3, 2),
                           "learned_gbs": round(total / dtc_l / 1e9, 3), "learned_ms": round(dtc_l * 1e3, 2),
                           "workers": 8, "identical_to_one_at_a_time": True},

@@ -5,14 +5,15 @@ U(-3, -1), bitrate log-uniform 0.5-8 bpv, DWT levels
 min(5, floor(log2(shortest axis)) - 2), the workload's codeblock shapes),
 with element counts log-uniform in [1e4, 1e6] instead of [1e5, 1e8].
 Every codestream of one concurrent bwc_amd_compress_batch call, plain and
-with the learned pass limits, must equal the oracle's for that field."""
+with the learned pass limits (network and margin from skipnet.choose), must
+equal the oracle's for that field."""
 import numpy as np
 import pytest
 
 import oracle_lib as ol
 from bigwhoop_amd import bwc, skipnet
 from bigwhoop_amd.fields import grf
-from bigwhoop_amd.workloads import BATCH_MARGIN, batch_spec
+from bigwhoop_amd.workloads import batch_spec
 
 pytestmark = pytest.mark.gpu
 
@@ -43,8 +44,9 @@ def test_uneven_batch_matches_oracle(fields_and_oracle, learned):
         for f, x in zip(SPEC, fields):
             c = bwc.Codec(f["shape"], prec=x.itemsize, rate=f["rate"], decomp=f["decomp"], cb=f["cb"])
             if learned:
-                c.set_skip_model(skipnet.default_weights())
-                c.set_pass_skip(2, BATCH_MARGIN[f["ndim"]])
+                w, margin = skipnet.choose(f["ndim"], float(f["rate"]))
+                c.set_skip_model(w)
+                c.set_pass_skip(2, margin)
             codecs.append(c)
         print("uneven batch", "learned" if learned else "plain", len(codecs), "fields", flush=True)
         sizes = bwc.compress_batch(codecs, [x.ctypes.data for x in fields], workers=8)

@@ -102,14 +102,15 @@ def test_learned_limit_matches_plain(spec, margin):
         skip.close()
 
 
+@pytest.mark.parametrize("model", ["headline", "wide"])
 @pytest.mark.parametrize("case", CASES, ids=[c["name"] for c in CASES])
-def test_learned_limit_matches_reference(case):
+def test_learned_limit_matches_reference(case, model):
     from bigwhoop_amd import skipnet
-    _say("skip learned ref", case["name"])
+    _say("skip learned ref", model, case["name"])
     data = make_input(case)
     c = bwc.Codec(case["gen"]["shape"], npar=case["npar"], prec=case["prec"], rate=case["rate"], **codec_kw(case))
     try:
-        c.set_skip_model(skipnet.default_weights())
+        c.set_skip_model(skipnet.default_weights() if model == "headline" else skipnet.wide_weights())
         c.set_pass_skip(2, 2)
         cs = c.compress(data)
         assert sha(cs) == case["codestream_sha256"], c.skip_stats()

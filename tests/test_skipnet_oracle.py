@@ -70,3 +70,20 @@ def test_limits_clamp_to_complete_coding():
     y = np.array([-1.0, 0.1, 1.0, 100.0])
     K = np.array([5, 5, 1, 3])
     assert sn.limits(y, K, 2).tolist() == [2, 3, 1, 7]
+
+
+def test_committed_networks_and_choice():
+    """Both committed weight sets load (5345 float32 values, finite), and the
+    selector keeps the headline population's network for 3-D fields up to
+    2 bpv and takes the wide one otherwise."""
+    from bigwhoop_amd import skipnet
+
+    a, b = skipnet.default_weights(), skipnet.wide_weights()
+    assert a.size == b.size == skipnet.NPARAM
+    assert np.isfinite(a).all() and np.isfinite(b).all()
+    assert not np.array_equal(a, b)
+    w, m = skipnet.choose(3, 2.0)
+    assert np.array_equal(w, a) and m == 6
+    for nd, r in ((3, 4.0), (1, 1.0), (2, 0.5), (4, 2.0)):
+        w, m = skipnet.choose(nd, r)
+        assert np.array_equal(w, b) and m == 8
