@@ -47,6 +47,14 @@ void refresh_qt(bwc_codec *c) {
 
 uint8_t levels_of(uint64_t n) { return (uint8_t)(std::log((double)n) / std::log(2)); }
 
+bool trace_on() {
+  static const bool on = [] {
+    const char *e = std::getenv("BWC_AMD_RC_TRACE");
+    return e && e[0] == '1';
+  }();
+  return on;
+}
+
 double now_ms() {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -618,10 +626,14 @@ size_t bwc_compress(bwc_codec *const codec, bwc_stream *const stream) {
       // packet headers, then every packet body gathered on the device into
       // one contiguous range of the body buffer (the parameter slots are
       // coded over by the next tile)
+      // (the gather runs on the GPU while the host writes the headers)
       const double t1 = now_ms();
-      t2_packets(codec, job, ti);
       if (gather_tile_bodies(codec, job, ws, ti, body_end)) goto fail;
+      const double t2 = now_ms();
+      t2_packets(codec, job, ti);
       ms[7] += now_ms() - t1;
+      if (trace_on())
+        std::fprintf(stderr, "t2: body gather %8.3f ms, packet headers %8.3f ms\n", t2 - t1, now_ms() - t2);
     }
   }
   {

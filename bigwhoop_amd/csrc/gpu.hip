@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -25,6 +26,17 @@ namespace gpu {
 
 static thread_local char g_err[256];
 const char *last_error() { return g_err; }
+
+// BWC_AMD_RC_TRACE=1: host-side phase times of tier-1 on stderr (measurement)
+static double trace_mark(const char *name, double last) {
+  static const bool on = [] {
+    const char *e = std::getenv("BWC_AMD_RC_TRACE");
+    return e && e[0] == '1';
+  }();
+  const double t = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+  if (on && name) std::fprintf(stderr, "t1: %-22s %8.3f ms\n", name, t - last);
+  return t;
+}
 static int process_setup();
 
 #define HIP_OK(x)                                                                                      \
@@ -230,9 +242,25 @@ int fill_minmax(Workspace *ws, const bwc_codec *c, const Tile &tl, int param, co
   if (!ws->d_norm) HIP_OK(hipMalloc((void **)&ws->d_norm, 4 * sizeof(double)));
   const int threads = 256;
   int slots = 0;
-  if (on_dev || !n) {
+  // page-locked host memory the GPU can address: the kernel reads it over
+  // PCIe itself (no staging copy), like a device field
+  const void *src = input;
+  bool direct = on_dev;
+  if (!on_dev && n) {
+    const char *zc = std::getenv("BWC_AMD_FILL_ZEROCOPY");
+    if (!(zc && zc[0] == '0')) {
+      hipPointerAttribute_t attr;
+      if (hipPointerGetAttributes(&attr, input) == hipSuccess && attr.type == hipMemoryTypeHost && attr.devicePointer) {
+        src = attr.devicePointer;
+        direct = true;
+      } else {
+        (void)hipGetLastError();
+      }
+    }
+  }
+  if (direct || !n) {
     const int blocks = (int)std::max<uint64_t>(1, std::min<uint64_t>((n + threads - 1) / threads, 4096));
-    hipLaunchKernelGGL(fill_minmax_kernel, dim3(blocks), dim3(threads), 0, ws->stream, input, prec, nX, nY, nZ, poff,
+    hipLaunchKernelGGL(fill_minmax_kernel, dim3(blocks), dim3(threads), 0, ws->stream, src, prec, nX, nY, nZ, poff,
                        x0, y0, z0, w, h, d, (uint64_t)0, n, work, ws->d_red);
     HIP_OK(hipGetLastError());
     slots = blocks;
@@ -2102,7 +2130,9 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
       zlim = zpred.data();
     }
     HIP_OK(hipMemcpyAsync(rec.K.data(), ws->d_K, ncb, hipMemcpyDeviceToHost, ws->stream));
+    double tm = trace_mark(nullptr, 0);
     HIP_OK(hipStreamSynchronize(ws->stream));
+    tm = trace_mark("bitplanes + net wait", tm);
     if (lim.out && zlim) std::memcpy(lim.out, zlim, ncb);
     std::vector<uint64_t> list;
     std::vector<uint32_t> m;
@@ -2112,7 +2142,9 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
         m.push_back(passes_coded(rec.K[i], zlim, i));
       }
     CHK(v2_code(ws, c, par, tb, rec.K.data(), list, zlim, d_data, bs.bp, d_hist, gp));
+    tm = trace_mark("list + offsets + launch", tm);
     CHK(fetch_rows(ws, rec, list, m, hfl));
+    tm = trace_mark("gen + mq + fetch", tm);
     if (lim.features) {
       // decisions per codeblock: the symbol position where its last coded
       // pass ends (measurement only)
@@ -2376,6 +2408,7 @@ int gather_bodies(Workspace *ws, const std::vector<uint64_t> &desc, const std::v
   }
   if (!n) return 0;
   std::vector<uint64_t> g(desc);
+  CHK(up_begin(ws, g.size() * sizeof(uint64_t) + 256));
   for (uint64_t k = 0; k < n; ++k) {
     const int s = slots[k];
     if (s < 0 || s > 255 || !ws->d_datap[s] || g[3 * k + 2] + g[3 * k + 1] > ws->cap_datap[s] ||
@@ -2388,12 +2421,12 @@ int gather_bodies(Workspace *ws, const std::vector<uint64_t> &desc, const std::v
   size_t cap = ws->cap_gather;
   CHK(ensure((void **)&ws->d_gather, &cap, g.size() * sizeof(uint64_t)));
   ws->cap_gather = cap;
-  HIP_OK(hipMemcpyAsync(ws->d_gather, g.data(), g.size() * sizeof(uint64_t), hipMemcpyHostToDevice, ws->stream));
+  // the descriptors go through the page-locked staging, so nothing here
+  // waits for the copy or the kernel (later work on the stream follows them)
+  CHK(up_copy(ws, ws->d_gather, g.data(), g.size() * sizeof(uint64_t)));
   hipLaunchKernelGGL(gather_abs_kernel, dim3((uint32_t)std::min<uint64_t>(n, 65535)), dim3(256), 0, ws->stream,
                      ws->d_gather, n, ws->d_bodies);
   HIP_OK(hipGetLastError());
-  // the descriptor upload reads g before this returns
-  HIP_OK(hipStreamSynchronize(ws->stream));
   return 0;
 }
 

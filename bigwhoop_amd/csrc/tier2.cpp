@@ -7,6 +7,7 @@
 // :895-1254 (quality layers and the bisection); tagtree.c:310-373;
 // bitstream.c:277-363, :561-612; codestream.c:139-570, :1329-1413.
 #include <algorithm>
+#include <chrono>
 #include <atomic>
 #include <climits>
 #include <cmath>
@@ -257,6 +258,40 @@ struct BitString {
   }
 };
 
+// Bits into preallocated, zeroed words (no growth, so several threads can
+// write their own ranges of one array), and a bit count alone.
+struct BitSpan {
+  uint64_t *p;
+  uint64_t nbits = 0;
+  void bit(uint64_t b) { put(b ? 1 : 0, 1); }
+  void put(uint64_t v, int n) {   // n <= 32
+    if (n <= 0) return;
+    v &= (1ull << n) - 1ull;
+    const size_t w = (size_t)(nbits >> 6);
+    const int room = 64 - (int)(nbits & 63);
+    nbits += (uint64_t)n;
+    if (n <= room) {
+      p[w] |= v << (room - n);
+    } else {
+      p[w] |= v >> (n - room);
+      p[w + 1] |= v << (64 - (n - room));
+    }
+  }
+  void zeros(int n) {
+    if (n > 0) nbits += (uint64_t)n;
+  }
+};
+struct BitTally {
+  uint64_t nbits = 0;
+  void bit(uint64_t) { ++nbits; }
+  void put(uint64_t, int n) {
+    if (n > 0) nbits += (uint64_t)n;
+  }
+  void zeros(int n) {
+    if (n > 0) nbits += (uint64_t)n;
+  }
+};
+
 // Tag-tree coding of one leaf up to `threshold` (tagtree.c:310-373). At each
 // node from the root down the reference emits one bit per threshold step: a
 // 0 while the node value is still >= the raised threshold, then a 1 once it
@@ -427,6 +462,27 @@ struct T2 {
     return (int64_t)w.sink.n;
   }
 
+  // what estimate_packet(.., est 1) adds to the packet's body size, without
+  // the header estimate; returns the body bytes
+  int64_t body_size_only(int j, Resolution &res, uint64_t p, int q) {
+    Packet &pk = res.pk[q * res.nprec + p];
+    if (!pk.e) return 1;
+    CblkRecords &R = rec(j);
+    uint64_t body = 0;
+    for (auto &sb : res.sb) {
+      const Precinct &pr = sb.prec[p];
+      for (uint64_t k = 0; k < pr.ncbs; ++k) {
+        const uint64_t i = pr.first_cb + k;
+        const int16_t *cp = &R.cp[i * (nl + 1) + 1];
+        if (cp[q] - cp[q - 1] == 0) continue;
+        const uint32_t *L = &R.L[i * kMaxPasses];
+        body += (cp[q] == 0 ? 0 : L[cp[q] - 1]) - (cp[q - 1] == 0 ? 0 : L[cp[q - 1] - 1]);
+      }
+    }
+    pk.body_size += body;
+    return (int64_t)body;
+  }
+
   void real_packet(int j, Resolution &res, uint64_t p, int q) {
     Packet &pk = res.pk[q * res.nprec + p];
     pk.built = true;
@@ -592,25 +648,51 @@ struct T2 {
       BitCache &BI = layer_cache.incl[j], &BM = layer_cache.msbs[j];
       BI.woff.assign(R.n, 0), BI.nbits.assign(R.n, 0);
       BM.woff.assign(R.n, 0), BM.nbits.assign(R.n, 0);
+      // the codeblocks not included yet, with their precinct and leaf: the
+      // trees are only read here (estimates), so leaves are coded in
+      // parallel, counted first and then written into their own words
+      struct Leaf {
+        uint64_t i, k;
+        Precinct *pr;
+      };
+      std::vector<Leaf> leaves;
       for (auto &res : par.res)
         for (auto &sb : res.sb)
           for (uint64_t p = 0; p < res.nprec; ++p) {
             Precinct &pr = sb.prec[p];
             for (uint64_t k = 0; k < pr.ncbs; ++k) {
               const uint64_t i = pr.first_cb + k;
-              if (R.cp[i * (nl + 1) + q] != 0) continue;   // cp[q - 1]
-              BI.woff[i] = (uint32_t)BI.words.size();
-              BitString bi{&BI.words};
-              tagtree_encode(pr.incl, bi, (uint32_t)q + 1, k, 2);
-              BI.nbits[i] = (uint32_t)bi.nbits;
-              if (bi.nbits == 0) BI.words.push_back(0);
-              BM.woff[i] = (uint32_t)BM.words.size();
-              BitString bm{&BM.words};
-              tagtree_encode_upto(pr.msbs, bm, R.Kmsbs[i] + 1u, k, 2);
-              BM.nbits[i] = (uint32_t)bm.nbits;
-              if (bm.nbits == 0) BM.words.push_back(0);
+              if (R.cp[i * (nl + 1) + q] == 0) leaves.push_back({i, k, &pr});   // cp[q - 1]
             }
           }
+      par_for(leaves.size(), [&](uint64_t a, uint64_t b) {
+        for (uint64_t x = a; x < b; ++x) {
+          const Leaf &lf = leaves[x];
+          BitTally ti, tm;
+          tagtree_encode(lf.pr->incl, ti, (uint32_t)q + 1, lf.k, 2);
+          tagtree_encode_upto(lf.pr->msbs, tm, R.Kmsbs[lf.i] + 1u, lf.k, 2);
+          BI.nbits[lf.i] = (uint32_t)ti.nbits;
+          BM.nbits[lf.i] = (uint32_t)tm.nbits;
+        }
+      });
+      // words per leaf: at least one (as a leaf with no bits had before)
+      uint64_t wi = 0, wm = 0;
+      for (const Leaf &lf : leaves) {
+        BI.woff[lf.i] = (uint32_t)wi;
+        wi += std::max<uint64_t>(1, (BI.nbits[lf.i] + 63) / 64);
+        BM.woff[lf.i] = (uint32_t)wm;
+        wm += std::max<uint64_t>(1, (BM.nbits[lf.i] + 63) / 64);
+      }
+      BI.words.assign(wi + 1, 0);
+      BM.words.assign(wm + 1, 0);
+      par_for(leaves.size(), [&](uint64_t a, uint64_t b) {
+        for (uint64_t x = a; x < b; ++x) {
+          const Leaf &lf = leaves[x];
+          BitSpan bi{&BI.words[BI.woff[lf.i]]}, bm{&BM.words[BM.woff[lf.i]]};
+          tagtree_encode(lf.pr->incl, bi, (uint32_t)q + 1, lf.k, 2);
+          tagtree_encode_upto(lf.pr->msbs, bm, R.Kmsbs[lf.i] + 1u, lf.k, 2);
+        }
+      });
     }
     lc = &layer_cache;
   }
@@ -650,7 +732,14 @@ struct T2 {
               }
             }
           }
-        for (uint64_t p = 0; p < res.nprec; ++p) ql += estimate_packet(j, res, p, q, est + 1);
+        if (est == 0 && q + 1 == nl) {
+          // the last layer's header estimate is not used by anything after
+          // it (the packets are written for real next); only the body
+          // sizes it would accumulate are
+          for (uint64_t p = 0; p < res.nprec; ++p) ql += body_size_only(j, res, p, q);
+        } else {
+          for (uint64_t p = 0; p < res.nprec; ++p) ql += estimate_packet(j, res, p, q, est + 1);
+        }
       }
     }
     return ql;
@@ -849,13 +938,14 @@ struct T2 {
   // Otherwise the codeblocks are flagged (or the tile, `fallback`) for
   // further coding.
   SkipCheck *chk = nullptr;
-  void check_layer(uint16_t slope, int l, int64_t target, int64_t base) {
+  void check_layer(uint16_t slope, int l, int64_t target, int64_t base, std::unordered_map<uint32_t, int64_t> &m) {
     if (target > 0) {
-      std::unordered_map<uint32_t, int64_t> m;
-      std::vector<uint16_t> Ts{slope};
-      if (slope > 0) Ts.push_back((uint16_t)(slope - 1));
+      // the bisection has usually probed both thresholds already
+      std::vector<uint16_t> Ts;
+      if (!m.count(slope)) Ts.push_back(slope);
+      if (slope > 0 && !m.count(slope - 1u)) Ts.push_back((uint16_t)(slope - 1));
       if (threads > 0) {
-        probe_many(Ts, l, m);
+        if (!Ts.empty()) probe_many(Ts, l, m);
       } else {
         for (uint16_t T : Ts) m[T] = quality_layer(T, l, 1);
       }
@@ -877,32 +967,35 @@ struct T2 {
     //     with Zs = min(Z, U). Slopes below 2^e map below t (slope2log,
     //     tier1.c:2100-2130), so D < 2^e (L[Zs - 1] - L[h - 1]) suffices.
     // Codeblocks failing either are flagged (each once) to be coded further.
+    const uint16_t ts[2] = {slope > 0 ? (uint16_t)(slope - 1) : (uint16_t)0, slope};
+    const double lams[2] = {slope_floor(ts[0]), slope_floor(ts[1])};
     for (int j = 0; j < c->info.nPar; ++j) {
       CblkRecords &R = rec(j);
-      std::vector<uint8_t> seen;
-      for (uint16_t t : {slope > 0 ? (uint16_t)(slope - 1) : (uint16_t)0, slope}) {
-        const double lam = slope_floor(t);
-        for (uint64_t i = 0; i < R.n; ++i) {
+      // per codeblock failing at either threshold: flagged once (in chunks
+      // on the probe threads; the flags are merged in codeblock order)
+      std::vector<uint8_t> bad(R.n, 0);
+      par_for(R.n, [&](uint64_t a, uint64_t b) {
+        for (uint64_t i = a; i < b; ++i) {
           if (R.Z[i] >= R.Zfull[i]) continue;
-          const int h = cut_of(j, R, i, t);
-          const int Zs = std::min<int>(R.Z[i], R.U[i]);
-          bool ok = h + chk->margin <= (int)R.U[i] && h < Zs && i < R.bfut.size();
-          if (ok) {
-            const double *ms = &R.mse[i * kMaxPasses];
-            double D = R.bfut[i];
-            for (int z = h; z < Zs; ++z) D += ms[z];
-            for (int z = Zs; z < R.Z[i]; ++z) D += std::max(ms[z], 0.0);
-            const uint32_t *L = &R.L[i * kMaxPasses];
-            const double dL = (double)L[Zs - 1] - (h > 0 ? (double)L[h - 1] : 0.0);
-            ok = D < lam * dL;
-          }
-          if (!ok) {
-            if (seen.empty()) seen.assign(R.n, 0);
-            if (!seen[i]) chk->flagged[j].push_back(i);
-            seen[i] = 1;
+          for (int k = 0; k < 2 && !bad[i]; ++k) {
+            const int h = cut_of(j, R, i, ts[k]);
+            const int Zs = std::min<int>(R.Z[i], R.U[i]);
+            bool ok = h + chk->margin <= (int)R.U[i] && h < Zs && i < R.bfut.size();
+            if (ok) {
+              const double *ms = &R.mse[i * kMaxPasses];
+              double D = R.bfut[i];
+              for (int z = h; z < Zs; ++z) D += ms[z];
+              for (int z = Zs; z < R.Z[i]; ++z) D += std::max(ms[z], 0.0);
+              const uint32_t *L = &R.L[i * kMaxPasses];
+              const double dL = (double)L[Zs - 1] - (h > 0 ? (double)L[h - 1] : 0.0);
+              ok = D < lams[k] * dL;
+            }
+            if (!ok) bad[i] = 1;
           }
         }
-      }
+      });
+      for (uint64_t i = 0; i < R.n; ++i)
+        if (bad[i]) chk->flagged[j].push_back(i);
     }
   }
 
@@ -917,22 +1010,41 @@ struct T2 {
     return std::ldexp(1.0, e);
   }
 
+  // BWC_AMD_RC_TRACE=1: phase times of rate control on stderr (measurement)
+  bool trace = false;
+  double t_last = 0;
+  void phase(const char *name) {
+    if (!trace) return;
+    const double t = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch())
+                         .count();
+    if (name) std::fprintf(stderr, "rc: %-22s %8.3f ms\n", name, t - t_last);
+    t_last = t;
+  }
+
   void rate_control() {
+    if (const char *e = std::getenv("BWC_AMD_RC_TRACE")) trace = e[0] == '1';
+    phase(nullptr);
     int64_t est_tile = 0, est_ql = 0;
     const uint64_t tsz = tl->box.volume() * (uint64_t)c->info.nPar;
     const uint64_t hdr = (uint64_t)std::ceil((float)((uint32_t)c->control.headerSize + tl->header_size) / nl);
     build_hull_index();
+    phase("hull index");
+    // size estimates of the thresholds probed for the current layer
+    std::unordered_map<uint32_t, int64_t> memo;
     for (int l = 0; l < nl; ++l) {
+      memo.clear();
       const int64_t target = (int64_t)std::ceil(c->control.bitrate[l] * tsz / 8.0f);
       uint16_t slope;
       set_prev_lengths(l);
+      phase("prev lengths");
       if (target > 0.0f) {
         build_layer_cache(l);
+        phase("layer cache");
         uint16_t smin = tl->slope_min, smax = tl->slope_max, tmp = tl->slope_min;
         slope = smin;
         // the reference's bisection, its probes estimated ahead on `threads`
         // host threads: the next three levels of its decision tree at once
-        std::unordered_map<uint32_t, int64_t> memo;
+        memo.clear();
         while (smin < smax) {
           slope = (uint16_t)((smax + smin) >> 1);
           int64_t f;
@@ -941,6 +1053,7 @@ struct T2 {
               std::vector<uint16_t> Ts;
               frontier(smin, smax, threads >= 15 ? 4 : (threads >= 7 ? 3 : (threads >= 3 ? 2 : 1)), Ts);
               probe_many(Ts, l, memo);
+              phase("probe round");
             }
             f = memo[slope];
           } else {
@@ -962,18 +1075,36 @@ struct T2 {
       } else {
         slope = 0;
       }
-      if (chk) check_layer(slope, l, target, est_tile + (int64_t)hdr);
+      if (chk) check_layer(slope, l, target, est_tile + (int64_t)hdr, memo);
+      phase("check layer");
       lc = nullptr;
       est_ql = quality_layer(slope, l, 0);
       est_tile += est_ql + (int64_t)hdr;
+      phase("final layer");
     }
   }
 
+  // the packets of each (parameter, resolution, precinct) in layer order;
+  // different precincts share no state (their codeblocks' beta, their tag
+  // trees, their packets), so they are written on the probe threads
   void packets() {
+    struct Group {
+      int j;
+      Resolution *res;
+      uint64_t p;
+    };
+    std::vector<Group> gs;
     for (int j = 0; j < c->info.nPar; ++j)
       for (auto &res : tl->par[j].res)
-        for (uint64_t p = 0; p < res.nprec; ++p)
-          for (int l = 0; l < nl; ++l) real_packet(j, res, p, l);
+        for (uint64_t p = 0; p < res.nprec; ++p) gs.push_back({j, &res, p});
+    const std::function<void(size_t)> fn = [&](size_t k) {
+      for (int l = 0; l < nl; ++l) real_packet(gs[k].j, *gs[k].res, gs[k].p, l);
+    };
+    if (threads > 1 && gs.size() > 1) {
+      pool_run(threads - 1, gs.size(), fn);
+    } else {
+      for (size_t k = 0; k < gs.size(); ++k) fn(k);
+    }
   }
 };
 
@@ -981,20 +1112,19 @@ struct T2 {
 
 // Rate control for one tile: fills cp (cut points per layer) and the packet
 // contribution flags / body sizes. Needs tile slope bounds and records.
+// host threads for the bisection probes: BWC_AMD_RC_THREADS (0 = the
+// sequential in-place probes, 1 = side-effect-free probes on one thread); by
+// default one less than the host threads, at most 15 (the bisection's next
+// four levels at once)
+static int rc_thread_count() {
+  if (const char *e = std::getenv("BWC_AMD_RC_THREADS")) return std::max(0, std::min(16, std::atoi(e)));
+  const unsigned hc = std::thread::hardware_concurrency();
+  return (int)std::max(1u, std::min(15u, hc > 1 ? hc - 1 : 1u));
+}
+
 int t2_rate_control(bwc_codec *codec, Job *job, uint64_t tile, SkipCheck *chk) {
   T2 t{codec, job, &job->plan.tiles[tile], tile, codec->control.nLayers};
-  // host threads for the bisection probes: BWC_AMD_RC_THREADS (0 = the
-  // sequential in-place probes, 1 = side-effect-free probes on one thread);
-  // by default one less than the host threads, at most 15 (the bisection's
-  // next four levels at once)
-  int rc_threads;
-  if (const char *e = std::getenv("BWC_AMD_RC_THREADS")) {
-    rc_threads = std::max(0, std::min(16, std::atoi(e)));
-  } else {
-    const unsigned hc = std::thread::hardware_concurrency();
-    rc_threads = (int)std::max(1u, std::min(15u, hc > 1 ? hc - 1 : 1u));
-  }
-  t.threads = rc_threads;
+  t.threads = rc_thread_count();
   if (chk) {
     chk->fallback = false;
     chk->flagged.assign(codec->info.nPar, {});
@@ -1012,6 +1142,7 @@ int t2_rate_control(bwc_codec *codec, Job *job, uint64_t tile, SkipCheck *chk) {
 // Packet headers; needs the codeblock data only at assembly time.
 int t2_packets(bwc_codec *codec, Job *job, uint64_t tile) {
   T2 t{codec, job, &job->plan.tiles[tile], tile, codec->control.nLayers};
+  t.threads = rc_thread_count();
   t.packets();
   return 0;
 }
