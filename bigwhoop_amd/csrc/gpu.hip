@@ -1558,15 +1558,79 @@ __device__ void finish_block(uint8_t *buf, uint64_t cap, const uint64_t *sn, con
 //  least K - {0,1,2,4,6,9}   13 log2(gain * step^2)/64   14 log2(bits/value)
 // and in tstat[0..1] the sum and count of log2(rms q + 1) + log2(gain *
 // step^2)/2 (the codeblock's distortion-weighted level).
-__global__ void bitplane_kernel(const double *__restrict__ work, uint64_t W, uint64_t H, uint64_t D,
-                                const uint32_t *__restrict__ cbbox, const double *__restrict__ cbsb,
-                                const uint64_t *__restrict__ bpoff, uint8_t *__restrict__ bp,
-                                uint8_t *__restrict__ Kout, uint64_t ncb, const uint32_t *__restrict__ cbmeta,
-                                float *__restrict__ feat, float *__restrict__ tstat, float rate,
-                                uint32_t *__restrict__ hist_out) {
-  __shared__ int smax[256];
+struct BpGeom {
+  uint64_t W, H, D;
+  uint32_t b0, b1, b2, b3, cw, ch, cd, ns, ncol;
+  double step;
+};
+// The four raw values of stripe column k (row r at index r); values outside
+// the codeblock (rows past its height, columns k >= ncol) are 0.
+__device__ __forceinline__ void bp_load(const double *__restrict__ work, const BpGeom &g, uint32_t k,
+                                        uint64_t raw[4]) {
+  const uint32_t x = k % g.cw, rest = k / g.cw;
+  const uint32_t j = rest % g.ns, sl = rest / g.ns;
+  const uint32_t z = sl % g.cd, t = sl / g.cd;
+  const bool colv = k < g.ncol;
+  // slots outside the codeblock read its first value and drop it
+  const uint64_t first = (uint64_t)g.b0 + g.W * ((uint64_t)g.b1 + g.H * ((uint64_t)g.b2 + g.D * g.b3));
+  const uint64_t base = (uint64_t)(g.b0 + x) + g.W * ((uint64_t)(g.b1 + 4 * j) + g.H * ((uint64_t)(g.b2 + z) +
+                                                                                       g.D * (g.b3 + t)));
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const bool v = colv && 4 * j + r < g.ch;
+    const uint64_t w = (uint64_t)__double_as_longlong(work[v ? base + g.W * r : first]);
+    raw[r] = v ? w : 0;
+  }
+}
+// |value| / step, the quotient that quantisation converts
+__device__ __forceinline__ double bp_quot(uint64_t raw, double step) {
+  return __longlong_as_double((long long)(raw & 0x7FFFFFFFFFFFFFFFull)) / step;
+}
+
+// Bit transpose across the eight lanes of an octet, lane c holding stripe
+// column 8g + c: on entry m[3 - r] is a 32-bit slice of row r's magnitude
+// (value bits as columns), on exit m[i] is the word of plane 4c + i for the
+// octet's four column pairs (byte u = pair 4g + u, bit 3 - r + 4h for row r
+// of column 8g + 2u + h). Exchanges off-diagonal blocks of 16, 8 and 4 bits
+// between lanes (rows 4c + i and 4c + i +- 16, 8, 4), then of 2 and 1 bits
+// within a lane.
+__device__ __forceinline__ void octet_transpose(uint32_t m[4]) {
+#pragma unroll
+  for (int st = 0; st < 3; ++st) {
+    const int s = 16 >> st, x = 4 >> st;
+    const uint32_t mask = st == 0 ? 0x0000FFFFu : st == 1 ? 0x00FF00FFu : 0x0F0F0F0Fu;
+    const bool up = threadIdx.x & x;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t p = __shfl_xor(m[i], x, 64);
+      const uint32_t t = (((up ? p : m[i]) >> s) ^ (up ? m[i] : p)) & mask;
+      m[i] ^= up ? t : t << s;
+    }
+  }
+#pragma unroll
+  for (int s = 2, k = 0; s > 0; s >>= 1, ++k) {
+    const uint32_t mask = k == 0 ? 0x33333333u : 0x55555555u;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (!(i & s)) {
+        const uint32_t t = ((m[i] >> s) ^ m[i + s]) & mask;
+        m[i] ^= t << s;
+        m[i + s] ^= t;
+      }
+  }
+}
+
+__global__ void __launch_bounds__(256) bitplane_kernel(const double *__restrict__ work, uint64_t W, uint64_t H,
+                                                       uint64_t D, const uint32_t *__restrict__ cbbox,
+                                                       const double *__restrict__ cbsb,
+                                                       const uint64_t *__restrict__ bpoff, uint8_t *__restrict__ bp,
+                                                       uint8_t *__restrict__ Kout, uint64_t ncb,
+                                                       const uint32_t *__restrict__ cbmeta, float *__restrict__ feat,
+                                                       float *__restrict__ tstat, float rate,
+                                                       uint32_t *__restrict__ hist_out) {
+  __shared__ int smax[16];
   __shared__ uint32_t hist[65];
-  __shared__ double s1[256], s2[256];
+  __shared__ double s1[16], s2[16];
   const bool want_hist = feat || hist_out;
   for (uint64_t cb = blockIdx.x; cb < ncb; cb += gridDim.x) {
     if (want_hist)
@@ -1574,73 +1638,149 @@ __global__ void bitplane_kernel(const double *__restrict__ work, uint64_t W, uin
     __syncthreads();
     double sq = 0, sqq = 0;
     const uint32_t *b = cbbox + 8 * cb;
-    const uint64_t cw = b[4] - b[0], ch = b[5] - b[1], cd = b[6] - b[2], ct = b[7] - b[3];
-    const uint64_t ns = (ch + 3) / 4;
-    const uint64_t ncol = cw * ns * cd * ct, ncolh = (ncol + 1) / 2;
-    const double step = cbsb[3 * cb];
+    BpGeom g;
+    g.W = W, g.H = H, g.D = D;
+    g.b0 = b[0], g.b1 = b[1], g.b2 = b[2], g.b3 = b[3];
+    g.cw = b[4] - b[0], g.ch = b[5] - b[1], g.cd = b[6] - b[2];
+    const uint32_t ct = b[7] - b[3];
+    g.ns = (g.ch + 3) / 4;
+    g.ncol = g.cw * g.ns * g.cd * ct;
+    g.step = cbsb[3 * cb];
+    const uint32_t ncolh = (g.ncol + 1) / 2;
     uint8_t *planes = bp + bpoff[cb];
-    int kb = 0;
-    for (uint64_t pr = threadIdx.x; pr < ncolh; pr += blockDim.x) {
-      uint64_t q[8];
-      uint32_t sg = 0;
-      for (int h = 0; h < 2; ++h) {
-        const uint64_t k = 2 * pr + h;
-        for (int r = 0; r < 4; ++r) {
-          uint64_t qq = 0;
-          if (k < ncol) {
-            const uint64_t x = k % cw, j = (k / cw) % ns, sl = k / (cw * ns);
-            const uint64_t z = sl % cd, t = sl / cd, row = 4 * j + r;
-            if (row < ch) {
-              const uint64_t idx = (b[0] + x) + W * ((b[1] + row) + H * ((b[2] + z) + D * (b[3] + t)));
-              uint64_t raw = (uint64_t)__double_as_longlong(work[idx]);
-              sg |= (uint32_t)((raw >> 63) << (3 - r + 4 * h));
-              raw &= 0x7FFFFFFFFFFFFFFFull;
-              qq = x86_to_u64(__longlong_as_double((long long)raw) / step);
-              const int nb = qq ? 64 - __clzll((long long)qq) : 0;
-              kb = nb > kb ? nb : kb;
-              if (want_hist) atomicAdd(&hist[nb], 1u);
-              if (feat) {
-                const double dq = (double)qq;
-                sq += dq;
-                sqq += dq * dq;
-              }
-            }
-          }
-          q[4 * h + r] = qq;
+    // thread per stripe column; with a plane stride of whole words the eight
+    // columns of a lane octet (four column pairs) are transposed together and
+    // each lane stores four plane words, else each even lane stores its
+    // pair's plane bytes. Columns past the last form whole octets or pairs
+    // of zero values.
+    const bool octets = (ncolh & 3) == 0;
+    const uint32_t kend = octets ? (g.ncol + 7) & ~7u : 2 * ncolh;
+    // K first: x86_to_u64 is monotone on quotients in [0, 2^64), maps NaN to
+    // 2^63 (bit length 64) and larger quotients to 0, so K is the bit length
+    // of the largest quotient below 2^64, or 64 with a NaN
+    double dmax = 0;
+    bool nan = false;
+#pragma unroll 1
+    for (uint32_t k = threadIdx.x; k < kend; k += blockDim.x) {
+      uint64_t raw[4];
+      bp_load(work, g, k, raw);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double d = bp_quot(raw[r], g.step);
+        nan |= d != d;
+        dmax = d < 18446744073709551616.0 && d > dmax ? d : dmax;
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      const double od = __shfl_xor(dmax, o, 64);
+      dmax = od > dmax ? od : dmax;
+    }
+    nan = __any(nan);
+    const uint32_t wv = threadIdx.x / 64, nw = blockDim.x / 64;
+    if ((threadIdx.x & 63) == 0) {
+      s1[wv] = dmax;
+      smax[wv] = nan;
+    }
+    __syncthreads();
+    for (uint32_t w = 0; w < nw; ++w) {
+      dmax = s1[w] > dmax ? s1[w] : dmax;
+      nan |= smax[w] != 0;
+    }
+    const uint64_t qmax = x86_to_u64(dmax);
+    const int K = __builtin_amdgcn_readfirstlane(nan ? 64 : (qmax ? 64 - __clzll((long long)qmax) : 0));
+    __syncthreads();
+    // quantisation, statistics and the planes: planes 0..K-1 (at most 62)
+    // and the sign plane; planes K..62 are never read
+#pragma unroll 1
+    for (uint32_t k = threadIdx.x; k < kend; k += blockDim.x) {
+      uint64_t raw[4];
+      bp_load(work, g, k, raw);
+      uint32_t lo[4], hi[4], sg = 0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        sg |= (uint32_t)(raw[r] >> 63) << (3 - r);
+        const uint64_t qq = x86_to_u64(bp_quot(raw[r], g.step));
+        lo[r] = (uint32_t)qq;
+        hi[r] = (uint32_t)(qq >> 32);
+        // values outside the codeblock are 0: bit length 0, nothing to the
+        // sums; the histogram takes their count back below
+        if (want_hist) atomicAdd(&hist[qq ? 64 - __clzll((long long)qq) : 0], 1u);
+        if (feat) {
+          const double dq = (double)qq;
+          sq += dq;
+          sqq += dq * dq;
         }
       }
-      for (int pl = 0; pl < 63; ++pl) {
-        uint32_t v = 0;
-        for (int h = 0; h < 2; ++h)
-          for (int r = 0; r < 4; ++r) v |= (uint32_t)((q[4 * h + r] >> pl) & 1) << (3 - r + 4 * h);
-        planes[(uint64_t)pl * ncolh + pr] = (uint8_t)v;
+      // sign plane: the pair's byte from its even column's lane
+      const uint32_t sgo = __shfl_xor(sg, 1, 64);
+      const uint32_t pr = k >> 1;
+      if (!(k & 1)) planes[63 * (uint64_t)ncolh + pr] = (uint8_t)(sg | sgo << 4);
+      if (octets) {
+        const uint32_t c = threadIdx.x & 7;
+        uint8_t *grp = planes + ((k >> 1) & ~3u);
+        uint32_t m[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) m[3 - r] = lo[r];
+        octet_transpose(m);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          if ((int)(4 * c + i) < K) *reinterpret_cast<uint32_t *>(grp + (uint64_t)(4 * c + i) * ncolh) = m[i];
+        if (K > 32) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) m[3 - r] = hi[r];
+          octet_transpose(m);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int pl = 32 + 4 * c + i;
+            if (pl < K && pl < 63) *reinterpret_cast<uint32_t *>(grp + (uint64_t)pl * ncolh) = m[i];
+          }
+        }
+      } else {
+        for (int pl = 0; pl < K && pl < 63; ++pl) {
+          uint32_t v = 0;
+          if (pl < 32) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) v |= ((lo[r] >> pl) & 1u) << (3 - r);
+          } else {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) v |= ((hi[r] >> (pl - 32)) & 1u) << (3 - r);
+          }
+          const uint32_t vo = __shfl_xor(v, 1, 64);
+          if (!(k & 1)) planes[(uint64_t)pl * ncolh + pr] = (uint8_t)(v | vo << 4);
+        }
       }
-      planes[63 * ncolh + pr] = (uint8_t)sg;
     }
-    smax[threadIdx.x] = kb;
-    s1[threadIdx.x] = sq;
-    s2[threadIdx.x] = sqq;
+    const uint32_t nvals = g.cw * g.ch * g.cd * ct;
+    if (want_hist && threadIdx.x == 0) atomicSub(&hist[0], 8 * ncolh - nvals);
+    for (int o = 32; o > 0; o >>= 1) {
+      sq += __shfl_xor(sq, o, 64);
+      sqq += __shfl_xor(sqq, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      s1[wv] = sq;
+      s2[wv] = sqq;
+    }
     __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-      if ((int)threadIdx.x < s) {
-        smax[threadIdx.x] = max(smax[threadIdx.x], smax[threadIdx.x + s]);
-        s1[threadIdx.x] += s1[threadIdx.x + s];
-        s2[threadIdx.x] += s2[threadIdx.x + s];
-      }
-      __syncthreads();
-    }
     if (threadIdx.x == 0) {
-      const int K = smax[0];
       Kout[cb] = (uint8_t)K;
       if (feat) {
         const uint32_t meta = cbmeta[cb];
         const uint32_t hb = meta & 0xFF, Kmax = (meta >> 8) & 0xFF, res = (meta >> 16) & 0xFF;
-        const double nv = (double)(cw * ch * cd * ct);
+        const double nv = (double)nvals;
+        for (uint32_t w = 1; w < nw; ++w) {
+          s1[0] += s1[w];
+          s2[0] += s2[w];
+        }
         double sb = 0;
         uint32_t cum[6] = {0, 0, 0, 0, 0, 0};
+        // (loops kept rolled: one thread runs them, unrolled they would
+        // hold the histogram in registers and set the kernel's register use)
+#pragma unroll 1
         for (int nb = 0; nb <= 64; ++nb) sb += (double)nb * hist[nb];
         const int offs[6] = {0, 1, 2, 4, 6, 9};
+#pragma unroll 1
         for (int o = 0; o < 6; ++o)
+#pragma unroll 1
           for (int nb = K - offs[o] > 1 ? K - offs[o] : 1; nb <= 64; ++nb) cum[o] += hist[nb];
         const double gain = cbsb[3 * cb + 2], stp = cbsb[3 * cb + 1];
         const float lvl_w = (float)(log2(gain) + 2.0 * log2(stp));
