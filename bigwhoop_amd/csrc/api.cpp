@@ -94,8 +94,10 @@ void reset_for_encode(bwc_codec *c, Job *job) {
 }
 
 // tier-2 state of one tile back to "before rate control", so rate control
-// can run again after codeblocks were coded further
-void reset_tile_t2(bwc_codec *c, Job *job, uint64_t ti) {
+// can run again after codeblocks were coded further; slopes: recompute the
+// tile's slope bounds from the records (tier-1 reports them for its own
+// codings, so only codings extended since need it)
+void reset_tile_t2(bwc_codec *c, Job *job, uint64_t ti, bool slopes = true) {
   Tile &tl = job->plan.tiles[ti];
   const int nl = c->control.nLayers;
   for (auto &par : tl.par)
@@ -110,13 +112,16 @@ void reset_tile_t2(bwc_codec *c, Job *job, uint64_t ti) {
         }
     }
   // slope bounds over the current records (nonzero slopes, as tier-1 reports)
-  tl.slope_min = 0xFFFF;
-  tl.slope_max = 0;
+  if (slopes) {
+    tl.slope_min = 0xFFFF;
+    tl.slope_max = 0;
+  }
   for (int p = 0; p < c->info.nPar; ++p) {
     CblkRecords &R = job->rec[ti * c->info.nPar + p];
     R.beta.assign(R.n, 3);
     R.beta_est.assign(R.n, 3);
     R.cp.assign(R.n * (nl + 1), 0);
+    if (!slopes) continue;
     for (uint64_t i = 0; i < R.n; ++i) {
       if (!R.K[i]) continue;
       const uint16_t *S = &R.S[i * (kMaxPasses + 1)];
@@ -185,18 +190,22 @@ int rate_control_prefix(bwc_codec *c, Job *job, gpu::Workspace *ws, uint64_t ti)
   // pass (measurement of the proof's slack; at least 1)
   if (const char *e = std::getenv("BWC_AMD_SKIP_SETTLE")) chk.margin = std::max(1, std::atoi(e));
   for (int round = 0;; ++round) {
-    reset_tile_t2(c, job, ti);
+    double tr = now_ms();
+    reset_tile_t2(c, job, ti, round > 0);
     // every codeblock coded completely: the outcome is exact as it stands
     uint64_t ntrunc = 0;
     for (int p = 0; p < np; ++p) {
       const CblkRecords &R = job->rec[ti * np + p];
       for (uint64_t i = 0; i < R.n; ++i) ntrunc += R.Z[i] < R.Zfull[i];
     }
+    if (trace_on()) std::fprintf(stderr, "rc: reset + count          %8.3f ms\n", now_ms() - tr);
     if (!ntrunc) {
       t2_rate_control(c, job, ti);
       return 0;
     }
+    tr = now_ms();
     t2_rate_control(c, job, ti, &chk);
+    if (trace_on()) std::fprintf(stderr, "rc: t2_rate_control total  %8.3f ms\n", now_ms() - tr);
     uint64_t nflag = 0;
     for (auto &f : chk.flagged) nflag += f.size();
     if (!chk.fallback && !nflag) return 0;

@@ -10,6 +10,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <functional>
 #include <vector>
 
 #include "../../include/bwc.h"
@@ -201,6 +202,11 @@ struct SkipCheck {
 };
 int t2_rate_control(bwc_codec *codec, Job *job, uint64_t tile, SkipCheck *chk = nullptr);
 int t2_packets(bwc_codec *codec, Job *job, uint64_t tile);
+// the host thread pool of rate control: fn(0 .. n-1) on `workers` pool
+// threads and the caller; rc_thread_count() is the rate-control thread count
+// (BWC_AMD_RC_THREADS, else one less than the host threads, at most 15)
+void host_pool_run(int workers, size_t n, const std::function<void(size_t)> &fn);
+int rc_thread_count();
 // Packet bodies already laid out elsewhere (on the device): copy(ctx, dst,
 // dev_off, n) writes the n body bytes of a packet, found at Packet::dev_off,
 // to dst; nonzero return = failure.

@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
+#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -2123,7 +2125,9 @@ static int fetch_rows(Workspace *ws, CblkRecords &rec, const std::vector<uint64_
     HIP_OK(hipMemcpyAsync(ws->h_down, ws->d_rpack, oB + 3 * n, hipMemcpyDeviceToHost, ws->stream));
   }
   HIP_OK(hipMemcpyAsync(ws->h_down + oF, ws->d_flags, 16, hipMemcpyDeviceToHost, ws->stream));
+  double tmf = trace_mark(nullptr, 0);
   HIP_OK(hipStreamSynchronize(ws->stream));
+  tmf = trace_mark("records: wait for the GPU", tmf);
   std::memcpy(hfl, ws->h_down + oF, 16);
   if (hfl[0]) return 0;
   const uint32_t *pL = (const uint32_t *)ws->h_down;
@@ -2132,21 +2136,41 @@ static int fetch_rows(Workspace *ws, CblkRecords &rec, const std::vector<uint64_
   const uint8_t *pb = ws->h_down + oB;
   if (rec.mse.size() != rec.n * kMaxPasses) rec.mse.assign(rec.n * kMaxPasses, 0.0);
   if (rec.bfut.size() != rec.n) rec.bfut.assign(rec.n, 0.0);
-  for (uint64_t r = 0; r < n; ++r) {
-    const uint64_t cb = rows[r];
-    if (pb[3 * r] != m[r]) {
-      std::snprintf(g_err, sizeof g_err, "codeblock %llu: %u passes coded, %u expected", (unsigned long long)cb,
-                    (unsigned)pb[3 * r], m[r]);
-      return 1;
+  // into the per-codeblock record arrays (strided by the pass maximum, so
+  // mostly cache and TLB misses): in chunks on the rate-control pool
+  std::atomic<uint64_t> bad{UINT64_MAX};
+  const size_t nch = (size_t)std::min<uint64_t>(64, (n + 511) / 512);
+  const std::function<void(size_t)> unpack = [&](size_t ch) {
+    for (uint64_t r = n * ch / nch, e = n * (ch + 1) / nch; r < e; ++r) {
+      const uint64_t cb = rows[r];
+      if (pb[3 * r] != m[r]) {
+        uint64_t cur = bad.load();
+        while (r < cur && !bad.compare_exchange_weak(cur, r)) {
+        }
+        continue;
+      }
+      rec.Z[cb] = pb[3 * r];
+      rec.U[cb] = pb[3 * r + 1];
+      rec.Kmsbs[cb] = pb[3 * r + 2];
+      std::memcpy(&rec.L[cb * kMaxPasses], pL + poff[r], m[r] * sizeof(uint32_t));
+      std::memcpy(&rec.S[cb * (kMaxPasses + 1)], pS + poff[r] + r, (m[r] + 1) * sizeof(uint16_t));
+      std::memcpy(&rec.mse[cb * kMaxPasses], pM + poff[r], m[r] * sizeof(double));
+      rec.bfut[cb] = pD[r];
     }
-    rec.Z[cb] = pb[3 * r];
-    rec.U[cb] = pb[3 * r + 1];
-    rec.Kmsbs[cb] = pb[3 * r + 2];
-    std::memcpy(&rec.L[cb * kMaxPasses], pL + poff[r], m[r] * sizeof(uint32_t));
-    std::memcpy(&rec.S[cb * (kMaxPasses + 1)], pS + poff[r] + r, (m[r] + 1) * sizeof(uint16_t));
-    std::memcpy(&rec.mse[cb * kMaxPasses], pM + poff[r], m[r] * sizeof(double));
-    rec.bfut[cb] = pD[r];
+  };
+  const int threads = rc_thread_count();
+  if (nch > 1 && threads > 1) {
+    host_pool_run(threads - 1, nch, unpack);
+  } else {
+    for (size_t ch = 0; ch < nch; ++ch) unpack(ch);
   }
+  if (bad.load() != UINT64_MAX) {
+    const uint64_t r = bad.load();
+    std::snprintf(g_err, sizeof g_err, "codeblock %llu: %u passes coded, %u expected", (unsigned long long)rows[r],
+                  (unsigned)pb[3 * r], m[r]);
+    return 1;
+  }
+  trace_mark("records: unpack", tmf);
   return 0;
 }
 
@@ -2155,10 +2179,12 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
   const uint8_t *zlim = lim.fixed;
   std::vector<uint8_t> zpred;
   const uint64_t ncb = par.ncb;
+  double tm0 = trace_mark(nullptr, 0);
   rec.resize(ncb, c->control.nLayers);
   if (!ncb) return 0;
   CbTables tb;
   build_tables(par, tb);
+  tm0 = trace_mark("records + tables", tm0);
   const std::vector<double> &sb = tb.sb;
   const std::vector<uint32_t> &meta = tb.meta;
   const std::vector<uint64_t> &qoff = tb.qoff, &doff = tb.doff;
@@ -2254,6 +2280,7 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
       bs.cap_hist = cap;
       d_hist = bs.hist;
     }
+    tm0 = trace_mark("uploads + buffers", tm0);
     hipLaunchKernelGGL(bitplane_kernel, dim3((uint32_t)std::min<uint64_t>(ncb, 65535)), dim3(256), 0, ws->stream,
                        work, W, H, D, ws->d_cbbox, ws->d_cbsb, ws->d_bpoff, bs.bp, ws->d_K, ncb, ws->d_cbmeta,
                        d_feat, ws->d_tstat, rate > 0 ? rate : 1.0f, d_hist);

@@ -1116,7 +1116,9 @@ struct T2 {
 // sequential in-place probes, 1 = side-effect-free probes on one thread); by
 // default one less than the host threads, at most 15 (the bisection's next
 // four levels at once)
-static int rc_thread_count() {
+void host_pool_run(int workers, size_t n, const std::function<void(size_t)> &fn) { pool_run(workers, n, fn); }
+
+int rc_thread_count() {
   if (const char *e = std::getenv("BWC_AMD_RC_THREADS")) return std::max(0, std::min(16, std::atoi(e)));
   const unsigned hc = std::thread::hardware_concurrency();
   return (int)std::max(1u, std::min(15u, hc > 1 ? hc - 1 : 1u));
