@@ -412,9 +412,17 @@ struct T2 {
   // header bytes of packet (param j, resolution r, precinct p, layer q)
   template <class W>
   void packet_header(W &w, int j, Resolution &res, uint64_t p, int q, int est, uint64_t *body) {
-    CblkRecords &R = rec(j);
     w.bit(1);
-    for (auto &sb : res.sb) {
+    for (auto &sb : res.sb) subband_bits(w, j, sb, p, q, est, body);
+    w.flush();
+  }
+
+  // the bits one subband's codeblocks add to a packet header (after its
+  // leading 1): inclusion, zero bitplanes, passes and lengths
+  template <class W>
+  void subband_bits(W &w, int j, Subband &sb, uint64_t p, int q, int est, uint64_t *body) {
+    CblkRecords &R = rec(j);
+    {
       Precinct &pr = sb.prec[p];
       for (uint64_t k = 0; k < pr.ncbs; ++k) {
         const uint64_t i = pr.first_cb + k;
@@ -444,7 +452,6 @@ struct T2 {
         }
       }
     }
-    w.flush();
   }
 
   // create_packet for est 1/2: returns the header estimate in bytes
@@ -1087,6 +1094,11 @@ struct T2 {
   // the packets of each (parameter, resolution, precinct) in layer order;
   // different precincts share no state (their codeblocks' beta, their tag
   // trees, their packets), so they are written on the probe threads
+  //
+  // A subband's precinct has its own tag trees, so the header bits of one
+  // packet are the concatenation of independent per-subband bit strings:
+  // those are written on the threads (each subband's layers in order), then
+  // each packet's strings go through the 0xFF-stuffing writer in order.
   void packets() {
     struct Group {
       int j;
@@ -1094,16 +1106,71 @@ struct T2 {
       uint64_t p;
     };
     std::vector<Group> gs;
+    struct Part {
+      size_t g;
+      size_t s;
+    };
+    std::vector<Part> parts;
+    std::vector<size_t> first;   // first part of each group
     for (int j = 0; j < c->info.nPar; ++j)
       for (auto &res : tl->par[j].res)
-        for (uint64_t p = 0; p < res.nprec; ++p) gs.push_back({j, &res, p});
-    const std::function<void(size_t)> fn = [&](size_t k) {
-      for (int l = 0; l < nl; ++l) real_packet(gs[k].j, *gs[k].res, gs[k].p, l);
+        for (uint64_t p = 0; p < res.nprec; ++p) {
+          first.push_back(parts.size());
+          for (size_t sb = 0; sb < res.sb.size(); ++sb) parts.push_back({gs.size(), sb});
+          gs.push_back({j, &res, p});
+        }
+    first.push_back(parts.size());
+    // raw[part * nl + layer]: the part's bits in that layer's packet
+    std::vector<std::vector<uint64_t>> raw(parts.size() * nl);
+    std::vector<uint64_t> rawbits(parts.size() * nl, 0);
+    const std::function<void(size_t)> sub = [&](size_t k) {
+      const Group &g = gs[parts[k].g];
+      Subband &sb = g.res->sb[parts[k].s];
+      for (int l = 0; l < nl; ++l) {
+        if (!g.res->pk[l * g.res->nprec + g.p].e) continue;
+        BitString w(&raw[k * nl + l]);
+        subband_bits(w, g.j, sb, g.p, l, 0, nullptr);
+        rawbits[k * nl + l] = w.nbits;
+      }
     };
-    if (threads > 1 && gs.size() > 1) {
-      pool_run(threads - 1, gs.size(), fn);
+    const std::function<void(size_t)> join = [&](size_t m) {
+      const size_t gi = m / nl;
+      const int l = (int)(m % nl);
+      const Group &g = gs[gi];
+      Packet &pk = g.res->pk[l * g.res->nprec + g.p];
+      pk.built = true;
+      if (!pk.e) {
+        pk.header.assign(1, 0);
+        pk.size = 1;
+        return;
+      }
+      BitWriter w;
+      w.bit(1);
+      for (size_t k = first[gi]; k < first[gi + 1]; ++k) {
+        const std::vector<uint64_t> &words = raw[k * nl + l];
+        uint64_t left = rawbits[k * nl + l];
+        for (size_t i = 0; left; ++i) {
+          const int n = left < 64 ? (int)left : 64;
+          const uint64_t v = words[i] >> (64 - n);
+          if (n > 32) {
+            w.put(v >> 32, n - 32);
+            w.put(v & 0xFFFFFFFFull, 32);
+          } else {
+            w.put(v, n);
+          }
+          left -= (uint64_t)n;
+        }
+      }
+      w.flush();
+      pk.header = std::move(w.sink.out);
+      pk.size = (uint32_t)(pk.body_size + pk.header.size());
+    };
+    if (threads > 1 && parts.size() > 1) {
+      pool_run(threads - 1, parts.size(), sub);
+      pool_run(threads - 1, gs.size() * nl, join);
     } else {
-      for (size_t k = 0; k < gs.size(); ++k) fn(k);
+      for (size_t k = 0; k < parts.size(); ++k) sub(k);
+      for (size_t m = 0; m < gs.size() * nl; ++m) join(m);
     }
   }
 };
