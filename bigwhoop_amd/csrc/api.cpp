@@ -668,10 +668,12 @@ uchar bwc_create_decompression(bwc_codec *const codec, bwc_stream *const stream,
     set_err("bwc_create_decompression: codec and codestream required");
     return 1;
   }
+  const double t0 = now_ms();
   if (parse_codestream(codec, stream, layer)) {
     set_err("invalid codestream");
     return 1;
   }
+  if (trace_on()) std::fprintf(stderr, "dec: parse codestream      %8.3f ms\n", now_ms() - t0);
   return 0;
 }
 
@@ -701,9 +703,16 @@ uchar bwc_decompress(bwc_codec *const codec, bwc_stream *const stream) {
     Tile &tl = job->plan.tiles[ti];
     for (int p = 0; p < inf.nPar; ++p) {
       Param &par = tl.par[p];
+      const double t0 = now_ms();
       if (gpu::t1_decode(ws, codec, par, job->drec[ti * inf.nPar + p], ws->d_work)) goto fail;
+      const double t1 = now_ms();
       if (gpu::dwt_inverse(ws, codec, par, ws->d_work)) goto fail;
       if (gpu::decode_output(ws, codec, tl, p, par, ws->d_work, dst)) goto fail;
+      if (trace_on()) {
+        gpu::sync(ws);
+        std::fprintf(stderr, "dec: tier-1 (host + kernel) %8.3f ms, inverse DWT + output %8.3f ms\n", t1 - t0,
+                     now_ms() - t1);
+      }
     }
   }
   // host output: only the samples a tile wrote cross back (the others keep

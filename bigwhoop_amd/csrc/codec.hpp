@@ -11,6 +11,7 @@
 #include <cstdint>
 #include <string>
 #include <functional>
+#include <memory>
 #include <vector>
 
 #include "../../include/bwc.h"
@@ -96,7 +97,10 @@ struct DecodeRecords {
   std::vector<uint8_t> Kmsbs, Z;
   std::vector<uint32_t> len;
   std::vector<uint64_t> off;   // n + 1
-  std::vector<uint8_t> data;
+  // the codeblocks' bytes back to back (off), 64 bytes of slack after the
+  // last; data_size bytes
+  std::unique_ptr<uint8_t[]> data;
+  uint64_t data_size = 0;
 };
 
 struct Tile {

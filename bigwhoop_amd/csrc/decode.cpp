@@ -202,12 +202,24 @@ int parse_main_header(bwc_codec *codec, bwc_stream *data, Reader &r) {
   }
 }
 
+// one packet body segment of a codeblock: bytes [start, start + n) of its
+// coded data, at src in the codestream
+struct BodyChunk {
+  uint64_t cb;
+  uint32_t start, n;
+  const uint8_t *src;
+};
+std::vector<BodyChunk> &empty_chunks() {
+  static std::vector<BodyChunk> e;
+  return e;
+}
+
 // parse_packet (tier2.c:1364-1634) for packet (param j, resolution res,
 // precinct p, layer l) at `at`; `avail` is the tile's remaining declared
 // body, `real` the bytes there. Returns header + body bytes, or -1.
 int64_t parse_packet(bwc_codec *codec, Job *job, uint64_t ti, int j, Resolution &res, uint64_t p, int l,
                      std::vector<std::vector<int16_t>> &cp, std::vector<std::vector<uint16_t>> &beta,
-                     std::vector<std::vector<uint32_t>> &Lcut, std::vector<std::vector<std::vector<uint8_t>>> &bytes,
+                     std::vector<std::vector<uint32_t>> &Lcut, std::vector<std::vector<BodyChunk>> &chunks,
                      const uint8_t *at, uint64_t avail, uint64_t real) {
   const int nl = codec->control.nLayers;
   DecodeRecords &D = job->drec[ti * codec->info.nPar + j];
@@ -290,9 +302,7 @@ int64_t parse_packet(bwc_codec *codec, Job *job, uint64_t ti, int j, Resolution 
         const uint32_t start = (c[l - 1] == 0) ? 0 : L[c[l - 1] - 1];
         const uint32_t end = (c[l] == 0) ? 0 : L[c[l] - 1];
         if (end < start || hsize + bsize + (end - start) > real) return -1;
-        std::vector<uint8_t> &b = bytes[j][i];
-        if (b.size() < end) b.resize(end, 0);
-        std::memcpy(b.data() + start, body + bsize, end - start);
+        if (end > start) chunks[j].push_back({i, start, end - start, body + bsize});
         bsize += end - start;
       }
     }
@@ -300,13 +310,13 @@ int64_t parse_packet(bwc_codec *codec, Job *job, uint64_t ti, int j, Resolution 
 }
 
 // Per-tile packet state: cut passes per layer, beta, lengths at the cuts,
-// each codeblock's bytes as assembled so far.
+// and per parameter the body chunks read so far (in codestream order).
 struct TileState {
   bool seen = false;
   std::vector<std::vector<int16_t>> cp;
   std::vector<std::vector<uint16_t>> beta;
   std::vector<std::vector<uint32_t>> Lcut;
-  std::vector<std::vector<std::vector<uint8_t>>> bytes;
+  std::vector<std::vector<BodyChunk>> chunks;
 };
 
 // one tile (parse_tile, codestream.c:1054-1200). Returns 0, 1 when the
@@ -326,13 +336,12 @@ int parse_tile(bwc_codec *codec, Reader &r, std::vector<TileState> &ts) {
   TileState &st = ts[ti];
   if (!st.seen) {
     st.seen = true;
-    st.cp.assign(np, {}), st.beta.assign(np, {}), st.Lcut.assign(np, {}), st.bytes.assign(np, {});
+    st.cp.assign(np, {}), st.beta.assign(np, {}), st.Lcut.assign(np, {}), st.chunks.assign(np, {});
     for (int j = 0; j < np; ++j) {
       const uint64_t n = tl.par[j].ncb;
       st.cp[j].assign(n * (nl + 1), 0);
       st.beta[j].assign(n, 3);
       st.Lcut[j].assign(n * kMaxPasses, 0);
-      st.bytes[j].assign(n, {});
     }
   }
   const bool rlcp = ctl.progression == bwc_prog_RLCP;
@@ -357,7 +366,7 @@ int parse_tile(bwc_codec *codec, Reader &r, std::vector<TileState> &ts) {
             r.L -= 2;
           }
           if (r.err) return -1;
-          const int64_t sz = parse_packet(codec, job, ti, j, res, p, l, st.cp, st.beta, st.Lcut, st.bytes,
+          const int64_t sz = parse_packet(codec, job, ti, j, res, p, l, st.cp, st.beta, st.Lcut, st.chunks,
                                           r.m + r.L, body, r.cap - r.L);
           if (sz < 0) return -1;
           r.L += (uint64_t)sz;
@@ -372,7 +381,8 @@ int parse_tile(bwc_codec *codec, Reader &r, std::vector<TileState> &ts) {
 int parse_body(bwc_codec *codec, Reader &r) {
   Job *job = job_of(codec);
   const int np = codec->info.nPar;
-  job->drec.assign(job->plan.tiles.size() * np, DecodeRecords{});
+  job->drec.clear();
+  job->drec.resize(job->plan.tiles.size() * np);
   for (size_t ti = 0; ti < job->plan.tiles.size(); ++ti)
     for (int j = 0; j < np; ++j) {
       DecodeRecords &D = job->drec[ti * np + j];
@@ -423,12 +433,40 @@ int parse_body(bwc_codec *codec, Reader &r) {
         tot += D.len[i];
       }
       D.off[n] = tot;
-      D.data.assign(tot + 64, 0);   // slack: the decoder reads a word past the end
-      for (uint64_t i = 0; ts[ti].seen && i < n; ++i)
-        if (D.len[i]) {
-          const std::vector<uint8_t> &b = ts[ti].bytes[j][i];
-          std::memcpy(D.data.data() + D.off[i], b.data(), std::min<size_t>(D.len[i], b.size()));
+      // each codeblock's bytes [0, len): zero, then its chunks in the order
+      // they were read (a later one overwrites an earlier one where they
+      // meet, bytes past len are dropped), as the reference assembles them;
+      // codeblocks in parallel on the host pool, the 64 bytes of slack (the
+      // decoder reads a word past the end) zeroed
+      D.data_size = tot + 64;
+      D.data.reset(new uint8_t[D.data_size]);
+      std::memset(D.data.get() + tot, 0, 64);
+      std::vector<BodyChunk> &ch = ts[ti].seen ? ts[ti].chunks[j] : empty_chunks();
+      std::vector<uint64_t> first(n + 1, 0);
+      for (const BodyChunk &c : ch) first[c.cb + 1]++;
+      for (uint64_t i = 0; i < n; ++i) first[i + 1] += first[i];
+      std::vector<uint32_t> order(ch.size());
+      {
+        std::vector<uint64_t> at(first.begin(), first.end() - 1);
+        for (uint32_t k = 0; k < (uint32_t)ch.size(); ++k) order[at[ch[k].cb]++] = k;
+      }
+      const size_t nparts = (size_t)std::min<uint64_t>(64, (n + 255) / 256);
+      const std::function<void(size_t)> fill = [&](size_t part) {
+        for (uint64_t i = n * part / nparts, e = n * (part + 1) / nparts; i < e; ++i) {
+          uint8_t *dst = D.data.get() + D.off[i];
+          std::memset(dst, 0, D.len[i]);
+          for (uint64_t q = first[i]; q < first[i + 1]; ++q) {
+            const BodyChunk &c = ch[order[q]];
+            if (c.start < D.len[i]) std::memcpy(dst + c.start, c.src, std::min<uint64_t>(c.n, D.len[i] - c.start));
+          }
         }
+      };
+      const int threads = rc_thread_count();
+      if (nparts > 1 && threads > 1) {
+        host_pool_run(threads - 1, nparts, fill);
+      } else {
+        for (size_t part = 0; part < nparts; ++part) fill(part);
+      }
     }
   return 0;
 }

@@ -30,13 +30,16 @@ static thread_local char g_err[256];
 const char *last_error() { return g_err; }
 
 // BWC_AMD_RC_TRACE=1: host-side phase times of tier-1 on stderr (measurement)
-static double trace_mark(const char *name, double last) {
+static bool trace_enabled() {
   static const bool on = [] {
     const char *e = std::getenv("BWC_AMD_RC_TRACE");
     return e && e[0] == '1';
   }();
+  return on;
+}
+static double trace_mark(const char *name, double last) {
   const double t = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
-  if (on && name) std::fprintf(stderr, "t1: %-22s %8.3f ms\n", name, t - last);
+  if (trace_enabled() && name) std::fprintf(stderr, "t1: %-22s %8.3f ms\n", name, t - last);
   return t;
 }
 static int process_setup();
@@ -1812,6 +1815,7 @@ __global__ void __launch_bounds__(256) bitplane_kernel(const double *__restrict_
 
 #include "t1v2.inc"
 #include "t1dec.inc"
+#include "t1dec2.inc"
 
 __global__ void zero_f32_kernel(float *p, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.f;
@@ -2466,7 +2470,7 @@ int t1_decode(Workspace *ws, const bwc_codec *c, const Param &par, const DecodeR
   CHK(up((void **)&ws->dec.Z, &ws->dec.cap_Z, rec.Z.data(), ncb));
   CHK(up((void **)&ws->dec.len, &ws->dec.cap_len, rec.len.data(), ncb * sizeof(uint32_t)));
   CHK(up((void **)&ws->dec.off, &ws->dec.cap_off, rec.off.data(), (ncb + 1) * sizeof(uint64_t)));
-  CHK(up((void **)&ws->dec.data, &ws->dec.cap_data, rec.data.data(), rec.data.size()));
+  CHK(up((void **)&ws->dec.data, &ws->dec.cap_data, rec.data.get(), rec.data_size));
   CHK(up((void **)&ws->dec.soff, &ws->dec.cap_soff, soff.data(), (ncb + 1) * sizeof(uint64_t)));
   CHK(ensure((void **)&ws->dec.state, &ws->dec.cap_state, (soff[ncb] + 1) * sizeof(uint16_t)));
   const uint64_t W = par.box.len(0), H = par.box.len(1), D = par.box.len(2);
@@ -2474,10 +2478,34 @@ int t1_decode(Workspace *ws, const bwc_codec *c, const Param &par, const DecodeR
   // so more wavefronts with fewer lanes hide more of the latency (512^3,
   // 2 bpv: 1 -> 137 ms, 2 -> 134, 4 -> 155, 8 -> 187, 16 -> 205 ms per
   // decompression)
-  constexpr int LPW = 2;
-  hipLaunchKernelGGL(t1_decode_kernel<LPW>, dim3((uint32_t)((ncb + LPW - 1) / LPW)), dim3(64), 0, ws->stream, ncb,
-                     ws->dec.box, ws->dec.sb, ws->dec.meta, ws->dec.K, ws->dec.Z, ws->dec.len, ws->dec.off,
-                     ws->dec.data, ws->dec.soff, ws->dec.state, W, H, D, work, (int)c->control.error_resilience);
+  uint32_t ncolmax = 0;
+  for (uint64_t i = 0; i < ncb; ++i) ncolmax = std::max<uint32_t>(ncolmax, (uint32_t)(soff[i + 1] - soff[i]));
+  if (trace_enabled()) {
+    const double tm = trace_mark(nullptr, 0);
+    HIP_OK(hipStreamSynchronize(ws->stream));
+    trace_mark("dec: uploads wait", tm);
+  }
+  // the LDS-state decoder when every lane's column states fit 32 KB
+  int lanes = 2;
+  if (const char *e = std::getenv("BWC_AMD_DEC_LANES")) lanes = std::max(1, std::min(4, std::atoi(e)));
+  if (lanes == 3) lanes = 2;
+  const size_t lds = (size_t)lanes * ncolmax * sizeof(uint16_t);
+  const char *force = std::getenv("BWC_AMD_DEC_LDS");
+  if (lds <= 32 * 1024 && !(force && force[0] == '0')) {
+#define DEC_LAUNCH(N)                                                                                              \
+  hipLaunchKernelGGL(t1_decode_lds_kernel<N>, dim3((uint32_t)((ncb + N - 1) / N)), dim3(64), lds, ws->stream, ncb, \
+                     ws->dec.box, ws->dec.sb, ws->dec.meta, ws->dec.K, ws->dec.Z, ws->dec.len, ws->dec.off,          \
+                     ws->dec.data, ncolmax, W, H, D, work, (int)c->control.error_resilience)
+    if (lanes == 1) DEC_LAUNCH(1);
+    else if (lanes == 4) DEC_LAUNCH(4);
+    else DEC_LAUNCH(2);
+#undef DEC_LAUNCH
+  } else {
+    constexpr int LPW = 2;
+    hipLaunchKernelGGL(t1_decode_kernel<LPW>, dim3((uint32_t)((ncb + LPW - 1) / LPW)), dim3(64), 0, ws->stream, ncb,
+                       ws->dec.box, ws->dec.sb, ws->dec.meta, ws->dec.K, ws->dec.Z, ws->dec.len, ws->dec.off,
+                       ws->dec.data, ws->dec.soff, ws->dec.state, W, H, D, work, (int)c->control.error_resilience);
+  }
   HIP_OK(hipGetLastError());
   // the host records are read by the uploads above before this returns
   HIP_OK(hipStreamSynchronize(ws->stream));
