@@ -26,8 +26,12 @@ ARCH = "gfx950"
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-Wall",
           "-Wno-unused-function", "-Wno-unused-variable", "-I" + str(PKG.parent / "include")]
+# -structurizecfg-skip-uniform-regions: branches whose condition is the same
+# on every lane stay scalar branches instead of being rewritten into
+# exec-masked flow blocks; the tier-1 decoder's control flow is uniform and
+# halves in size with it
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-             "-munsafe-fp-atomics"]
+             "-munsafe-fp-atomics", "-mllvm", "-structurizecfg-skip-uniform-regions=true"]
 
 
 def _compile(src: Path) -> Path:

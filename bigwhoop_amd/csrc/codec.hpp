@@ -97,8 +97,8 @@ struct DecodeRecords {
   std::vector<uint8_t> Kmsbs, Z;
   std::vector<uint32_t> len;
   std::vector<uint64_t> off;   // n + 1
-  // the codeblocks' bytes back to back (off), 64 bytes of slack after the
-  // last; data_size bytes
+  // the codeblocks' bytes back to back (off), 512 zeroed bytes of slack
+  // after the last; data_size bytes
   std::unique_ptr<uint8_t[]> data;
   uint64_t data_size = 0;
 };

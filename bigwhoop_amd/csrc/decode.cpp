@@ -436,11 +436,11 @@ int parse_body(bwc_codec *codec, Reader &r) {
       // each codeblock's bytes [0, len): zero, then its chunks in the order
       // they were read (a later one overwrites an earlier one where they
       // meet, bytes past len are dropped), as the reference assembles them;
-      // codeblocks in parallel on the host pool, the 64 bytes of slack (the
-      // decoder reads a word past the end) zeroed
-      D.data_size = tot + 64;
+      // codeblocks in parallel on the host pool. 512 bytes of zeroed slack:
+      // the decoders read 256-byte chunks up to one chunk past the end.
+      D.data_size = tot + 512;
       D.data.reset(new uint8_t[D.data_size]);
-      std::memset(D.data.get() + tot, 0, 64);
+      std::memset(D.data.get() + tot, 0, 512);
       std::vector<BodyChunk> &ch = ts[ti].seen ? ts[ti].chunks[j] : empty_chunks();
       std::vector<uint64_t> first(n + 1, 0);
       for (const BodyChunk &c : ch) first[c.cb + 1]++;

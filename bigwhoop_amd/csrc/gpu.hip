@@ -1816,6 +1816,7 @@ __global__ void __launch_bounds__(256) bitplane_kernel(const double *__restrict_
 #include "t1v2.inc"
 #include "t1dec.inc"
 #include "t1dec2.inc"
+#include "t1dec3.inc"
 
 __global__ void zero_f32_kernel(float *p, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.f;
@@ -2484,6 +2485,37 @@ int t1_decode(Workspace *ws, const bwc_codec *c, const Param &par, const DecodeR
     const double tm = trace_mark(nullptr, 0);
     HIP_OK(hipStreamSynchronize(ws->stream));
     trace_mark("dec: uploads wait", tm);
+  }
+  // the scalar decoder (one codeblock per wavefront, state in registers)
+  // when every codeblock is at most 64 columns wide and has at most 256
+  // stripe rows (stripes x slices)
+  uint32_t cwmax = 0, nrowmax = 0;
+  for (uint64_t i = 0; i < ncb; ++i) {
+    const uint32_t *bx = par.cb_box.data() + 8 * i;
+    cwmax = std::max(cwmax, bx[4] - bx[0]);
+    nrowmax = std::max(nrowmax, ((bx[5] - bx[1] + 3) / 4) * (bx[6] - bx[2]) * (bx[7] - bx[3]));
+  }
+  const char *force_sc = std::getenv("BWC_AMD_DEC_SC");
+  if (cwmax <= 64 && nrowmax <= 256 && !(force_sc && force_sc[0] == '0')) {
+    const uint32_t *d32 = reinterpret_cast<const uint32_t *>(ws->dec.data);
+#define SC_LAUNCH(W64, NB)                                                                                          \
+  hipLaunchKernelGGL((t1_decode_sc_kernel<W64, NB>), dim3((uint32_t)ncb), dim3(64), 0, ws->stream, ncb, ws->dec.box,  \
+                     ws->dec.sb, ws->dec.meta, ws->dec.K, ws->dec.Z, ws->dec.len, ws->dec.off, d32, W, H, D, work,    \
+                     (int)c->control.error_resilience)
+    const int nb = nrowmax <= 64 ? 1 : (nrowmax <= 128 ? 2 : 4);
+    if (cwmax <= 32) {
+      if (nb == 1) SC_LAUNCH(false, 1);
+      else if (nb == 2) SC_LAUNCH(false, 2);
+      else SC_LAUNCH(false, 4);
+    } else {
+      if (nb == 1) SC_LAUNCH(true, 1);
+      else if (nb == 2) SC_LAUNCH(true, 2);
+      else SC_LAUNCH(true, 4);
+    }
+#undef SC_LAUNCH
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(ws->stream));
+    return 0;
   }
   // the LDS-state decoder when every lane's column states fit 32 KB
   int lanes = 2;
