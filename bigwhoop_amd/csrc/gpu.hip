@@ -66,7 +66,7 @@ Workspace::~Workspace() {
                   d_state, d_data,  d_K,    d_Kmsbs, d_Z,    d_L,    d_S,      d_snap,   d_mse,  d_flags,
                   d_packed, d_gather, d_bp, d_bpoff, d_sym, d_symoff, d_pcnt, d_pmse, d_list, d_U, d_zlim,
                   d_feat,  d_tstat, d_skipw, d_bodies, dec.box, dec.sb, dec.meta, dec.K, dec.Z, dec.len,
-                  dec.off, dec.data, dec.soff, dec.state, dec.out, d_poff, d_rpack, d_bfut};
+                  dec.off, dec.data, dec.soff, dec.state, dec.out, d_poff, d_rpack, d_bfut, d_t1info};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (auto &b : bp_slots) {
@@ -82,6 +82,7 @@ Workspace::~Workspace() {
   }
   if (h_up) (void)hipHostFree(h_up);
   if (h_down) (void)hipHostFree(h_down);
+  if (h_kz) (void)hipHostFree(h_kz);
   for (auto &e : ev)
     if (e) (void)hipEventDestroy(e);
   if (copy_stream) (void)hipStreamSynchronize(copy_stream);
@@ -1973,6 +1974,79 @@ static uint64_t stream_offsets(const Param &par, const CbTables &t, const uint8_
   return so;
 }
 
+// The same on the device, with the list of codeblocks to code: one workgroup
+// of 1024 threads, each over a contiguous range of codeblocks, then a
+// workgroup scan of the ranges' sums. Writes, in codeblock order, list[]
+// (codeblocks with K > 0), poff[] (their first coded pass in the packed
+// records, as fetch_rows lays them out), symoff[] (every codeblock's symbol
+// stream offset, as stream_offsets) and info = {list length, symbol bytes,
+// passes, overflow}: bit 0 of overflow when the symbols exceed cap_sym
+// bytes, bit 1 when the passes exceed cap_pass; the coding kernels then do
+// nothing and the host grows the buffers and launches again.
+__device__ __forceinline__ uint32_t dev_passes_coded(uint32_t K, const uint8_t *zlim, uint64_t cb) {
+  if (K == 0) return 0;
+  const uint32_t full = 3 * K - 2;
+  return (zlim && zlim[cb] < full) ? (zlim[cb] > 0 ? zlim[cb] : 1u) : full;
+}
+__device__ __forceinline__ uint64_t dev_sym_bytes(uint64_t i, uint32_t K, const uint8_t *zlim,
+                                                  const uint32_t *cbbox, const uint64_t *qoff) {
+  const uint32_t *b = cbbox + 8 * i;
+  const uint64_t nv = (uint64_t)(b[4] - b[0]) * (b[5] - b[1]) * (b[6] - b[2]) * (b[7] - b[3]);
+  const uint64_t ncol = (qoff[i + 1] - qoff[i]) / 4;
+  uint64_t planes = K;
+  if (zlim && K && zlim[i] < 3u * K - 2) planes = (zlim[i] + 2) / 3 + 1;
+  return (nv * (planes + 1) + 3 * ncol * planes + 4 * planes + 32 + 15) & ~15ull;
+}
+__global__ void __launch_bounds__(1024) t1_list_kernel(uint64_t ncb, const uint8_t *__restrict__ Kp,
+                                                       const uint8_t *__restrict__ zlim,
+                                                       const uint32_t *__restrict__ cbbox,
+                                                       const uint64_t *__restrict__ qoff, uint64_t cap_sym,
+                                                       uint64_t cap_pass, uint64_t *__restrict__ list,
+                                                       uint64_t *__restrict__ symoff, uint64_t *__restrict__ poff,
+                                                       uint64_t *__restrict__ info) {
+  __shared__ uint64_t ssym[1024], scnt[1024], spas[1024];
+  const uint32_t t = threadIdx.x;
+  const uint64_t a = ncb * t / 1024, e = ncb * (t + 1) / 1024;
+  uint64_t sy = 0, cn = 0, pa = 0;
+  for (uint64_t i = a; i < e; ++i) {
+    const uint32_t K = Kp[i];
+    sy += dev_sym_bytes(i, K, zlim, cbbox, qoff);
+    if (K) {
+      ++cn;
+      pa += dev_passes_coded(K, zlim, i);
+    }
+  }
+  ssym[t] = sy, scnt[t] = cn, spas[t] = pa;
+  __syncthreads();
+  // inclusive scan of the three sums
+  for (uint32_t d = 1; d < 1024; d <<= 1) {
+    const uint64_t y0 = t >= d ? ssym[t - d] : 0, y1 = t >= d ? scnt[t - d] : 0, y2 = t >= d ? spas[t - d] : 0;
+    __syncthreads();
+    ssym[t] += y0, scnt[t] += y1, spas[t] += y2;
+    __syncthreads();
+  }
+  uint64_t so = ssym[t] - sy, c = scnt[t] - cn, po = spas[t] - pa;
+  for (uint64_t i = a; i < e; ++i) {
+    const uint32_t K = Kp[i];
+    symoff[i] = so;
+    so += dev_sym_bytes(i, K, zlim, cbbox, qoff);
+    if (K) {
+      list[c] = i;
+      poff[c] = po;
+      ++c;
+      po += dev_passes_coded(K, zlim, i);
+    }
+  }
+  if (t == 1023) {
+    symoff[ncb] = ssym[t];
+    poff[scnt[t]] = spas[t];
+    info[0] = scnt[t];
+    info[1] = ssym[t];
+    info[2] = spas[t];
+    info[3] = (ssym[t] + 64 > cap_sym ? 1u : 0u) | (spas[t] > cap_pass ? 2u : 0u);
+  }
+}
+
 // LDS of the symbol generator: per codeblock it packs P = 64 / pow2ceil(width)
 // slices side by side, each with its own pass-output buffer; the buffers take
 // the largest, over the codeblocks, of P times the codeblock's per-slice bound
@@ -1997,17 +2071,11 @@ static GenParams gen_params(const Param &par) {
   return g;
 }
 
-// Symbol generation and MQ coding of the codeblocks in `list`; the
-// (tile, param) tables, K and bitplane offsets are on the device already.
-// zlim: per-codeblock pass limits (host array of ncb) or null for all passes.
-static int v2_code(Workspace *ws, const bwc_codec *c, const Param &par, const CbTables &tb, const uint8_t *Kh,
-                   const std::vector<uint64_t> &list, const uint8_t *zlim, uint8_t *d_data, const uint8_t *d_bp,
-                   const uint32_t *d_hist, const GenParams &gp) {
-  const uint64_t ncb = par.ncb;
-  std::vector<uint64_t> symoff;
-  const uint64_t so = stream_offsets(par, tb, Kh, zlim, symoff);
+// Buffers of symbol generation and MQ coding: the symbol streams (sym_bytes
+// bytes), per-codeblock stream offsets, pass ends and distortions, the list.
+static int v2_buffers(Workspace *ws, uint64_t ncb, uint64_t sym_bytes, uint64_t nlist) {
   size_t cap = ws->cap_sym;
-  CHK(ensure((void **)&ws->d_sym, &cap, so + 64));
+  CHK(ensure((void **)&ws->d_sym, &cap, sym_bytes));
   ws->cap_sym = cap;
   cap = ws->cap_symoff;
   CHK(ensure((void **)&ws->d_symoff, &cap, (ncb + 1) * sizeof(uint64_t)));
@@ -2019,46 +2087,63 @@ static int v2_code(Workspace *ws, const bwc_codec *c, const Param &par, const Cb
   CHK(ensure((void **)&ws->d_pmse, &cap, (ncb + 1) * kMaxPasses * sizeof(int64_t)));
   ws->cap_pm = cap;
   cap = ws->cap_list;
-  CHK(ensure((void **)&ws->d_list, &cap, (list.size() + 1) * sizeof(uint64_t)));
+  CHK(ensure((void **)&ws->d_list, &cap, (nlist + 1) * sizeof(uint64_t)));
   ws->cap_list = cap;
-  CHK(up_copy(ws, ws->d_symoff, symoff.data(), (ncb + 1) * sizeof(uint64_t)));
-  if (zlim) CHK(up_copy(ws, ws->d_zlim, zlim, ncb));
-  const uint8_t *dz = zlim ? ws->d_zlim : nullptr;
-  if (list.empty()) return 0;
+  return 0;
+}
+
+// Symbol generation and MQ coding of the codeblocks in ws->d_list: grid for
+// `nl` entries; info (device) holds the list's real length when the list
+// was built on the device, else null (nl entries). dz: the pass limits on
+// the device or null.
+static int v2_launch(Workspace *ws, const bwc_codec *c, uint64_t nl, const uint8_t *dz, uint8_t *d_data,
+                     const uint8_t *d_bp, const uint32_t *d_hist, const GenParams &gp, const uint64_t *info) {
+  if (!nl) return 0;
   const int erres = (int)c->control.error_resilience;
   // codeblocks per wavefront of the MQ kernel (lanes beyond them idle), for
   // complete codings and for codings to a pass limit (tuning.hpp); halved
   // while fewer than 256 wavefronts (one per CU) would run, since lanes of
   // one wavefront share its issue slots and a short list leaves CUs idle
-  int lpw = zlim ? tuning().mq_lanes_limited : tuning().mq_lanes_full;
-  while (lpw > 1 && (list.size() + lpw - 1) / lpw < 256) lpw >>= 1;
-  auto launch_mq = [&](const uint64_t *dl, uint32_t nl, hipStream_t s, int lanes) -> int {
-#define MQ_LAUNCH(N)                                                                                           \
-  hipLaunchKernelGGL(t1_mq_kernel<N>, dim3((nl + N - 1) / N), dim3(64), 0, s, dl, (uint64_t)nl, ws->d_cbmeta, \
-                     ws->d_cbsb, ws->d_K, ws->d_symoff, ws->d_sym, ws->d_pcnt, ws->d_pmse, ws->d_doff, d_data,    \
-                     ws->d_Kmsbs, ws->d_Z, ws->d_L, ws->d_S, ws->d_snap, ws->d_mse, ws->d_flags, dz, ws->d_U,    \
-                     d_hist, ws->d_bfut)
-    switch (lanes) {
-      case 1: MQ_LAUNCH(1); break;
-      case 2: MQ_LAUNCH(2); break;
-      case 4: MQ_LAUNCH(4); break;
-      case 8: MQ_LAUNCH(8); break;
-      case 32: MQ_LAUNCH(32); break;
-      default: MQ_LAUNCH(16); break;
-    }
+  int lpw = dz ? tuning().mq_lanes_limited : tuning().mq_lanes_full;
+  while (lpw > 1 && (nl + lpw - 1) / lpw < 256) lpw >>= 1;
+  const uint64_t *dl = ws->d_list;
+  hipLaunchKernelGGL(t1_gen_kernel, dim3((uint32_t)nl), dim3(64), gp.lds, ws->stream, dl, ws->d_cbbox, ws->d_cbmeta,
+                     ws->d_K, ws->d_bpoff, d_bp, ws->d_symoff, ws->d_sym, ws->d_pcnt, ws->d_pmse, ws->d_flags, erres,
+                     gp.maxcol, gp.obuf_bytes, dz, info);
+  HIP_OK(hipGetLastError());
+#define MQ_LAUNCH(N)                                                                                                 \
+  hipLaunchKernelGGL(t1_mq_kernel<N>, dim3((uint32_t)((nl + N - 1) / N)), dim3(64), 0, ws->stream, dl, nl,           \
+                     ws->d_cbmeta, ws->d_cbsb, ws->d_K, ws->d_symoff, ws->d_sym, ws->d_pcnt, ws->d_pmse, ws->d_doff, \
+                     d_data, ws->d_Kmsbs, ws->d_Z, ws->d_L, ws->d_S, ws->d_snap, ws->d_mse, ws->d_flags, dz, ws->d_U, \
+                     d_hist, ws->d_bfut, info)
+  switch (lpw) {
+    case 1: MQ_LAUNCH(1); break;
+    case 2: MQ_LAUNCH(2); break;
+    case 4: MQ_LAUNCH(4); break;
+    case 8: MQ_LAUNCH(8); break;
+    case 32: MQ_LAUNCH(32); break;
+    default: MQ_LAUNCH(16); break;
+  }
 #undef MQ_LAUNCH
-    HIP_OK(hipGetLastError());
-    return 0;
-  };
-  auto launch = [&](const uint64_t *dl, uint32_t nl, hipStream_t s, int lanes) -> int {
-    hipLaunchKernelGGL(t1_gen_kernel, dim3(nl), dim3(64), gp.lds, s, dl, ws->d_cbbox, ws->d_cbmeta, ws->d_K,
-                       ws->d_bpoff, d_bp, ws->d_symoff, ws->d_sym, ws->d_pcnt, ws->d_pmse, ws->d_flags, erres,
-                       gp.maxcol, gp.obuf_bytes, dz);
-    HIP_OK(hipGetLastError());
-    return launch_mq(dl, nl, s, lanes);
-  };
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+// Symbol generation and MQ coding of the codeblocks in `list` (host); the
+// (tile, param) tables, K and bitplane offsets are on the device already.
+// zlim: per-codeblock pass limits (host array of ncb) or null for all passes.
+static int v2_code(Workspace *ws, const bwc_codec *c, const Param &par, const CbTables &tb, const uint8_t *Kh,
+                   const std::vector<uint64_t> &list, const uint8_t *zlim, uint8_t *d_data, const uint8_t *d_bp,
+                   const uint32_t *d_hist, const GenParams &gp) {
+  const uint64_t ncb = par.ncb;
+  std::vector<uint64_t> symoff;
+  const uint64_t so = stream_offsets(par, tb, Kh, zlim, symoff);
+  CHK(v2_buffers(ws, ncb, so + 64, list.size()));
+  CHK(up_copy(ws, ws->d_symoff, symoff.data(), (ncb + 1) * sizeof(uint64_t)));
+  if (zlim) CHK(up_copy(ws, ws->d_zlim, zlim, ncb));
+  if (list.empty()) return 0;
   CHK(up_copy(ws, ws->d_list, list.data(), list.size() * sizeof(uint64_t)));
-  return launch(ws->d_list, (uint32_t)list.size(), ws->stream, lpw);
+  return v2_launch(ws, c, list.size(), zlim ? ws->d_zlim : nullptr, d_data, d_bp, d_hist, gp, nullptr);
 }
 
 // passes tier-1 codes for a codeblock with K bitplanes: all 3K - 2, or the
@@ -2079,8 +2164,13 @@ __global__ void pack_records_kernel(const uint64_t *__restrict__ rows, uint32_t 
                                     const double *__restrict__ bfut, const uint8_t *__restrict__ Z,
                                     const uint8_t *__restrict__ U, const uint8_t *__restrict__ Km,
                                     uint32_t *__restrict__ pL, uint16_t *__restrict__ pS, double *__restrict__ pM,
-                                    double *__restrict__ pB, uint8_t *__restrict__ pb) {
+                                    double *__restrict__ pB, uint8_t *__restrict__ pb,
+                                    const uint64_t *__restrict__ info) {
   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (info) {   // a list built on the device (t1_list_kernel)
+    if (info[3]) return;
+    nrows = (uint32_t)info[0];
+  }
   if (r >= nrows) return;
   const uint64_t cb = rows[r], o = poff[r], m = poff[r + 1] - o;
   const uint32_t *l = L + cb * kMaxPasses;
@@ -2095,50 +2185,58 @@ __global__ void pack_records_kernel(const uint64_t *__restrict__ rows, uint32_t 
   pb[3 * r + 2] = Km[cb];
 }
 
-// Bring the records of `rows` (the list v2_code launched, in d_list), coded
-// to m[r] passes, into rec, together with the flag words (hfl); ends with a
-// stream synchronisation. The offsets go up through the open upload staging.
-static int fetch_rows(Workspace *ws, CblkRecords &rec, const std::vector<uint64_t> &rows,
-                      const std::vector<uint32_t> &m, uint32_t hfl[4]) {
-  const uint64_t n = rows.size();
-  std::vector<uint64_t> poff(n + 1, 0);
-  for (uint64_t r = 0; r < n; ++r) poff[r + 1] = poff[r] + m[r];
-  const uint64_t tl = poff[n];
-  const size_t oS = (4 * tl + 255) & ~(size_t)255, oM = (oS + 2 * (tl + n) + 255) & ~(size_t)255;
-  const size_t oD = (oM + 8 * tl + 255) & ~(size_t)255, oB = (oD + 8 * n + 255) & ~(size_t)255;
-  const size_t oF = (oB + 3 * n + 255) & ~(size_t)255, total = oF + 16;
+// Byte offsets of the packed records of n rows and tl passes in all (L,
+// then S, mse, the bounds, Z/U/Kmsbs), then 16 bytes of flag words and 32
+// of list totals.
+struct PackLayout {
+  size_t oS, oM, oD, oB, oF, total;
+};
+static PackLayout pack_layout(uint64_t n, uint64_t tl) {
+  PackLayout p;
+  p.oS = (4 * tl + 255) & ~(size_t)255;
+  p.oM = (p.oS + 2 * (tl + n) + 255) & ~(size_t)255;
+  p.oD = (p.oM + 8 * tl + 255) & ~(size_t)255;
+  p.oB = (p.oD + 8 * n + 255) & ~(size_t)255;
+  p.oF = (p.oB + 3 * n + 255) & ~(size_t)255;
+  p.total = p.oF + 48;
+  return p;
+}
+
+// device and page-locked host buffers of a packed-record layout
+static int pack_alloc(Workspace *ws, const PackLayout &lo, uint64_t n) {
   size_t cap = ws->cap_poff;
   CHK(ensure((void **)&ws->d_poff, &cap, (n + 1) * sizeof(uint64_t)));
   ws->cap_poff = cap;
   cap = ws->cap_rpack;
-  CHK(ensure((void **)&ws->d_rpack, &cap, total));
+  CHK(ensure((void **)&ws->d_rpack, &cap, lo.total));
   ws->cap_rpack = cap;
-  if (total > ws->cap_down) {
+  if (lo.total > ws->cap_down) {
     if (ws->h_down) HIP_OK(hipHostFree(ws->h_down));
     ws->h_down = nullptr;
     ws->cap_down = 0;
-    HIP_OK(hipHostMalloc((void **)&ws->h_down, total));
-    ws->cap_down = total;
+    HIP_OK(hipHostMalloc((void **)&ws->h_down, lo.total));
+    ws->cap_down = lo.total;
   }
-  if (n) {
-    CHK(up_copy(ws, ws->d_poff, poff.data(), (n + 1) * sizeof(uint64_t)));
-    hipLaunchKernelGGL(pack_records_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, ws->stream, ws->d_list,
-                       (uint32_t)n, ws->d_poff, ws->d_L, ws->d_S, ws->d_mse, ws->d_bfut, ws->d_Z, ws->d_U,
-                       ws->d_Kmsbs, (uint32_t *)ws->d_rpack, (uint16_t *)(ws->d_rpack + oS),
-                       (double *)(ws->d_rpack + oM), (double *)(ws->d_rpack + oD), ws->d_rpack + oB);
-    HIP_OK(hipGetLastError());
-    HIP_OK(hipMemcpyAsync(ws->h_down, ws->d_rpack, oB + 3 * n, hipMemcpyDeviceToHost, ws->stream));
-  }
-  HIP_OK(hipMemcpyAsync(ws->h_down + oF, ws->d_flags, 16, hipMemcpyDeviceToHost, ws->stream));
-  double tmf = trace_mark(nullptr, 0);
-  HIP_OK(hipStreamSynchronize(ws->stream));
-  tmf = trace_mark("records: wait for the GPU", tmf);
-  std::memcpy(hfl, ws->h_down + oF, 16);
-  if (hfl[0]) return 0;
+  return 0;
+}
+
+static void pack_launch(Workspace *ws, const PackLayout &lo, uint64_t ngrid, const uint64_t *info) {
+  hipLaunchKernelGGL(pack_records_kernel, dim3((uint32_t)((ngrid + 255) / 256)), dim3(256), 0, ws->stream,
+                     ws->d_list, (uint32_t)ngrid, ws->d_poff, ws->d_L, ws->d_S, ws->d_mse, ws->d_bfut, ws->d_Z,
+                     ws->d_U, ws->d_Kmsbs, (uint32_t *)ws->d_rpack, (uint16_t *)(ws->d_rpack + lo.oS),
+                     (double *)(ws->d_rpack + lo.oM), (double *)(ws->d_rpack + lo.oD), ws->d_rpack + lo.oB, info);
+}
+
+// The packed records in h_down (layout lo) into rec: rows[r] coded to m[r]
+// passes, from pass offset poff[r].
+static int unpack_rows(Workspace *ws, CblkRecords &rec, const std::vector<uint64_t> &rows,
+                       const std::vector<uint32_t> &m, const std::vector<uint64_t> &poff, const PackLayout &lo,
+                       double tmf) {
+  const uint64_t n = rows.size();
   const uint32_t *pL = (const uint32_t *)ws->h_down;
-  const uint16_t *pS = (const uint16_t *)(ws->h_down + oS);
-  const double *pM = (const double *)(ws->h_down + oM), *pD = (const double *)(ws->h_down + oD);
-  const uint8_t *pb = ws->h_down + oB;
+  const uint16_t *pS = (const uint16_t *)(ws->h_down + lo.oS);
+  const double *pM = (const double *)(ws->h_down + lo.oM), *pD = (const double *)(ws->h_down + lo.oD);
+  const uint8_t *pb = ws->h_down + lo.oB;
   if (rec.mse.size() != rec.n * kMaxPasses) rec.mse.assign(rec.n * kMaxPasses, 0.0);
   if (rec.bfut.size() != rec.n) rec.bfut.assign(rec.n, 0.0);
   // into the per-codeblock record arrays (strided by the pass maximum, so
@@ -2177,6 +2275,115 @@ static int fetch_rows(Workspace *ws, CblkRecords &rec, const std::vector<uint64_
   }
   trace_mark("records: unpack", tmf);
   return 0;
+}
+
+// Bring the records of `rows` (the host list v2_code launched, in d_list),
+// coded to m[r] passes, into rec, together with the flag words (hfl); ends
+// with a stream synchronisation. The offsets go up through the open upload
+// staging.
+static int fetch_rows(Workspace *ws, CblkRecords &rec, const std::vector<uint64_t> &rows,
+                      const std::vector<uint32_t> &m, uint32_t hfl[4]) {
+  const uint64_t n = rows.size();
+  std::vector<uint64_t> poff(n + 1, 0);
+  for (uint64_t r = 0; r < n; ++r) poff[r + 1] = poff[r] + m[r];
+  const PackLayout lo = pack_layout(n, poff[n]);
+  CHK(pack_alloc(ws, lo, n));
+  if (n) {
+    CHK(up_copy(ws, ws->d_poff, poff.data(), (n + 1) * sizeof(uint64_t)));
+    pack_launch(ws, lo, n, nullptr);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(ws->h_down, ws->d_rpack, lo.oB + 3 * n, hipMemcpyDeviceToHost, ws->stream));
+  }
+  HIP_OK(hipMemcpyAsync(ws->h_down + lo.oF, ws->d_flags, 16, hipMemcpyDeviceToHost, ws->stream));
+  double tmf = trace_mark(nullptr, 0);
+  HIP_OK(hipStreamSynchronize(ws->stream));
+  tmf = trace_mark("records: wait for the GPU", tmf);
+  std::memcpy(hfl, ws->h_down + lo.oF, 16);
+  if (hfl[0]) return 0;
+  return unpack_rows(ws, rec, rows, m, poff, lo, tmf);
+}
+
+// Tier-1 coding of every codeblock with K > 0 with the list, symbol-stream
+// offsets and packed-record offsets built on the device from K and the pass
+// limits dz (device, or null): the host launches the whole chain without
+// waiting and waits once, for the totals; records then come back by section.
+// When the symbol or record buffers (kept from earlier calls) are too small
+// the coding kernels skip their work; the buffers grow to the totals and the
+// chain runs again (once). On return rec.K, zh (the limits, when dz), list
+// and m are on the host and the records unpacked.
+static int t1_code_dev(Workspace *ws, const bwc_codec *c, const Param &par, const uint8_t *dz, uint8_t *d_data,
+                       const uint8_t *d_bp, const uint32_t *d_hist, const GenParams &gp, CblkRecords &rec,
+                       std::vector<uint8_t> &zh, std::vector<uint64_t> &list, std::vector<uint32_t> &m,
+                       uint32_t hfl[4]) {
+  const uint64_t ncb = par.ncb;
+  if (!ws->d_t1info) HIP_OK(hipMalloc((void **)&ws->d_t1info, 4 * sizeof(uint64_t)));
+  if (2 * ncb > ws->cap_kz) {
+    if (ws->h_kz) HIP_OK(hipHostFree(ws->h_kz));
+    ws->h_kz = nullptr;
+    ws->cap_kz = 0;
+    HIP_OK(hipHostMalloc((void **)&ws->h_kz, 2 * ncb));
+    ws->cap_kz = 2 * ncb;
+  }
+  uint64_t info[4] = {0, 0, 0, 0};
+  PackLayout lo{};
+  double tm = trace_mark(nullptr, 0);
+  for (int attempt = 0;; ++attempt) {
+    CHK(v2_buffers(ws, ncb, ws->cap_sym, ncb));
+    lo = pack_layout(ncb, ws->cap_pass);
+    CHK(pack_alloc(ws, lo, ncb));
+    hipLaunchKernelGGL(t1_list_kernel, dim3(1), dim3(1024), 0, ws->stream, ncb, ws->d_K, dz, ws->d_cbbox,
+                       ws->d_qoff, (uint64_t)ws->cap_sym, ws->cap_pass, ws->d_list, ws->d_symoff, ws->d_poff,
+                       ws->d_t1info);
+    HIP_OK(hipGetLastError());
+    CHK(v2_launch(ws, c, ncb, dz, d_data, d_bp, d_hist, gp, ws->d_t1info));
+    pack_launch(ws, lo, ncb, ws->d_t1info);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(ws->h_down + lo.oF, ws->d_flags, 16, hipMemcpyDeviceToHost, ws->stream));
+    HIP_OK(hipMemcpyAsync(ws->h_down + lo.oF + 16, ws->d_t1info, 32, hipMemcpyDeviceToHost, ws->stream));
+    HIP_OK(hipMemcpyAsync(ws->h_kz, ws->d_K, ncb, hipMemcpyDeviceToHost, ws->stream));
+    if (dz) HIP_OK(hipMemcpyAsync(ws->h_kz + ncb, dz, ncb, hipMemcpyDeviceToHost, ws->stream));
+    if (attempt == 0) tm = trace_mark("list + offsets + launch", tm);
+    HIP_OK(hipStreamSynchronize(ws->stream));
+    tm = trace_mark("records: wait for the GPU", tm);
+    std::memcpy(info, ws->h_down + lo.oF + 16, 32);
+    if (!info[3]) break;
+    if (attempt > 0) {
+      std::snprintf(g_err, sizeof g_err, "tier-1 buffers too small after growing them");
+      return 1;
+    }
+    CHK(v2_buffers(ws, ncb, info[1] + 64, ncb));
+    ws->cap_pass = std::max<uint64_t>(ws->cap_pass, info[2]);
+  }
+  std::memcpy(hfl, ws->h_down + lo.oF, 16);
+  std::memcpy(rec.K.data(), ws->h_kz, ncb);
+  if (dz) zh.assign(ws->h_kz + ncb, ws->h_kz + 2 * ncb);
+  const uint8_t *zl = dz ? zh.data() : nullptr;
+  // the same list and offsets on the host, for unpacking
+  list.clear();
+  m.clear();
+  for (uint64_t i = 0; i < ncb; ++i)
+    if (rec.K[i]) {
+      list.push_back(i);
+      m.push_back(passes_coded(rec.K[i], zl, i));
+    }
+  const uint64_t n = list.size();
+  std::vector<uint64_t> poff(n + 1, 0);
+  for (uint64_t r = 0; r < n; ++r) poff[r + 1] = poff[r] + m[r];
+  if (n != info[0] || poff[n] != info[2]) {
+    std::snprintf(g_err, sizeof g_err, "device codeblock list (%llu, %llu passes) differs from the host's",
+                  (unsigned long long)info[0], (unsigned long long)info[2]);
+    return 1;
+  }
+  if (hfl[0] || !n) return 0;
+  const uint64_t tl = poff[n];
+  HIP_OK(hipMemcpyAsync(ws->h_down, ws->d_rpack, 4 * tl, hipMemcpyDeviceToHost, ws->stream));
+  HIP_OK(hipMemcpyAsync(ws->h_down + lo.oS, ws->d_rpack + lo.oS, 2 * (tl + n), hipMemcpyDeviceToHost, ws->stream));
+  HIP_OK(hipMemcpyAsync(ws->h_down + lo.oM, ws->d_rpack + lo.oM, 8 * tl, hipMemcpyDeviceToHost, ws->stream));
+  HIP_OK(hipMemcpyAsync(ws->h_down + lo.oD, ws->d_rpack + lo.oD, 8 * n, hipMemcpyDeviceToHost, ws->stream));
+  HIP_OK(hipMemcpyAsync(ws->h_down + lo.oB, ws->d_rpack + lo.oB, 3 * n, hipMemcpyDeviceToHost, ws->stream));
+  HIP_OK(hipStreamSynchronize(ws->stream));
+  tm = trace_mark("records: copy", tm);
+  return unpack_rows(ws, rec, list, m, poff, lo, tm);
 }
 
 int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double *work, CblkRecords &rec,
@@ -2290,6 +2497,8 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
                        work, W, H, D, ws->d_cbbox, ws->d_cbsb, ws->d_bpoff, bs.bp, ws->d_K, ncb, ws->d_cbmeta,
                        d_feat, ws->d_tstat, rate > 0 ? rate : 1.0f, d_hist);
     HIP_OK(hipGetLastError());
+    // pass limits on the device: the network's or the fixed ones
+    const uint8_t *dz = nullptr;
     if (lim.model) {
       char e[200];
       if (skipnet_infer(ws->stream, ws->d_feat, ncb, ws->d_skipw, ws->d_tstat, ws->d_K, lim.margin, ws->d_zlim,
@@ -2297,26 +2506,17 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
         std::snprintf(g_err, sizeof g_err, "%s", e);
         return 1;
       }
-      zpred.resize(ncb);
-      HIP_OK(hipMemcpyAsync(zpred.data(), ws->d_zlim, ncb, hipMemcpyDeviceToHost, ws->stream));
-      zlim = zpred.data();
+      dz = ws->d_zlim;
+    } else if (lim.fixed) {
+      CHK(up_copy(ws, ws->d_zlim, lim.fixed, ncb));
+      dz = ws->d_zlim;
     }
-    HIP_OK(hipMemcpyAsync(rec.K.data(), ws->d_K, ncb, hipMemcpyDeviceToHost, ws->stream));
-    double tm = trace_mark(nullptr, 0);
-    HIP_OK(hipStreamSynchronize(ws->stream));
-    tm = trace_mark("bitplanes + net wait", tm);
-    if (lim.out && zlim) std::memcpy(lim.out, zlim, ncb);
+    // list, offsets, coding and record packing on the device, one wait
     std::vector<uint64_t> list;
     std::vector<uint32_t> m;
-    for (uint64_t i = 0; i < ncb; ++i)
-      if (rec.K[i]) {
-        list.push_back(i);
-        m.push_back(passes_coded(rec.K[i], zlim, i));
-      }
-    CHK(v2_code(ws, c, par, tb, rec.K.data(), list, zlim, d_data, bs.bp, d_hist, gp));
-    tm = trace_mark("list + offsets + launch", tm);
-    CHK(fetch_rows(ws, rec, list, m, hfl));
-    tm = trace_mark("gen + mq + fetch", tm);
+    CHK(t1_code_dev(ws, c, par, dz, d_data, bs.bp, d_hist, gp, rec, zpred, list, m, hfl));
+    if (lim.model) zlim = zpred.data();
+    if (lim.out && zlim) std::memcpy(lim.out, zlim, ncb);
     if (lim.features) {
       // decisions per codeblock: the symbol position where its last coded
       // pass ends (measurement only)

@@ -102,6 +102,12 @@ struct Workspace {
   uint64_t *d_poff = nullptr;     // packed-record offsets per listed codeblock
   uint8_t *d_rpack = nullptr;     // packed records
   size_t cap_poff = 0, cap_rpack = 0;
+  // list built on the device: {length, symbol bytes, passes, overflow}, and
+  // the packed-record pass capacity the record layout was sized for
+  uint64_t *d_t1info = nullptr;
+  uint64_t cap_pass = 0;
+  uint8_t *h_kz = nullptr;        // page-locked: K, then the pass limits, of every codeblock
+  size_t cap_kz = 0;
   int last_dwt_launches = 0;      // kernel launches of the last DWT call
   // forward DWT launch sequences as HIP graphs, one per (parameter geometry,
   // buffers, normalisation, launch knobs): recorded the second time a key is
