@@ -110,6 +110,7 @@ def lib():
             "bwc_amd_set_skip_model": (C.c_int, [vp, C.POINTER(C.c_float), C.c_size_t]),
             "bwc_amd_skip_stats": (C.c_int, [vp, C.POINTER(u64)]),
             "bwc_amd_cblk_cuts": (C.c_int, [vp, u64, u8, C.POINTER(C.c_int16)]),
+            "bwc_amd_rc_trace": (C.c_int, [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(C.c_int64), C.c_int]),
             "bwc_amd_collect_features": (C.c_int, [vp, C.c_int]),
             "bwc_amd_cblk_features": (C.c_int, [vp, u64, u8, C.POINTER(C.c_float)]),
             "bwc_amd_cblk_decisions": (C.c_int, [vp, u64, u8, C.POINTER(u32)]),
@@ -379,6 +380,18 @@ class Codec:
         if lib().bwc_amd_cblk_cuts(self.h, tile, param, _p(out, C.c_int16)):
             raise RuntimeError("bwc_amd_cblk_cuts failed")
         return out
+
+    def rc_trace(self, cap=4096):
+        """(layer, threshold, estimate) of every probe the device PCRD
+        bisection of the last compression's last tile made."""
+        lay = np.zeros(cap, np.uint32)
+        slo = np.zeros(cap, np.uint32)
+        est = np.zeros(cap, np.int64)
+        n = lib().bwc_amd_rc_trace(self.h, _p(lay, C.c_uint32), _p(slo, C.c_uint32), _p(est, C.c_int64), cap)
+        if n < 0:
+            raise RuntimeError("bwc_amd_rc_trace: no compression on this codec")
+        n = min(n, cap)
+        return list(zip(lay[:n].tolist(), slo[:n].tolist(), est[:n].tolist()))
 
     def collect_features(self, on: bool = True):
         lib().bwc_amd_collect_features(self.h, 1 if on else 0)

@@ -236,6 +236,72 @@ int rate_control_prefix(bwc_codec *c, Job *job, gpu::Workspace *ws, uint64_t ti)
   }
 }
 
+// Rate control on the device (the default; BWC_AMD_RC=host keeps it on the
+// host pool over records copied back from tier-1)
+bool rc_on_device() {
+  const char *e = std::getenv("BWC_AMD_RC");
+  return !(e && std::strcmp(e, "host") == 0);
+}
+
+// Rate control of tile ti on the device, with the records left there by
+// tier-1: as rate_control_prefix, the pass-skip check runs while codeblocks
+// are coded to a prefix of their passes, and flagged codeblocks are coded
+// further until the outcome is that of complete codings. The host then only
+// sets the packet flags, body sizes and tag trees from the cuts.
+int rate_control_dev(bwc_codec *c, Job *job, gpu::Workspace *ws, uint64_t ti) {
+  Tile &tl = job->plan.tiles[ti];
+  const int np = c->info.nPar;
+  SkipCheck chk;
+  chk.margin = 3;
+  if (const char *e = std::getenv("BWC_AMD_SKIP_SETTLE")) chk.margin = std::max(1, std::atoi(e));
+  for (int round = 0;; ++round) {
+    reset_tile_t2(c, job, ti, false);
+    uint64_t ntrunc = 0;
+    for (int p = 0; p < np; ++p) {
+      const CblkRecords &R = job->rec[ti * np + p];
+      for (uint64_t i = 0; i < R.n; ++i) ntrunc += R.Z[i] < R.Zfull[i];
+    }
+    const double tr = now_ms();
+    if (gpu::rc_tile(ws, c, job, ti, ntrunc ? &chk : nullptr)) {
+      set_err(gpu::rc_last_error());
+      return 1;
+    }
+    if (trace_on()) std::fprintf(stderr, "rc: device rate control   %8.3f ms\n", now_ms() - tr);
+    uint64_t nflag = 0;
+    if (ntrunc)
+      for (auto &f : chk.flagged) nflag += f.size();
+    if (!ntrunc || (!chk.fallback && !nflag)) {
+      t2_apply_cuts(c, job, ti);
+      return 0;
+    }
+    if (nflag) chk.fallback = false;
+    job->skip_stats[3] += 1;
+    if (chk.fallback) job->skip_stats[5] += 1;
+    for (int p = 0; p < np; ++p) {
+      CblkRecords &R = job->rec[ti * np + p];
+      std::vector<uint64_t> list;
+      if (chk.fallback) {
+        for (uint64_t i = 0; i < R.n; ++i)
+          if (R.Z[i] < R.Zfull[i]) list.push_back(i);
+      } else {
+        list = chk.flagged[p];
+      }
+      if (list.empty()) continue;
+      for (uint64_t i : list) {
+        const int step = 9 << std::min(round, 3);
+        int nz = (chk.fallback || round >= 3) ? R.Zfull[i] : std::min<int>(R.Zfull[i], R.Z[i] + step);
+        R.zlim[i] = (uint8_t)nz;
+        job->skip_stats[1] += (uint64_t)nz;
+      }
+      job->skip_stats[4] += list.size();
+      if (gpu::t1_extend(ws, c, tl.par[p], R, p, list, R.zlim.data(), false)) {
+        set_err(gpu::last_error());
+        return 1;
+      }
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -552,6 +618,7 @@ size_t bwc_compress(bwc_codec *const codec, bwc_stream *const stream) {
     return 0;
   }
   bwc_gl_ctrl &ctl = codec->control;
+  const bool rc_dev = rc_on_device();
   const double t_start = now_ms();
   reset_for_encode(codec, job);
   std::fill(std::begin(job->skip_stats), std::end(job->skip_stats), 0);
@@ -607,6 +674,7 @@ size_t bwc_compress(bwc_codec *const codec, bwc_stream *const stream) {
           pl.out = zl.data();
         }
         pl.features = job->collect_features;
+        pl.records = !rc_dev;
         if (gpu::t1_encode(ws, codec, par, ws->d_work, R, &tl.slope_min, &tl.slope_max, p, pl)) goto fail;
         R.zlim = zl;
         if (job->collect_features) {
@@ -626,7 +694,12 @@ size_t bwc_compress(bwc_codec *const codec, bwc_stream *const stream) {
     }
     {
       const double t0 = now_ms();
-      if (job->skip_mode) {
+      if (rc_dev) {
+        if (rate_control_dev(codec, job, ws, ti)) {
+          ms[6] += now_ms() - t0;
+          return 0;
+        }
+      } else if (job->skip_mode) {
         if (rate_control_prefix(codec, job, ws, ti)) goto fail;
       } else {
         t2_rate_control(codec, job, ti);
@@ -1128,6 +1201,12 @@ int bwc_amd_skipnet_predict(const float *feat, uint64_t n, const float *weights,
     return 1;
   }
   return 0;
+}
+
+int bwc_amd_rc_trace(bwc_codec *codec, uint32_t *layer, uint32_t *slope, int64_t *est, int cap) {
+  Job *job = job_of(codec);
+  if (!job || !job->ws) return -1;
+  return gpu::rc_trace(static_cast<gpu::Workspace *>(job->ws), layer, slope, est, cap);
 }
 
 int bwc_amd_cblk_cuts(bwc_codec *codec, uint64_t tile, uint8_t param, int16_t *cuts) {

@@ -206,6 +206,10 @@ struct SkipCheck {
 };
 int t2_rate_control(bwc_codec *codec, Job *job, uint64_t tile, SkipCheck *chk = nullptr);
 int t2_packets(bwc_codec *codec, Job *job, uint64_t tile);
+// the packet flags, body sizes and tag-tree values create_quality_layer
+// leaves for the cuts already in the records (R.cp, R.L at the cuts): the
+// host's share of rate control done on the device
+int t2_apply_cuts(bwc_codec *codec, Job *job, uint64_t tile);
 // the host thread pool of rate control: fn(0 .. n-1) on `workers` pool
 // threads and the caller; rc_thread_count() is the rate-control thread count
 // (BWC_AMD_RC_THREADS, else one less than the host threads, at most 15)

@@ -63,12 +63,18 @@ Workspace::~Workspace() {
   if (device < 0) return;
   (void)hipSetDevice(device);
   void *ptrs[] = {d_input, d_work,  d_work2, d_red,  d_norm, d_cbbox, d_cbsb, d_cbmeta, d_qoff,   d_doff, d_q,
-                  d_state, d_data,  d_K,    d_Kmsbs, d_Z,    d_L,    d_S,      d_snap,   d_mse,  d_flags,
-                  d_packed, d_gather, d_bp, d_bpoff, d_sym, d_symoff, d_pcnt, d_pmse, d_list, d_U, d_zlim,
+                  d_state, d_data,  d_snap,  d_flags,
+                  d_packed, d_gather, d_bp, d_bpoff, d_sym, d_symoff, d_pcnt, d_pmse, d_list, d_zlim,
                   d_feat,  d_tstat, d_skipw, d_bodies, dec.box, dec.sb, dec.meta, dec.K, dec.Z, dec.len,
-                  dec.off, dec.data, dec.soff, dec.state, dec.out, d_poff, d_rpack, d_bfut, d_t1info};
+                  dec.off, dec.data, dec.soff, dec.state, dec.out, d_poff, d_rpack, d_t1info};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  for (auto &r : rec_slots) {
+    void *rp[] = {r.K, r.Kmsbs, r.Z, r.U, r.L, r.S, r.mse, r.bfut};
+    for (void *p : rp)
+      if (p) (void)hipFree(p);
+  }
+  rc_release(this);
   for (auto &b : bp_slots) {
     if (b.bp) (void)hipFree(b.bp);
     if (b.hist) (void)hipFree(b.hist);
@@ -102,6 +108,40 @@ static int ensure(void **p, size_t *cap, size_t bytes) {
   size_t nb = std::max<size_t>(bytes, 256);
   HIP_OK(hipMalloc(p, nb));
   *cap = nb;
+  return 0;
+}
+
+// Point the workspace's record aliases (d_K, d_L, ...) at parameter slot
+// `slot`, with room for ncb codeblocks (grown without keeping contents).
+static int bind_slot(Workspace *ws, int slot, uint64_t ncb) {
+  if (slot < 0 || slot > 255) {
+    std::snprintf(g_err, sizeof g_err, "parameter slot out of range");
+    return 1;
+  }
+  if (slot >= (int)ws->rec_slots.size()) ws->rec_slots.resize(slot + 1);
+  Workspace::RecSlot &r = ws->rec_slots[slot];
+  if (r.cap < ncb + 1) {
+    void **rp[] = {(void **)&r.K, (void **)&r.Kmsbs, (void **)&r.Z,   (void **)&r.U,
+                   (void **)&r.L, (void **)&r.S,     (void **)&r.mse, (void **)&r.bfut};
+    for (void **p : rp)
+      if (*p) {
+        HIP_OK(hipFree(*p));
+        *p = nullptr;
+      }
+    r.cap = 0;
+    const uint64_t n = ncb + 1;
+    HIP_OK(hipMalloc((void **)&r.K, n));
+    HIP_OK(hipMalloc((void **)&r.Kmsbs, n));
+    HIP_OK(hipMalloc((void **)&r.Z, n));
+    HIP_OK(hipMalloc((void **)&r.U, n));
+    HIP_OK(hipMalloc((void **)&r.L, n * kMaxPasses * sizeof(uint32_t)));
+    HIP_OK(hipMalloc((void **)&r.S, n * (kMaxPasses + 1) * sizeof(uint16_t)));
+    HIP_OK(hipMalloc((void **)&r.mse, n * kMaxPasses * sizeof(double)));
+    HIP_OK(hipMalloc((void **)&r.bfut, n * sizeof(double)));
+    r.cap = n;
+  }
+  ws->d_K = r.K, ws->d_Kmsbs = r.Kmsbs, ws->d_Z = r.Z, ws->d_U = r.U;
+  ws->d_L = r.L, ws->d_S = r.S, ws->d_mse = r.mse, ws->d_bfut = r.bfut;
   return 0;
 }
 
@@ -2277,13 +2317,56 @@ static int unpack_rows(Workspace *ws, CblkRecords &rec, const std::vector<uint64
   return 0;
 }
 
+// page-locked room for K, the pass limits, Z and Kmsbs of ncb codeblocks and
+// 16 bytes of flag words
+static int kz_room(Workspace *ws, uint64_t ncb) {
+  const size_t need = 4 * ncb + 16;
+  if (need <= ws->cap_kz) return 0;
+  if (ws->h_kz) HIP_OK(hipHostFree(ws->h_kz));
+  ws->h_kz = nullptr;
+  ws->cap_kz = 0;
+  HIP_OK(hipHostMalloc((void **)&ws->h_kz, need));
+  ws->cap_kz = need;
+  return 0;
+}
+
+// Z and Kmsbs of the listed rows (coded to m[r] passes) from the device
+// arrays' copies Zh and Kh; a row whose Z is not m[r] is an error.
+static int take_z_kmsbs(CblkRecords &rec, const std::vector<uint64_t> &rows, const std::vector<uint32_t> &m,
+                        const uint8_t *Zh, const uint8_t *Kh) {
+  for (size_t r = 0; r < rows.size(); ++r) {
+    const uint64_t cb = rows[r];
+    if (Zh[cb] != m[r]) {
+      std::snprintf(g_err, sizeof g_err, "codeblock %llu: %u passes coded, %u expected", (unsigned long long)cb,
+                    (unsigned)Zh[cb], m[r]);
+      return 1;
+    }
+    rec.Z[cb] = Zh[cb];
+    rec.Kmsbs[cb] = Kh[cb];
+  }
+  return 0;
+}
+
 // Bring the records of `rows` (the host list v2_code launched, in d_list),
 // coded to m[r] passes, into rec, together with the flag words (hfl); ends
 // with a stream synchronisation. The offsets go up through the open upload
 // staging.
 static int fetch_rows(Workspace *ws, CblkRecords &rec, const std::vector<uint64_t> &rows,
-                      const std::vector<uint32_t> &m, uint32_t hfl[4]) {
+                      const std::vector<uint32_t> &m, uint32_t hfl[4], bool records) {
   const uint64_t n = rows.size();
+  if (!records) {
+    // the records stay on the device (rate control runs there): Z and Kmsbs
+    // of the rows, and the flag words
+    const uint64_t ncb = rec.n;
+    CHK(kz_room(ws, ncb));
+    HIP_OK(hipMemcpyAsync(ws->h_kz + 2 * ncb, ws->d_Z, ncb, hipMemcpyDeviceToHost, ws->stream));
+    HIP_OK(hipMemcpyAsync(ws->h_kz + 3 * ncb, ws->d_Kmsbs, ncb, hipMemcpyDeviceToHost, ws->stream));
+    HIP_OK(hipMemcpyAsync(ws->h_kz + 4 * ncb, ws->d_flags, 16, hipMemcpyDeviceToHost, ws->stream));
+    HIP_OK(hipStreamSynchronize(ws->stream));
+    std::memcpy(hfl, ws->h_kz + 4 * ncb, 16);
+    if (hfl[0]) return 0;
+    return take_z_kmsbs(rec, rows, m, ws->h_kz + 2 * ncb, ws->h_kz + 3 * ncb);
+  }
   std::vector<uint64_t> poff(n + 1, 0);
   for (uint64_t r = 0; r < n; ++r) poff[r + 1] = poff[r] + m[r];
   const PackLayout lo = pack_layout(n, poff[n]);
@@ -2314,16 +2397,10 @@ static int fetch_rows(Workspace *ws, CblkRecords &rec, const std::vector<uint64_
 static int t1_code_dev(Workspace *ws, const bwc_codec *c, const Param &par, const uint8_t *dz, uint8_t *d_data,
                        const uint8_t *d_bp, const uint32_t *d_hist, const GenParams &gp, CblkRecords &rec,
                        std::vector<uint8_t> &zh, std::vector<uint64_t> &list, std::vector<uint32_t> &m,
-                       uint32_t hfl[4]) {
+                       uint32_t hfl[4], bool records) {
   const uint64_t ncb = par.ncb;
   if (!ws->d_t1info) HIP_OK(hipMalloc((void **)&ws->d_t1info, 4 * sizeof(uint64_t)));
-  if (2 * ncb > ws->cap_kz) {
-    if (ws->h_kz) HIP_OK(hipHostFree(ws->h_kz));
-    ws->h_kz = nullptr;
-    ws->cap_kz = 0;
-    HIP_OK(hipHostMalloc((void **)&ws->h_kz, 2 * ncb));
-    ws->cap_kz = 2 * ncb;
-  }
+  CHK(kz_room(ws, ncb));
   uint64_t info[4] = {0, 0, 0, 0};
   PackLayout lo{};
   double tm = trace_mark(nullptr, 0);
@@ -2336,12 +2413,18 @@ static int t1_code_dev(Workspace *ws, const bwc_codec *c, const Param &par, cons
                        ws->d_t1info);
     HIP_OK(hipGetLastError());
     CHK(v2_launch(ws, c, ncb, dz, d_data, d_bp, d_hist, gp, ws->d_t1info));
-    pack_launch(ws, lo, ncb, ws->d_t1info);
-    HIP_OK(hipGetLastError());
+    if (records) {
+      pack_launch(ws, lo, ncb, ws->d_t1info);
+      HIP_OK(hipGetLastError());
+    }
     HIP_OK(hipMemcpyAsync(ws->h_down + lo.oF, ws->d_flags, 16, hipMemcpyDeviceToHost, ws->stream));
     HIP_OK(hipMemcpyAsync(ws->h_down + lo.oF + 16, ws->d_t1info, 32, hipMemcpyDeviceToHost, ws->stream));
     HIP_OK(hipMemcpyAsync(ws->h_kz, ws->d_K, ncb, hipMemcpyDeviceToHost, ws->stream));
     if (dz) HIP_OK(hipMemcpyAsync(ws->h_kz + ncb, dz, ncb, hipMemcpyDeviceToHost, ws->stream));
+    if (!records) {
+      HIP_OK(hipMemcpyAsync(ws->h_kz + 2 * ncb, ws->d_Z, ncb, hipMemcpyDeviceToHost, ws->stream));
+      HIP_OK(hipMemcpyAsync(ws->h_kz + 3 * ncb, ws->d_Kmsbs, ncb, hipMemcpyDeviceToHost, ws->stream));
+    }
     if (attempt == 0) tm = trace_mark("list + offsets + launch", tm);
     HIP_OK(hipStreamSynchronize(ws->stream));
     tm = trace_mark("records: wait for the GPU", tm);
@@ -2375,6 +2458,7 @@ static int t1_code_dev(Workspace *ws, const bwc_codec *c, const Param &par, cons
     return 1;
   }
   if (hfl[0] || !n) return 0;
+  if (!records) return take_z_kmsbs(rec, list, m, ws->h_kz + 2 * ncb, ws->h_kz + 3 * ncb);
   const uint64_t tl = poff[n];
   HIP_OK(hipMemcpyAsync(ws->h_down, ws->d_rpack, 4 * tl, hipMemcpyDeviceToHost, ws->stream));
   HIP_OK(hipMemcpyAsync(ws->h_down + lo.oS, ws->d_rpack + lo.oS, 2 * (tl + n), hipMemcpyDeviceToHost, ws->stream));
@@ -2404,36 +2488,23 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
   size_t cap;
   if (ncb > ws->cap_cb) {
     void **tabs[] = {(void **)&ws->d_cbbox, (void **)&ws->d_cbsb, (void **)&ws->d_cbmeta, (void **)&ws->d_qoff,
-                     (void **)&ws->d_doff,  (void **)&ws->d_K,    (void **)&ws->d_Kmsbs,  (void **)&ws->d_Z,
-                     (void **)&ws->d_L,     (void **)&ws->d_S,    (void **)&ws->d_snap,   (void **)&ws->d_mse,
-                     (void **)&ws->d_U,     (void **)&ws->d_zlim,   (void **)&ws->d_bfut};
+                     (void **)&ws->d_doff,  (void **)&ws->d_snap, (void **)&ws->d_zlim};
     for (void **p : tabs)
       if (*p) {
         HIP_OK(hipFree(*p));
         *p = nullptr;
       }
     const uint64_t n = ncb + 1;
-    HIP_OK(hipMalloc((void **)&ws->d_U, n));
-    HIP_OK(hipMalloc((void **)&ws->d_bfut, n * sizeof(double)));
     HIP_OK(hipMalloc((void **)&ws->d_zlim, n));
     HIP_OK(hipMalloc((void **)&ws->d_cbbox, 8 * n * sizeof(uint32_t)));
     HIP_OK(hipMalloc((void **)&ws->d_cbsb, 3 * n * sizeof(double)));
     HIP_OK(hipMalloc((void **)&ws->d_cbmeta, n * sizeof(uint32_t)));
     HIP_OK(hipMalloc((void **)&ws->d_qoff, n * sizeof(uint64_t)));
     HIP_OK(hipMalloc((void **)&ws->d_doff, n * sizeof(uint64_t)));
-    HIP_OK(hipMalloc((void **)&ws->d_K, n));
-    HIP_OK(hipMalloc((void **)&ws->d_Kmsbs, n));
-    HIP_OK(hipMalloc((void **)&ws->d_Z, n));
-    HIP_OK(hipMalloc((void **)&ws->d_L, n * kMaxPasses * sizeof(uint32_t)));
-    HIP_OK(hipMalloc((void **)&ws->d_S, n * (kMaxPasses + 1) * sizeof(uint16_t)));
     HIP_OK(hipMalloc((void **)&ws->d_snap, n * 2 * kMaxPasses * sizeof(uint64_t)));
-    HIP_OK(hipMalloc((void **)&ws->d_mse, n * kMaxPasses * sizeof(double)));
     ws->cap_cb = ncb;
   }
-  if (slot < 0 || slot > 255) {
-    std::snprintf(g_err, sizeof g_err, "parameter slot out of range");
-    return 1;
-  }
+  CHK(bind_slot(ws, slot, ncb));
   cap = ws->cap_datap[slot];
   CHK(ensure((void **)&ws->d_datap[slot], &cap, dof));
   ws->cap_datap[slot] = cap;
@@ -2514,7 +2585,7 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
     // list, offsets, coding and record packing on the device, one wait
     std::vector<uint64_t> list;
     std::vector<uint32_t> m;
-    CHK(t1_code_dev(ws, c, par, dz, d_data, bs.bp, d_hist, gp, rec, zpred, list, m, hfl));
+    CHK(t1_code_dev(ws, c, par, dz, d_data, bs.bp, d_hist, gp, rec, zpred, list, m, hfl, lim.records));
     if (lim.model) zlim = zpred.data();
     if (lim.out && zlim) std::memcpy(lim.out, zlim, ncb);
     if (lim.features) {
@@ -2576,13 +2647,15 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
 // limits in `zlim` (ncb entries; those of other codeblocks are ignored), and
 // refresh their records.
 int t1_extend(Workspace *ws, const bwc_codec *c, const Param &par, CblkRecords &rec, int slot,
-              const std::vector<uint64_t> &cbs, const uint8_t *zlim) {
+              const std::vector<uint64_t> &cbs, const uint8_t *zlim, bool records) {
   const uint64_t ncb = par.ncb;
   if (cbs.empty()) return 0;
-  if (slot < 0 || slot >= (int)ws->bp_slots.size() || !ws->bp_slots[slot].bp || ncb > ws->cap_cb) {
+  if (slot < 0 || slot >= (int)ws->bp_slots.size() || !ws->bp_slots[slot].bp || ncb > ws->cap_cb ||
+      slot >= (int)ws->rec_slots.size() || ws->rec_slots[slot].cap < ncb + 1) {
     std::snprintf(g_err, sizeof g_err, "no bitplanes kept for parameter slot %d", slot);
     return 1;
   }
+  CHK(bind_slot(ws, slot, ncb));
   CbTables tb;
   build_tables(par, tb);
   const GenParams gp = gen_params(par);
@@ -2601,7 +2674,7 @@ int t1_extend(Workspace *ws, const bwc_codec *c, const Param &par, CblkRecords &
   std::vector<uint32_t> m(cbs.size());
   for (size_t r = 0; r < cbs.size(); ++r) m[r] = passes_coded(rec.K[cbs[r]], zlim, cbs[r]);
   uint32_t hfl[4];
-  CHK(fetch_rows(ws, rec, cbs, m, hfl));
+  CHK(fetch_rows(ws, rec, cbs, m, hfl, records));
   if (hfl[0]) {
     std::snprintf(g_err, sizeof g_err, "tier-1 output exceeded a per-codeblock buffer (flags %u)", hfl[0]);
     return 1;

@@ -43,6 +43,10 @@ struct Workspace {
   uint8_t *d_data = nullptr;      // unused; MQ bytes live in one slot per parameter:
   uint8_t *d_datap[256] = {};     // slot p holds parameter p's codeblock byte buffers until
   size_t cap_datap[256] = {};     // rate control has chosen what to gather
+  // tier-1 records of the parameter slot being coded (aliases of one
+  // rec_slots entry, see bind_slot): K, Kmsbs, Z, U per codeblock, L[190],
+  // S[191], mse[190] per codeblock, the bound on the uncoded passes'
+  // distortion
   uint8_t *d_K = nullptr, *d_Kmsbs = nullptr, *d_Z = nullptr;
   uint32_t *d_L = nullptr;
   uint16_t *d_S = nullptr;
@@ -77,6 +81,18 @@ struct Workspace {
     size_t cap_hist = 0;
   };
   std::vector<BpSlot> bp_slots;
+  // tier-1 records per parameter slot, kept on the device until rate
+  // control (which runs on the device over every parameter of a tile) is done
+  struct RecSlot {
+    uint8_t *K = nullptr, *Kmsbs = nullptr, *Z = nullptr, *U = nullptr;
+    uint32_t *L = nullptr;
+    uint16_t *S = nullptr;
+    double *mse = nullptr, *bfut = nullptr;
+    size_t cap = 0;   // codeblocks
+  };
+  std::vector<RecSlot> rec_slots;
+  // device rate control (pcrd.hip): its buffers and per-tile tables
+  void *rc = nullptr;
   uint8_t *d_U = nullptr, *d_zlim = nullptr;
   // pass-limit network: features (16 per codeblock), tile statistics,
   // weights on the device
@@ -151,6 +167,10 @@ struct PassLimits {
   int margin = 0;
   uint8_t *out = nullptr;   // optional: the limits used (host, par.ncb)
   bool features = false;    // compute the network's features even without a model
+  // copy the records (L, S, distortions, U, bound) to the host; without it
+  // they stay on the device for rate control there and only K, Z and Kmsbs
+  // come back
+  bool records = true;
 };
 int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double *work, CblkRecords &rec,
               uint16_t *slope_min, uint16_t *slope_max, int slot, const PassLimits &lim = PassLimits());
@@ -174,7 +194,16 @@ int skipnet_predict(const float *feat, uint64_t n, const float *w, float *y, cha
 // re-code codeblocks `cbs` of the parameter last coded into `slot` with new
 // pass limits (zlim has par.ncb entries) and refresh their records
 int t1_extend(Workspace *ws, const bwc_codec *c, const Param &par, CblkRecords &rec, int slot,
-              const std::vector<uint64_t> &cbs, const uint8_t *zlim);
+              const std::vector<uint64_t> &cbs, const uint8_t *zlim, bool records = true);
+// PCRD rate control of tile `tile` on the device (pcrd.hip) over the records
+// kept in the parameter slots 0 .. nPar-1: fills R.cp of every parameter and
+// R.L at the cut points; with chk, the pass-skip check as well
+int rc_tile(Workspace *ws, bwc_codec *c, Job *job, uint64_t tile, SkipCheck *chk);
+// bisection probes of the last rc_tile call: (layer, threshold, estimate)
+// triples, at most cap; returns the count
+int rc_trace(Workspace *ws, uint32_t *layer, uint32_t *slope, int64_t *est, int cap);
+void rc_release(Workspace *ws);
+const char *rc_last_error();
 // gather each codeblock's prefix [0, len[i]) into rec.data / rec.data_off
 int gather(Workspace *ws, CblkRecords &rec, const std::vector<uint32_t> &len);
 // Packet bodies on the device: desc holds (offset in the body buffer,

@@ -1091,6 +1091,36 @@ struct T2 {
     }
   }
 
+  // What quality_layer(.., est 0) leaves besides the cuts, for cuts made
+  // elsewhere (on the device): contribution flags, the MSB tree values (at
+  // the first layer), the inclusion tree values (at the last) and the body
+  // sizes of the packets (tier2.c:1030-1064, :751-849)
+  void apply_cuts() {
+    for (int j = 0; j < c->info.nPar; ++j) {
+      Param &par = tl->par[j];
+      CblkRecords &R = rec(j);
+      for (int q = 0; q < nl; ++q)
+        for (auto &res : par.res) {
+          for (auto &sb : res.sb)
+            for (uint64_t p = 0; p < res.nprec; ++p) {
+              Precinct &pr = sb.prec[p];
+              for (uint64_t k = 0; k < pr.ncbs; ++k) {
+                const uint64_t i = pr.first_cb + k;
+                const int16_t *cp = &R.cp[i * (nl + 1) + 1];
+                if (cp[q] > cp[q - 1]) res.pk[q * res.nprec + p].e = 1;
+                if (q == 0) pr.msbs.set(k, R.Kmsbs[i]);
+                if (q + 1 == nl) {
+                  uint8_t l = 0;
+                  while (l < nl && cp[l] == 0) ++l;
+                  pr.incl.set(k, l);
+                }
+              }
+            }
+          for (uint64_t p = 0; p < res.nprec; ++p) body_size_only(j, res, p, q);
+        }
+    }
+  }
+
   // the packets of each (parameter, resolution, precinct) in layer order;
   // different precincts share no state (their codeblocks' beta, their tag
   // trees, their packets), so they are written on the probe threads
@@ -1213,6 +1243,12 @@ int t2_packets(bwc_codec *codec, Job *job, uint64_t tile) {
   T2 t{codec, job, &job->plan.tiles[tile], tile, codec->control.nLayers};
   t.threads = rc_thread_count();
   t.packets();
+  return 0;
+}
+
+int t2_apply_cuts(bwc_codec *codec, Job *job, uint64_t tile) {
+  T2 t{codec, job, &job->plan.tiles[tile], tile, codec->control.nLayers};
+  t.apply_cuts();
   return 0;
 }
 

@@ -154,6 +154,13 @@ int bwc_amd_skip_stats(bwc_codec *codec, uint64_t stats[8]);
 /* Last bwc_compress: number of passes each codeblock of (tile, param)
    contributes to the last quality layer (ncblk entries). */
 int bwc_amd_cblk_cuts(bwc_codec *codec, uint64_t tile, uint8_t param, int16_t *cuts);
+/* Last bwc_compress whose rate control ran on the device (the default;
+   BWC_AMD_RC=host runs it on the host): the probes the PCRD bisection of its
+   last tile made, in the reference's order (tier2.c:1193-1235), as (layer,
+   slope threshold, size estimate including the tile's earlier layers and
+   header bytes); at most cap entries written, the count returned (-1 for a
+   codec that has not compressed). */
+int bwc_amd_rc_trace(bwc_codec *codec, uint32_t *layer, uint32_t *slope, int64_t *est, int cap);
 /* Keep the pass-limit network's 16 features per codeblock during
    bwc_compress (on != 0), and read them for (tile, param) afterwards
    (ncblk x 16 floats; feature 15 relative to the tile mean). */

@@ -95,6 +95,9 @@ uint64_t  oc_cblk_record(const oc_codec *c, uint64_t tile, uint32_t param, uint6
 void      oc_tile_slopes(const oc_codec *c, uint64_t tile, uint16_t mm[2]);
 /* tier-2 rate control + packets for a tile; cuts[cb*nLayers+l] = cp_contr after PCRD */
 int       oc_t2_encode(oc_codec *c, uint64_t tile, int16_t *cuts_or_null);
+/* (layer, threshold, estimate) of every probe of the last tile's PCRD
+ * bisection, at most cap; returns the count */
+int       oc_rc_trace(uint32_t *layer, uint32_t *slope, int64_t *est, int cap);
 size_t    oc_assemble(oc_codec *c, uint8_t *out, size_t cap);
 uint8_t   oc_nlayers(const oc_codec *c);
 

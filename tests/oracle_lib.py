@@ -95,6 +95,8 @@ def lib():
         L.oc_tile_slopes.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(C.c_uint16)]
         L.oc_t2_encode.restype = C.c_int
         L.oc_t2_encode.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(C.c_int16)]
+        L.oc_rc_trace.restype = C.c_int
+        L.oc_rc_trace.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_int64), C.c_int]
         L.oc_assemble.restype = C.c_size_t
         L.oc_assemble.argtypes = [C.c_void_p, u8p, C.c_size_t]
         L.oc_t1_probe_block.restype = C.c_int
@@ -284,6 +286,16 @@ class Oracle:
         out = np.zeros(cap, np.uint8)
         n = lib().oc_assemble(self.h, ptr(out, C.c_uint8), cap)
         return out[:n].tobytes()
+
+
+def rc_trace(cap=4096):
+    """(layer, threshold, estimate) of every probe of the oracle's last
+    tile's PCRD bisection (this thread's last compression)."""
+    lay = np.zeros(cap, np.uint32)
+    slo = np.zeros(cap, np.uint32)
+    est = np.zeros(cap, np.int64)
+    n = lib().oc_rc_trace(ptr(lay, C.c_uint32), ptr(slo, C.c_uint32), ptr(est, C.c_int64), cap)
+    return list(zip(lay[:n].tolist(), slo[:n].tolist(), est[:n].tolist()))
 
 
 def oracle_compress(data, shape, **kw) -> bytes:
