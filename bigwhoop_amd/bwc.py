@@ -111,6 +111,8 @@ def lib():
             "bwc_amd_skip_stats": (C.c_int, [vp, C.POINTER(u64)]),
             "bwc_amd_cblk_cuts": (C.c_int, [vp, u64, u8, C.POINTER(C.c_int16)]),
             "bwc_amd_rc_trace": (C.c_int, [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(C.c_int64), C.c_int]),
+            "bwc_amd_stuffed_bytes": (C.c_int, [C.POINTER(u32), C.POINTER(u64), C.POINTER(u64), C.c_int,
+                                                C.POINTER(u64)]),
             "bwc_amd_collect_features": (C.c_int, [vp, C.c_int]),
             "bwc_amd_cblk_features": (C.c_int, [vp, u64, u8, C.POINTER(C.c_float)]),
             "bwc_amd_cblk_decisions": (C.c_int, [vp, u64, u8, C.POINTER(u32)]),

@@ -1203,6 +1203,15 @@ int bwc_amd_skipnet_predict(const float *feat, uint64_t n, const float *weights,
   return 0;
 }
 
+int bwc_amd_stuffed_bytes(const uint32_t *words, const uint64_t *woff, const uint64_t *nbits, int n,
+                          uint64_t *bytes) {
+  if (gpu::rc_stuffed_bytes(words, woff, nbits, n, bytes)) {
+    set_err(gpu::rc_last_error());
+    return 1;
+  }
+  return 0;
+}
+
 int bwc_amd_rc_trace(bwc_codec *codec, uint32_t *layer, uint32_t *slope, int64_t *est, int cap) {
   Job *job = job_of(codec);
   if (!job || !job->ws) return -1;

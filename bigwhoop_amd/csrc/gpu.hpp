@@ -204,6 +204,9 @@ int rc_tile(Workspace *ws, bwc_codec *c, Job *job, uint64_t tile, SkipCheck *chk
 int rc_trace(Workspace *ws, uint32_t *layer, uint32_t *slope, int64_t *est, int cap);
 void rc_release(Workspace *ws);
 const char *rc_last_error();
+// the header writer's stuffed byte count of n bit strings, by the device
+// rate control's kernels (string i: nbits[i] bits from word woff[i])
+int rc_stuffed_bytes(const uint32_t *words, const uint64_t *woff, const uint64_t *nbits, int n, uint64_t *bytes);
 // gather each codeblock's prefix [0, len[i]) into rec.data / rec.data_off
 int gather(Workspace *ws, CblkRecords &rec, const std::vector<uint32_t> &len);
 // Packet bodies on the device: desc holds (offset in the body buffer,

@@ -15,11 +15,11 @@
 //   scan             exclusive scan of the header bits (bit offsets)
 //   rc_write_kernel  per (probe, codeblock): its header bits into the packet's
 //                    bit string (leading 1, tag-tree bits, delta_z, length)
-//   rc_piece_kernel  per (probe, 8192-bit piece): the byte count of JPEG 2000
+//   rc_piece_kernel  per (probe, 64 Kibit piece): the byte count of JPEG 2000
 //                    stuffing (after an 0xFF byte the next holds 7 bits) for
 //                    each of the 16 states a piece can be entered in (next
 //                    byte start 0-7 bits in, 8- or 7-bit byte), composed
-//                    over the piece's words in LDS
+//                    over the threads' chunks in LDS
 //   rc_packet_kernel per (probe, packet): pieces composed from the packet's
 //                    start; 1 byte for a packet no codeblock contributes to
 // Every round evaluates the next five levels of the bisection's decision tree
@@ -54,8 +54,6 @@ const char *rc_last_error() { return rc_err; }
 namespace {
 
 constexpr int kTraceMax = 4096;
-constexpr uint32_t kPieceWords = 256;              // words per rc_piece_kernel workgroup
-constexpr uint32_t kPieceBits = kPieceWords * 32;
 constexpr uint32_t kCbSlackBits = 96;              // dz flag, delta_z and length code bound per codeblock
 
 // one codeblock in packet order
@@ -386,55 +384,86 @@ struct ProbeBufs {
   uint32_t *zl;   // L[cut - 1]
   uint32_t *nb;   // header bits (the packet's leading 1 with its first codeblock)
   uint32_t *off;  // exclusive scan of nb
-  int64_t *bd;    // body bytes the codeblock adds to the layer's estimate
+  int64_t *body;  // per (probe, packet): body bytes its codeblocks add to the layer's estimate
+  uint32_t *any;  // per (probe, packet): some codeblock contributes new passes
 };
 
-__global__ void rc_cut_kernel(const RcState *__restrict__ st, uint32_t mcap, const RcCb *__restrict__ cb,
-                              uint64_t G, const RcSlot *__restrict__ slots, const uint64_t *__restrict__ hoff,
-                              const uint32_t *__restrict__ hz, const uint32_t *__restrict__ hl,
-                              const uint32_t *__restrict__ hcnt, const uint8_t *__restrict__ hscan,
-                              const uint8_t *__restrict__ zg, const int16_t *__restrict__ prev,
-                              const uint32_t *__restrict__ plen, const uint16_t *__restrict__ beta,
-                              const uint32_t *__restrict__ mcn, const uint32_t *__restrict__ pk_first,
-                              ProbeBufs pb) {
+// zero the header bit strings and the per-packet sums of a round
+__global__ void rc_round_zero_kernel(uint32_t *bits, uint64_t nw, int64_t *body, uint32_t *any, uint64_t nk) {
+  const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < nw; k += step) bits[k] = 0;
+  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < nk; k += step) body[k] = 0, any[k] = 0;
+}
+
+// Sum v and OR a over the lanes of the wavefront whose key equals lane 0's
+// (consecutive codeblocks share a packet); lanes with another key add
+// their own. key ~0: nothing to add.
+__device__ __forceinline__ void packet_sums(uint64_t key, int64_t v, uint32_t a, int64_t *body, uint32_t *any) {
+  const uint64_t k0 = __shfl(key, 0);
+  const bool same = key == k0;
+  int64_t sv = same ? v : 0;
+  uint32_t sa = same ? a : 0u;
+  for (int d = 32; d >= 1; d >>= 1) {
+    sv += __shfl_xor(sv, d);
+    sa |= __shfl_xor(sa, d);
+  }
+  if (__lane_id() == 0 && k0 != ~0ull) {
+    if (sv) atomicAdd((unsigned long long *)&body[k0], (unsigned long long)sv);
+    if (sa) atomicOr(&any[k0], 1u);
+  }
+  if (!same && key != ~0ull) {
+    if (v) atomicAdd((unsigned long long *)&body[key], (unsigned long long)v);
+    if (a) atomicOr(&any[key], 1u);
+  }
+}
+
+__global__ void __launch_bounds__(256) rc_cut_kernel(
+    const RcState *__restrict__ st, uint32_t mcap, const RcCb *__restrict__ cb, uint64_t G, uint32_t NP,
+    const RcSlot *__restrict__ slots, const uint64_t *__restrict__ hoff, const uint32_t *__restrict__ hz,
+    const uint32_t *__restrict__ hl, const uint32_t *__restrict__ hcnt, const uint8_t *__restrict__ hscan,
+    const uint8_t *__restrict__ zg, const int16_t *__restrict__ prev, const uint32_t *__restrict__ plen,
+    const uint16_t *__restrict__ beta, const uint32_t *__restrict__ mcn, const uint32_t *__restrict__ pk_first,
+    ProbeBufs pb) {
   const uint64_t idx = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-  if (idx >= (uint64_t)mcap * G) return;
   const uint32_t t = (uint32_t)(idx / G);
   const uint64_t g = idx - (uint64_t)t * G;
-  if (t >= st->m) {
-    pb.nb[idx] = 0;
-    return;
-  }
-  const uint32_t T = st->T[t], q = st->layer;
-  const RcCb c = cb[g];
-  const RcSlot s = slots[c.j];
-  uint32_t zl;
-  const int z = cut_at(s, c.i, zg[g], hz, hl, hoff[g], hcnt[g], hscan[g], T, zl);
-  const int pv = prev[g];
-  int cur;
+  uint64_t key = ~0ull;
   int64_t bd = 0;
-  if (z < pv) {
-    cur = pv;
-    if (cur == 1) bd = s.L[(uint64_t)c.i * kMaxPasses];
-  } else {
-    cur = z;
-    if (z > 0) {
-      bd = zl;
-      if (z > 1 && pv > 0) bd -= plen[g];
+  uint32_t contrib = 0;
+  if (idx < (uint64_t)mcap * G && t >= st->m) pb.nb[idx] = 0;
+  if (idx < (uint64_t)mcap * G && t < st->m) {
+    const uint32_t T = st->T[t], q = st->layer;
+    const RcCb c = cb[g];
+    const RcSlot s = slots[c.j];
+    uint32_t zl;
+    const int z = cut_at(s, c.i, zg[g], hz, hl, hoff[g], hcnt[g], hscan[g], T, zl);
+    const int pv = prev[g];
+    int cur;
+    if (z < pv) {
+      cur = pv;
+      if (cur == 1) bd = s.L[(uint64_t)c.i * kMaxPasses];
+    } else {
+      cur = z;
+      if (z > 0) {
+        bd = zl;
+        if (z > 1 && pv > 0) bd -= plen[g];
+      }
     }
+    const int dz = cur - pv;
+    uint32_t n = (pk_first[c.P] == g) ? 1u : 0u;
+    if (pv == 0) n += (q + 1) + (cur ? mcn[g] : 0u);
+    else n += 1;
+    if (dz) {
+      const LenCode lc = len_code(beta[g], dz, zl - (pv > 0 ? plen[g] : 0u));
+      n += dz_bits(dz) + lc.ones + 1 + lc.nb;
+    }
+    pb.cur[idx] = (int16_t)cur;
+    pb.zl[idx] = zl;
+    pb.nb[idx] = n;
+    key = (uint64_t)t * NP + c.P;
+    contrib = dz > 0;
   }
-  const int dz = cur - pv;
-  uint32_t n = (pk_first[c.P] == g) ? 1u : 0u;
-  if (pv == 0) n += (q + 1) + (cur ? mcn[g] : 0u);
-  else n += 1;
-  if (dz) {
-    const LenCode lc = len_code(beta[g], dz, zl - (pv > 0 ? plen[g] : 0u));
-    n += dz_bits(dz) + lc.ones + 1 + lc.nb;
-  }
-  pb.cur[idx] = (int16_t)cur;
-  pb.zl[idx] = zl;
-  pb.nb[idx] = n;
-  pb.bd[idx] = bd;
+  packet_sums(key, bd, contrib, pb.body, pb.any);
 }
 
 // exclusive scan of n u32 values, 1024 per workgroup: local scans and the
@@ -565,41 +594,98 @@ __global__ void rc_write_kernel(const RcState *__restrict__ st, uint32_t mcap, c
   }
 }
 
-// Stuffed byte count of one 32-bit word of a packet's header string for
-// each entry state e = o | (cap == 7) << 3 (next byte starts o bits into the
-// word, holding cap bits): starts before the packet's end pe are counted, and
-// a 7-bit byte starting exactly at pe (the writer's flush after an 0xFF) too.
-__device__ __forceinline__ void word_function(const uint32_t *__restrict__ w, uint64_t ws, uint64_t pe,
-                                              uint8_t *ex, uint32_t *cn) {
-  if (ws >= pe) {
-    for (int e = 0; e < 16; ++e) ex[e] = (uint8_t)e, cn[e] = 0;
+// ---------------------------------------------------------------------------
+// byte count with 0xFF stuffing
+// ---------------------------------------------------------------------------
+// The header writer (tier2.c, bitstream.c) fills bytes MSB first; after an
+// 0xFF byte the next byte holds 7 bits. Over a range of a packet's header
+// string, the bytes it starts depend on the state it is entered in: the next
+// byte starts o = 0..7 bits in and holds 8 or 7 bits (e = o | (cap == 7) << 3).
+// A range function maps each of the 16 entry states to the exit state and the
+// bytes started in the range; starts at or past the packet's end pe are not
+// bytes, except a 7-bit byte starting exactly at pe (the writer's flush after
+// a final 0xFF). Functions of consecutive ranges compose.
+//
+// The chunk functions below work on a thread's kChunkWords words (plus one
+// word of lookahead) held in registers: wv[k] is word k of the chunk.
+constexpr int kChunkWords = 8;
+
+// whether 8 ones start at some bit p < lim of the chunk
+__device__ __forceinline__ bool chunk_has_ff(const uint32_t (&wv)[kChunkWords + 1], uint64_t lim) {
+  bool any = false;
+#pragma unroll
+  for (int k = 0; k < kChunkWords; ++k) {
+    const uint64_t x = ((uint64_t)wv[k] << 32) | (uint64_t)wv[k + 1];
+    uint64_t y = x;
+#pragma unroll
+    for (int j = 1; j < 8; ++j) y &= x << j;
+    // bit 63 - p of y: 8 ones from bit p of word k; keep 32k + p < lim
+    uint32_t m = (uint32_t)(y >> 32);
+    const int64_t room = (int64_t)lim - 32 * k;
+    if (room <= 0) m = 0;
+    else if (room < 32) m &= ~(0xFFFFFFFFu >> room);
+    any |= m != 0;
+  }
+  return any;
+}
+
+// Range function of [a, b) for entry state e when no 0xFF byte can start in
+// it: bytes at a + o, then every 8 bits after the first (c bits) one.
+__device__ __forceinline__ void plain_range(uint32_t e, uint64_t a, uint64_t b, uint64_t pe, uint8_t &ex,
+                                            uint32_t &cn) {
+  if (a >= pe) {
+    ex = (uint8_t)e, cn = 0;
     return;
   }
-  const uint64_t wi = ws >> 5;
-  // bits ws .. ws + 63 (the next word for bytes crossing the boundary)
-  const uint64_t win = ((uint64_t)w[wi] << 32) | (uint64_t)w[wi + 1];
-  const uint32_t lim = (uint32_t)((pe - ws) < 32 ? (pe - ws) : 32);
-  for (int e = 0; e < 16; ++e) {
-    uint32_t pos = e & 7, cap = (e & 8) ? 7 : 8, cnt = 0;
-    while (pos < lim) {
+  const uint64_t lim = b < pe ? b : pe;
+  uint64_t next = a + (e & 7);
+  uint32_t cap = (e & 8) ? 7 : 8, cnt = 0;
+  if (next < lim) {
+    next += cap;
+    cap = 8;
+    cnt = 1;
+    if (next < lim) {
+      const uint64_t k = (lim - 1 - next) / 8 + 1;
+      cnt += (uint32_t)k;
+      next += 8 * k;
+    }
+  }
+  if (pe <= b && next == pe && cap == 7) ++cnt;
+  ex = (uint8_t)((next >= b ? (next - b) & 7 : 0) | (cap == 7 ? 8 : 0));
+  cn = cnt;
+}
+
+// the same for a chunk starting at bit a of the packet, byte by byte (where
+// 0xFF bytes can start)
+__device__ __forceinline__ void step_chunk(const uint32_t (&wv)[kChunkWords + 1], uint32_t e, uint64_t a, uint64_t pe,
+                                           uint8_t &ex, uint32_t &cn) {
+  const uint32_t span = kChunkWords * 32;
+  const uint32_t lim = (uint32_t)(pe - a < span ? pe - a : span);   // pe > a
+  uint32_t pos = e & 7, cap = (e & 8) ? 7 : 8, cnt = 0;
+#pragma unroll
+  for (int k = 0; k < kChunkWords; ++k) {
+    const uint64_t x = ((uint64_t)wv[k] << 32) | (uint64_t)wv[k + 1];
+    while (pos < 32u * (k + 1) && pos < lim) {
       ++cnt;
-      uint32_t next = 8;
-      if (cap == 8 && ((win >> (56 - pos)) & 0xFFu) == 0xFFu) next = 7;
+      const uint32_t next = (cap == 8 && ((x >> (56 - (pos - 32u * k))) & 0xFFu) == 0xFFu) ? 7 : 8;
       pos += cap;
       cap = next;
     }
-    if (pos == lim && lim < 32 && cap == 7) ++cnt;       // flush at pe inside the word
-    else if (pos == 32 && lim == 32 && ws + 32 == pe && cap == 7) ++cnt;   // pe at the word's end
-    ex[e] = (uint8_t)(((pos >= 32 ? pos - 32 : 0) & 7) | (cap == 7 ? 8 : 0));
-    cn[e] = cnt;
   }
+  if (a + span >= pe && pos == lim && cap == 7) ++cnt;
+  ex = (uint8_t)((pos >= span ? (pos - span) & 7 : 0) | (cap == 7 ? 8 : 0));
+  cn = cnt;
 }
 
-// Per (probe, piece of kPieceWords words of one packet's header string): the
-// piece's function from entry state to (exit state, bytes), composed in LDS
-// over its words in order.
-__global__ void __launch_bounds__(256) rc_piece_kernel(const RcState *__restrict__ st, const uint32_t *__restrict__ bits,
-                                                        uint64_t capT, const uint64_t *__restrict__ pbase,
+constexpr uint64_t kPieceSpan = 256ull * kChunkWords * 32;   // bits per rc_piece_kernel workgroup
+
+// Per (probe, piece of kPieceSpan bits of one packet's header string): the
+// piece's range function, from the threads' 256-bit chunks composed in LDS;
+// a piece where no 0xFF byte can start is computed in one go. Pieces at or
+// past the packet's end are skipped (rc_packet_kernel stops before them).
+__global__ void __launch_bounds__(256) rc_piece_kernel(const RcState *__restrict__ st,
+                                                        const uint32_t *__restrict__ bits, uint64_t capT,
+                                                        const uint64_t *__restrict__ pbase,
                                                         const uint32_t *__restrict__ piece_first, uint32_t NP,
                                                         uint32_t npiece, const uint32_t *__restrict__ pk_first,
                                                         uint64_t G, ProbeBufs pb, uint8_t *__restrict__ pf_ex,
@@ -621,23 +707,50 @@ __global__ void __launch_bounds__(256) rc_piece_kernel(const RcState *__restrict
     const uint64_t r0 = (uint64_t)t * G + a0, r1 = (uint64_t)t * G + a1 - 1;
     pe = (uint32_t)(pb.off[r1] + pb.nb[r1] - pb.off[r0]);
   }
-  const uint64_t pw0 = ((uint64_t)t * capT + pbase[P]) >> 5;   // the packet's first word
-  const uint64_t wrel = (uint64_t)(b - piece_first[P]) * kPieceWords + threadIdx.x;
-  uint8_t e8[16];
-  uint32_t c32[16];
-  word_function(bits + pw0, wrel * 32, pe, e8, c32);
-  for (int e = 0; e < 16; ++e) ex[threadIdx.x][e] = e8[e], cn[threadIdx.x][e] = c32[e];
+  const uint64_t ps = (uint64_t)(b - piece_first[P]) * kPieceSpan;
+  if (ps >= pe) return;
+  const uint32_t *w = bits + (((uint64_t)t * capT + pbase[P]) >> 5);   // the packet's first word
+  const uint64_t nw = (pbase[P + 1] - pbase[P]) >> 5;                   // words of its region
+  const uint64_t ca = ps + (uint64_t)threadIdx.x * kChunkWords * 32, cb = ca + kChunkWords * 32;
+  uint32_t wv[kChunkWords + 1];
+#pragma unroll
+  for (int k = 0; k <= kChunkWords; ++k) {
+    const uint64_t wi = (ca >> 5) + k;
+    wv[k] = (ca < pe && wi < nw) ? w[wi] : 0u;
+  }
+  const bool hot = ca < pe && chunk_has_ff(wv, pe - ca);
+  if (!__syncthreads_or(hot)) {
+    if (threadIdx.x < 16) {
+      uint8_t e8;
+      uint32_t c32;
+      plain_range(threadIdx.x, ps, ps + kPieceSpan, pe, e8, c32);
+      const uint64_t o = ((uint64_t)t * npiece + b) * 16 + threadIdx.x;
+      pf_ex[o] = e8;
+      pf_cn[o] = c32;
+    }
+    return;
+  }
+  for (uint32_t e = 0; e < 16; ++e) {
+    uint8_t e8;
+    uint32_t c32;
+    if (hot) step_chunk(wv, e, ca, pe, e8, c32);
+    else plain_range(e, ca, cb, pe, e8, c32);
+    ex[threadIdx.x][e] = e8;
+    cn[threadIdx.x][e] = c32;
+  }
   __syncthreads();
   for (uint32_t d = 1; d < 256; d <<= 1) {
-    if ((threadIdx.x & (2 * d - 1)) == 0) {
+    uint8_t e8[16];
+    uint32_t c32[16];
+    const bool lead = (threadIdx.x & (2 * d - 1)) == 0;
+    if (lead)
       for (int e = 0; e < 16; ++e) {
         const uint8_t m = ex[threadIdx.x][e];
         e8[e] = ex[threadIdx.x + d][m];
         c32[e] = cn[threadIdx.x][e] + cn[threadIdx.x + d][m];
       }
-    }
     __syncthreads();
-    if ((threadIdx.x & (2 * d - 1)) == 0)
+    if (lead)
       for (int e = 0; e < 16; ++e) ex[threadIdx.x][e] = e8[e], cn[threadIdx.x][e] = c32[e];
     __syncthreads();
   }
@@ -648,42 +761,36 @@ __global__ void __launch_bounds__(256) rc_piece_kernel(const RcState *__restrict
   }
 }
 
-// Per (probe, packet), one wavefront: whether any codeblock contributes, the
-// body bytes, and the header bytes from its pieces composed from the start
-// (a packet no codeblock contributes to counts 1 byte, tier2.c:447-459).
+// Per (probe, packet), one wavefront: the header bytes from its pieces
+// composed from the packet's start, or 1 byte for a packet no codeblock
+// contributes to (tier2.c:447-459), plus the body bytes its codeblocks add;
+// into the probe's estimate (and per packet into pkt when given).
 __global__ void __launch_bounds__(64) rc_packet_kernel(RcState *st, const uint32_t *__restrict__ piece_first,
                                                        uint32_t npiece, const uint32_t *__restrict__ pk_first,
-                                                       uint64_t G, const int16_t *__restrict__ prev, ProbeBufs pb,
+                                                       uint64_t G, uint32_t NP, ProbeBufs pb,
                                                        const uint8_t *__restrict__ pf_ex,
-                                                       const uint32_t *__restrict__ pf_cn) {
+                                                       const uint32_t *__restrict__ pf_cn, int64_t *pkt) {
   __shared__ uint8_t sex[64][16];
   __shared__ uint32_t scn[64][16];
   const uint32_t t = blockIdx.y, P = blockIdx.x;
   if (t >= st->m) return;
-  const uint64_t a0 = pk_first[P], a1 = pk_first[P + 1];
-  int64_t body = 0;
-  int any = 0;
-  for (uint64_t g = a0 + threadIdx.x; g < a1; g += 64) {
-    const uint64_t r = (uint64_t)t * G + g;
-    body += pb.bd[r];
-    any |= pb.cur[r] > prev[g];
-  }
-  for (int d = 32; d >= 1; d >>= 1) {
-    body += __shfl_xor(body, d);
-    any |= __shfl_xor(any, d);
-  }
+  const uint64_t key = (uint64_t)t * NP + P;
   int64_t bytes = 1;
-  if (any) {
+  if (pb.any[key]) {
+    const uint64_t a0 = pk_first[P], a1 = pk_first[P + 1];
+    const uint64_t r0 = (uint64_t)t * G + a0, r1 = (uint64_t)t * G + a1 - 1;
+    const uint64_t pe = (uint32_t)(pb.off[r1] + pb.nb[r1] - pb.off[r0]);
+    const uint32_t k1 = piece_first[P] + (uint32_t)((pe + kPieceSpan - 1) / kPieceSpan);
     uint32_t e = 0, total = 0;
-    for (uint32_t k0 = piece_first[P]; k0 < piece_first[P + 1]; k0 += 64) {
+    for (uint32_t k0 = piece_first[P]; k0 < k1; k0 += 64) {
       const uint32_t k = k0 + threadIdx.x;
-      if (k < piece_first[P + 1]) {
+      if (k < k1) {
         const uint64_t o = ((uint64_t)t * npiece + k) * 16;
         for (int s = 0; s < 16; ++s) sex[threadIdx.x][s] = pf_ex[o + s], scn[threadIdx.x][s] = pf_cn[o + s];
       }
       __syncthreads();
       if (threadIdx.x == 0) {
-        const uint32_t kn = piece_first[P + 1] - k0 < 64 ? piece_first[P + 1] - k0 : 64;
+        const uint32_t kn = k1 - k0 < 64 ? k1 - k0 : 64;
         for (uint32_t x = 0; x < kn; ++x) {
           total += scn[x][e];
           e = sex[x][e];
@@ -693,7 +800,11 @@ __global__ void __launch_bounds__(64) rc_packet_kernel(RcState *st, const uint32
     }
     bytes = total;
   }
-  if (threadIdx.x == 0) atomicAdd((unsigned long long *)&st->f[t], (unsigned long long)(bytes + body));
+  if (threadIdx.x == 0) {
+    const int64_t v = bytes + pb.body[key];
+    if (pkt) pkt[key] = bytes;
+    atomicAdd((unsigned long long *)&st->f[t], (unsigned long long)v);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -816,7 +927,7 @@ struct RcCtx {
     size_t cap = 0;
   };
   Buf slots, hoff, hz, hl, hcnt, hscan, kms, zg, prev, plen, beta, bad, nodeval, mcw, mcn, cpo, lcut, pbase,
-      piece_first, cur, zl, nb, off, bd, bsum, bits, pf_ex, pf_cn;
+      piece_first, cur, zl, nb, off, pbody, pany, bsum, bits, pf_ex, pf_cn, pkt;
   uint8_t *h_io = nullptr;        // page-locked staging of uploads and results
   size_t cap_io = 0;
   ~RcCtx() {
@@ -826,7 +937,7 @@ struct RcCtx {
       if (g.d_parent) (void)hipFree(g.d_parent);
     }
     Buf *bs[] = {&slots, &hoff, &hz, &hl, &hcnt, &hscan, &kms, &zg, &prev, &plen, &beta, &bad, &nodeval, &mcw,
-                 &mcn, &cpo, &lcut, &pbase, &piece_first, &cur, &zl, &nb, &off, &bd, &bsum, &bits, &pf_ex, &pf_cn};
+                 &mcn, &cpo, &lcut, &pbase, &piece_first, &cur, &zl, &nb, &off, &pbody, &pany, &bsum, &bits, &pf_ex, &pf_cn, &pkt};
     for (Buf *b : bs)
       if (b->p) (void)hipFree(b->p);
     if (d_st) (void)hipFree(d_st);
@@ -917,6 +1028,84 @@ int rc_trace(Workspace *ws, uint32_t *layer, uint32_t *slope, int64_t *est, int 
   return n;
 }
 
+// Byte counts of n bit strings under the header writer's 0xFF stuffing, by
+// the kernels rate control counts header estimates with (each string a
+// one-codeblock packet of a single probe): string i is nbits[i] bits MSB
+// first from word woff[i] of words (big-endian 32-bit words, host).
+int rc_stuffed_bytes(const uint32_t *words, const uint64_t *woff, const uint64_t *nbits, int n, uint64_t *bytes) {
+  if (n <= 0) return 0;
+  std::vector<uint64_t> pbase(n + 1), hw;
+  std::vector<uint32_t> pfirst(n + 1, 0), pk(n + 1), nb(n), off(n, 0), any(n, 1);
+  uint64_t tot = 0;
+  for (int i = 0; i < n; ++i) {
+    if (nbits[i] >= (1ull << 32) || woff[i + 1] < woff[i] + (nbits[i] + 31) / 32) {
+      std::snprintf(rc_err, sizeof rc_err, "bit string %d: bad length", i);
+      return 1;
+    }
+    const uint64_t cw = woff[i + 1] - woff[i] + 2;
+    pbase[i] = tot * 32;
+    tot += cw;
+    pfirst[i + 1] = pfirst[i] + (uint32_t)((cw * 32 + kPieceSpan - 1) / kPieceSpan);
+    pk[i] = (uint32_t)i;
+    nb[i] = (uint32_t)nbits[i];
+  }
+  pk[n] = (uint32_t)n;
+  pbase[n] = tot * 32;
+  hw.assign(tot + 2, 0);
+  std::vector<uint32_t> wbuf((tot + 2) * 2, 0);
+  for (int i = 0; i < n; ++i)
+    std::memcpy(&wbuf[pbase[i] / 32], words + woff[i], (woff[i + 1] - woff[i]) * 4);
+  const uint32_t npiece = pfirst[n];
+  RcState *st = nullptr;
+  uint32_t *d_w = nullptr, *d_pf = nullptr, *d_pk = nullptr, *d_nb = nullptr, *d_off = nullptr, *d_any = nullptr,
+           *d_cn = nullptr;
+  uint64_t *d_pb = nullptr;
+  uint8_t *d_ex = nullptr;
+  int64_t *d_body = nullptr, *d_pkt = nullptr;
+  int rc = 1;
+  do {
+    RcState h{};
+    h.m = 1;
+    if (hipMalloc((void **)&st, sizeof(RcState)) != hipSuccess ||
+        hipMalloc((void **)&d_w, wbuf.size() * 4) != hipSuccess ||
+        hipMalloc((void **)&d_pf, (n + 1) * 4) != hipSuccess || hipMalloc((void **)&d_pk, (n + 1) * 4) != hipSuccess ||
+        hipMalloc((void **)&d_nb, n * 4) != hipSuccess || hipMalloc((void **)&d_off, n * 4) != hipSuccess ||
+        hipMalloc((void **)&d_any, n * 4) != hipSuccess || hipMalloc((void **)&d_pb, (n + 1) * 8) != hipSuccess ||
+        hipMalloc((void **)&d_ex, (uint64_t)npiece * 16) != hipSuccess ||
+        hipMalloc((void **)&d_cn, (uint64_t)npiece * 64) != hipSuccess ||
+        hipMalloc((void **)&d_body, n * 8) != hipSuccess || hipMalloc((void **)&d_pkt, n * 8) != hipSuccess)
+      break;
+    std::vector<int64_t> zero(n, 0);
+    if (hipMemcpy(st, &h, sizeof h, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_w, wbuf.data(), wbuf.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_pf, pfirst.data(), (n + 1) * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_pk, pk.data(), (n + 1) * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_nb, nb.data(), n * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_off, off.data(), n * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_any, any.data(), n * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_pb, pbase.data(), (n + 1) * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_body, zero.data(), n * 8, hipMemcpyHostToDevice) != hipSuccess)
+      break;
+    ProbeBufs pb{nullptr, nullptr, d_nb, d_off, d_body, d_any};
+    const uint64_t capT = pbase[n] + 64;
+    hipLaunchKernelGGL(rc_piece_kernel, dim3(npiece, 1), dim3(256), 0, 0, st, d_w, capT, d_pb, d_pf, (uint32_t)n,
+                       npiece, d_pk, (uint64_t)n, pb, d_ex, d_cn);
+    hipLaunchKernelGGL(rc_packet_kernel, dim3((uint32_t)n, 1), dim3(64), 0, 0, st, d_pf, npiece, d_pk, (uint64_t)n,
+                       (uint32_t)n, pb, d_ex, d_cn, d_pkt);
+    std::vector<int64_t> out(n);
+    if (hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(out.data(), d_pkt, n * 8, hipMemcpyDeviceToHost) != hipSuccess)
+      break;
+    for (int i = 0; i < n; ++i) bytes[i] = (uint64_t)out[i];
+    rc = 0;
+  } while (0);
+  if (rc) std::snprintf(rc_err, sizeof rc_err, "stuffed byte count: %s", hipGetErrorString(hipGetLastError()));
+  void *ps[] = {st, d_w, d_pf, d_pk, d_nb, d_off, d_any, d_pb, d_ex, d_cn, d_body, d_pkt};
+  for (void *q : ps)
+    if (q) (void)hipFree(q);
+  return rc;
+}
+
 int rc_tile(Workspace *ws, bwc_codec *c, Job *job, uint64_t ti, SkipCheck *chk) {
   if (!ws->rc) ws->rc = new RcCtx();
   RcCtx &x = *static_cast<RcCtx *>(ws->rc);
@@ -963,7 +1152,7 @@ int rc_tile(Workspace *ws, bwc_codec *c, Job *job, uint64_t ti, SkipCheck *chk) 
   for (uint64_t P = 0; P < NP; ++P) {
     const uint64_t cw = (pcap[P] + 31) / 32 + 2;   // whole words, and room for a window past the end
     pbase[P + 1] = pbase[P] + cw * 32;
-    pfirst[P + 1] = pfirst[P] + (uint32_t)((cw + kPieceWords - 1) / kPieceWords);
+    pfirst[P + 1] = pfirst[P] + (uint32_t)((cw * 32 + kPieceSpan - 1) / kPieceSpan);
   }
   const uint64_t capT = pbase[NP] + 64;
   const uint32_t npiece = pfirst[NP];
@@ -978,7 +1167,7 @@ int rc_tile(Workspace *ws, bwc_codec *c, Job *job, uint64_t ti, SkipCheck *chk) 
       grow(x.nodeval, g.N * 4) || grow(x.mcw, G * (uint64_t)stride * 4) || grow(x.mcn, G * 4) ||
       grow(x.cpo, G * nl * 2) || grow(x.lcut, G * nl * 4) || grow(x.pbase, (NP + 1) * 8) ||
       grow(x.piece_first, (NP + 1) * 4) || grow(x.cur, MG * 2) || grow(x.zl, MG * 4) || grow(x.nb, MG * 4) ||
-      grow(x.off, MG * 4) || grow(x.bd, MG * 8) || grow(x.bsum, (MG / 1024 + 2) * 4) ||
+      grow(x.off, MG * 4) || grow(x.pbody, (uint64_t)mmax * NP * 8) || grow(x.pany, (uint64_t)mmax * NP * 4) || grow(x.bsum, (MG / 1024 + 2) * 4) ||
       grow(x.bits, mmax * capT / 8 + 64) || grow(x.pf_ex, (uint64_t)mmax * npiece * 16) ||
       grow(x.pf_cn, (uint64_t)mmax * npiece * 64))
     return 1;
@@ -1030,7 +1219,7 @@ int rc_tile(Workspace *ws, bwc_codec *c, Job *job, uint64_t ti, SkipCheck *chk) 
   const uint64_t *dpbase = ptr<uint64_t>(x.pbase);
   const uint32_t *dpfirst = ptr<uint32_t>(x.piece_first);
   ProbeBufs pb{ptr<int16_t>(x.cur), ptr<uint32_t>(x.zl), ptr<uint32_t>(x.nb), ptr<uint32_t>(x.off),
-               ptr<int64_t>(x.bd)};
+               ptr<int64_t>(x.pbody), ptr<uint32_t>(x.pany)};
 
   hipLaunchKernelGGL(rc_state_init_kernel, dim3(1), dim3(1), 0, s, st, (int64_t)hdr);
   if (g.N) hipLaunchKernelGGL(rc_fill_u32_kernel, dim3(blocks(g.N, 256)), dim3(256), 0, s, dnodeval, g.N, 0xFFFFu);
@@ -1046,10 +1235,10 @@ int rc_tile(Workspace *ws, bwc_codec *c, Job *job, uint64_t ti, SkipCheck *chk) 
   auto probe_round = [&](uint32_t mcap) -> int {
     if (!G) return 0;
     const uint64_t n = (uint64_t)mcap * G;
-    hipLaunchKernelGGL(rc_fill_u32_kernel, dim3(blocks(mcap * capT / 32, 256 * 8)), dim3(256), 0, s, dbits,
-                       mcap * capT / 32 + 2, 0u);
-    hipLaunchKernelGGL(rc_cut_kernel, dim3(blocks(n, 256)), dim3(256), 0, s, st, mcap, g.d_cb, G, dslots, dhoff,
-                       dhz, dhl, dhcnt, dhscan, dzg, dprev, dplen, dbeta, dmcn, g.d_pk_first, pb);
+    hipLaunchKernelGGL(rc_round_zero_kernel, dim3(blocks(mcap * capT / 32, 256 * 8)), dim3(256), 0, s, dbits,
+                       mcap * capT / 32 + 2, pb.body, pb.any, (uint64_t)mcap * NP);
+    hipLaunchKernelGGL(rc_cut_kernel, dim3(blocks(n, 256)), dim3(256), 0, s, st, mcap, g.d_cb, G, (uint32_t)NP,
+                       dslots, dhoff, dhz, dhl, dhcnt, dhscan, dzg, dprev, dplen, dbeta, dmcn, g.d_pk_first, pb);
     const uint64_t nblk = (n + 1023) / 1024;
     hipLaunchKernelGGL(rc_scan_local_kernel, dim3((uint32_t)nblk), dim3(256), 0, s, pb.nb, pb.off, n, dbsum);
     hipLaunchKernelGGL(rc_scan_top_kernel, dim3(1), dim3(1024), 0, s, dbsum, nblk);
@@ -1059,7 +1248,7 @@ int rc_tile(Workspace *ws, bwc_codec *c, Job *job, uint64_t ti, SkipCheck *chk) 
     hipLaunchKernelGGL(rc_piece_kernel, dim3(npiece, mcap), dim3(256), 0, s, st, dbits, capT, dpbase, dpfirst,
                        (uint32_t)NP, npiece, g.d_pk_first, G, pb, dpf_ex, dpf_cn);
     hipLaunchKernelGGL(rc_packet_kernel, dim3((uint32_t)NP, mcap), dim3(64), 0, s, st, dpfirst, npiece,
-                       g.d_pk_first, G, dprev, pb, dpf_ex, dpf_cn);
+                       g.d_pk_first, G, (uint32_t)NP, pb, dpf_ex, dpf_cn, (int64_t *)nullptr);
     HIP_RC(hipGetLastError());
     return 0;
   };

@@ -161,6 +161,13 @@ int bwc_amd_cblk_cuts(bwc_codec *codec, uint64_t tile, uint8_t param, int16_t *c
    header bytes); at most cap entries written, the count returned (-1 for a
    codec that has not compressed). */
 int bwc_amd_rc_trace(bwc_codec *codec, uint32_t *layer, uint32_t *slope, int64_t *est, int cap);
+/* Test entry point of the device rate control's header-size count: the
+   bytes the packet-header writer emits for each of n bit strings (string i:
+   nbits[i] bits, MSB first, from 32-bit big-endian word woff[i] of words;
+   woff has n + 1 entries), with its 0xFF stuffing (after an 0xFF byte the
+   next byte holds 7 bits; tier2.c / bitstream.c). Returns 0 on success. */
+int bwc_amd_stuffed_bytes(const uint32_t *words, const uint64_t *woff, const uint64_t *nbits, int n,
+                          uint64_t *bytes);
 /* Keep the pass-limit network's 16 features per codeblock during
    bwc_compress (on != 0), and read them for (tile, param) afterwards
    (ncblk x 16 floats; feature 15 relative to the tile mean). */

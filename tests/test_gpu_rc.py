@@ -111,3 +111,62 @@ def test_skip_check_on_device_matches_host(which, margin, monkeypatch):
     assert sa == sb
     ref = ol.oracle_compress(data, shape, **kw)
     assert a == ref
+
+
+def _stuffed_bytes(bits):
+    """Bytes the packet-header writer emits for a bit string: MSB-first
+    bytes, a byte after an 0xFF one holding 7 bits, and a last byte for
+    pending bits or for a 7-bit byte due after a final 0xFF."""
+    n, pos, cap, N = 0, 0, 8, len(bits)
+    while pos < N:
+        n += 1
+        nxt = 7 if cap == 8 and pos + 8 <= N and bits[pos:pos + 8].all() else 8
+        pos += cap
+        cap = nxt
+    if pos == N and cap == 7:
+        n += 1
+    return n
+
+
+def test_stuffed_byte_count_matches_writer():
+    rng = np.random.default_rng(5)
+    strings = []
+    for k in range(160):
+        n = int(rng.choice([1, 7, 8, 9, 15, 16, 17, 31, 32, 33, 63, 64, 65])) if k < 40 else \
+            int(rng.integers(1, 150_000))
+        kind = k % 4
+        if kind == 0:     # mostly ones: 0xFF bytes everywhere
+            b = rng.random(n) < 0.97
+        elif kind == 1:   # runs of ones and zeros
+            b = np.repeat(rng.integers(0, 2, n // 5 + 1).astype(bool), rng.integers(1, 40, n // 5 + 1))[:n]
+            if len(b) < n:
+                b = np.concatenate([b, np.ones(n - len(b), bool)])
+        elif kind == 2:   # all ones
+            b = np.ones(n, bool)
+        else:             # mostly zeros with some 8-bit runs of ones
+            b = rng.random(n) < 0.02
+            for s in rng.integers(0, n, max(1, n // 200)):
+                b[s:s + 8] = True
+        strings.append(b.astype(np.uint8))
+    woff = [0]
+    words = []
+    for b in strings:
+        nw = (len(b) + 31) // 32
+        pad = np.zeros(nw * 32, np.uint8)
+        pad[:len(b)] = b
+        w = np.packbits(pad).view(">u4").astype(np.uint32)
+        words.append(w)
+        woff.append(woff[-1] + nw + int(rng.integers(0, 3)))
+        words.append(np.zeros(woff[-1] - woff[-2] - nw, np.uint32))
+    wa = np.ascontiguousarray(np.concatenate(words), dtype=np.uint32)
+    wo = np.array(woff, np.uint64)
+    nb = np.array([len(b) for b in strings], np.uint64)
+    out = np.zeros(len(strings), np.uint64)
+    import ctypes as C
+    rc = bwc.lib().bwc_amd_stuffed_bytes(wa.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                         wo.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                         nb.ctypes.data_as(C.POINTER(C.c_uint64)), len(strings),
+                                         out.ctypes.data_as(C.POINTER(C.c_uint64)))
+    assert rc == 0, bwc.last_error()
+    want = [_stuffed_bytes(b.astype(bool)) for b in strings]
+    assert out.tolist() == want
