@@ -215,11 +215,22 @@ def main():
     dt_dv, cs_dv, dwt_dv = leg("learned pass skip, device input", dv, field.data_ptr())
     ms_dev, _ = dv.timings()
     dv.close()
+    # The DWT stage timed with HIP events around its launches on the
+    # library's stream: the same device-input compression with pipeline
+    # graphs off (with them on, the DWT is recorded into tier-1's graph and
+    # has no events of its own); these calls also give the per-stage times
+    os.environ["BWC_AMD_GRAPH"] = "0"
+    tm = learned_codec()
+    _, cs_tm, dwt_tm = leg("learned pass skip, device input, no graphs (stage timing)", tm, field.data_ptr())
+    ms_stage, _ = tm.timings()
+    tm.close()
+    del os.environ["BWC_AMD_GRAPH"]
     # plain encoder (every pass of every codeblock coded), page-locked host input
     plain = bwc.Codec((n, n, n), pinned=True, **SETTINGS)
     dt_plain, cs_plain, dwt_plain = leg("plain encoder, page-locked host input", plain, host.data_ptr())
     plain.close()
-    for name, other in (("pageable-input", cs_pg), ("device-input", cs_dv), ("plain-encoder", cs_plain)):
+    for name, other in (("pageable-input", cs_pg), ("device-input", cs_dv), ("plain-encoder", cs_plain),
+                        ("no-graph", cs_tm)):
         if other != cs:
             print(f"bench: {name} codestream differs from the headline one", file=sys.stderr)
             sys.exit(1)
@@ -253,8 +264,7 @@ def main():
     value = bytes_in * args.steps / dt / 1e9
     plain_gbs = bytes_in * args.steps / dt_plain / 1e9
     alg = dwt_algorithmic_bytes(n, 5)
-    dwt_all = dwt_ms + dwt_dv + dwt_plain
-    dwt_avg = sum(dwt_all) / len(dwt_all)
+    dwt_avg = sum(dwt_tm) / len(dwt_tm)
     achieved = alg / (dwt_avg / 1e3) / 1e9
     cpu = None if args.no_cpu_baseline else cpu_baseline()
     stage_names = ["fill_minmax", "normalize", "dwt", "quantize", "tier1", "records_d2h", "rate_control",
@@ -280,7 +290,8 @@ def main():
                            "ms_per_step": round(dt_pg / args.steps * 1e3, 3)},
         "device_input": {"value": round(bytes_in * args.steps / dt_dv / 1e9, 4), "unit": "GB/s",
                          "ms_per_step": round(dt_dv / args.steps * 1e3, 3),
-                         "stages_ms": {k: round(v, 3) for k, v in zip(stage_names, ms_dev)}},
+                         "stages_ms": {k: round(v, 3) for k, v in zip(stage_names, ms_dev)},
+                         "stages_ms_without_graphs": {k: round(v, 3) for k, v in zip(stage_names, ms_stage)}},
         "decompress": {"value": round(bytes_in * args.steps / dt_dec / 1e9, 4), "unit": "GB/s",
                        "ms_per_step": round(dt_dec / args.steps * 1e3, 3), "psnr_db": round(psnr_db, 3),
                        "output": "device memory", "identical_to_reference": dec_ok},

@@ -111,6 +111,7 @@ def lib():
             "bwc_amd_skip_stats": (C.c_int, [vp, C.POINTER(u64)]),
             "bwc_amd_cblk_cuts": (C.c_int, [vp, u64, u8, C.POINTER(C.c_int16)]),
             "bwc_amd_rc_trace": (C.c_int, [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(C.c_int64), C.c_int]),
+            "bwc_amd_graph_counts": (C.c_int, [vp, C.POINTER(u64)]),
             "bwc_amd_stuffed_bytes": (C.c_int, [C.POINTER(u32), C.POINTER(u64), C.POINTER(u64), C.c_int,
                                                 C.POINTER(u64)]),
             "bwc_amd_collect_features": (C.c_int, [vp, C.c_int]),
@@ -382,6 +383,13 @@ class Codec:
         if lib().bwc_amd_cblk_cuts(self.h, tile, param, _p(out, C.c_int16)):
             raise RuntimeError("bwc_amd_cblk_cuts failed")
         return out
+
+    def graph_counts(self):
+        """(graphs recorded, graph launches) of this codec's pipeline graphs."""
+        out = (C.c_uint64 * 2)()
+        if lib().bwc_amd_graph_counts(self.h, out):
+            return (0, 0)
+        return int(out[0]), int(out[1])
 
     def rc_trace(self, cap=4096):
         """(layer, threshold, estimate) of every probe the device PCRD

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <atomic>
 #include <mutex>
 #include <string>
@@ -645,22 +646,16 @@ size_t bwc_compress(bwc_codec *const codec, bwc_stream *const stream) {
       if (gpu::fill_minmax(ws, codec, tl, p, stream->inp, on_dev, ws->d_work)) goto fail;
       gpu::record(ws, 1);
       // normalisation is fused into the DWT's first pass (stage time 0 here)
+      // With pipeline graphs the DWT is launched as the head of tier-1's
+      // device chain (recorded with it; its stage time is then counted in
+      // tier-1's), otherwise on its own.
+      const bool chain = gpu::graphs_on() && !job->collect_features;
+      const std::function<int()> dwt_launch_fn = [&]() {
+        return gpu::dwt_forward_direct(ws, codec, par, ws->d_work, ctl.Qm);
+      };
       gpu::record(ws, 2);
-      if (gpu::dwt_forward(ws, codec, par, ws->d_work, ctl.Qm)) goto fail;
+      if (!chain && gpu::dwt_forward(ws, codec, par, ws->d_work, ctl.Qm)) goto fail;
       gpu::record(ws, 3);
-      {
-        double mm[4];
-        if (gpu::read_norm(ws, mm)) goto fail;
-        par.pmin = mm[0];
-        par.pmax = mm[1];
-        par.alpha = mm[2];
-        par.beta = mm[3];
-      }
-      ms[0] += gpu::elapsed(ws, 0, 1);
-      ms[1] += gpu::elapsed(ws, 1, 2);
-      ms[2] += gpu::elapsed(ws, 2, 3);
-      dwt_launch += ws->last_dwt_launches;
-      gpu::record(ws, 4);
       {
         CblkRecords &R = job->rec[ti * codec->info.nPar + p];
         std::vector<uint8_t> zl(par.ncb, 255);
@@ -675,7 +670,20 @@ size_t bwc_compress(bwc_codec *const codec, bwc_stream *const stream) {
         }
         pl.features = job->collect_features;
         pl.records = !rc_dev;
+        if (chain) {
+          pl.pre = &dwt_launch_fn;
+          pl.pre_key = gpu::dwt_launch_key(ws, codec, par, ws->d_work, ctl.Qm) | 1;
+        }
+        gpu::record(ws, 4);
         if (gpu::t1_encode(ws, codec, par, ws->d_work, R, &tl.slope_min, &tl.slope_max, p, pl)) goto fail;
+        {
+          double mm[4];
+          if (gpu::read_norm(ws, mm)) goto fail;
+          par.pmin = mm[0];
+          par.pmax = mm[1];
+          par.alpha = mm[2];
+          par.beta = mm[3];
+        }
         R.zlim = zl;
         if (job->collect_features) {
           R.feat.resize(par.ncb * 16);
@@ -690,6 +698,10 @@ size_t bwc_compress(bwc_codec *const codec, bwc_stream *const stream) {
       }
       gpu::record(ws, 5);
       gpu::sync(ws);
+      ms[0] += gpu::elapsed(ws, 0, 1);
+      ms[1] += gpu::elapsed(ws, 1, 2);
+      ms[2] += gpu::elapsed(ws, 2, 3);
+      dwt_launch += ws->last_dwt_launches;
       ms[4] += gpu::elapsed(ws, 4, 5);
     }
     {
@@ -1209,6 +1221,15 @@ int bwc_amd_stuffed_bytes(const uint32_t *words, const uint64_t *woff, const uin
     set_err(gpu::rc_last_error());
     return 1;
   }
+  return 0;
+}
+
+int bwc_amd_graph_counts(bwc_codec *codec, uint64_t counts[2]) {
+  Job *job = job_of(codec);
+  if (!job || !job->ws) return 1;
+  const gpu::Workspace *ws = static_cast<const gpu::Workspace *>(job->ws);
+  counts[0] = ws->graph_records;
+  counts[1] = ws->graph_replays;
   return 0;
 }
 

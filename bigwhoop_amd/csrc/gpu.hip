@@ -98,6 +98,8 @@ Workspace::~Workspace() {
   if (stream) (void)hipStreamSynchronize(stream);
   for (auto &g : dwt_graphs)
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  for (auto &g : graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -971,6 +973,54 @@ int dwt_forward(Workspace *ws, const bwc_codec *c, const Param &par, double *wor
   if (ws->dwt_graphs.size() < 64) ws->dwt_graphs.push_back({dwt_key(ws, c, par, work, norm_qm), nullptr, 0});
   return 0;
 }
+int dwt_forward_direct(Workspace *ws, const bwc_codec *c, const Param &par, double *work, int norm_qm) {
+  return dwt_run(ws, c, par, work, true, norm_qm);
+}
+uint64_t dwt_launch_key(const Workspace *ws, const bwc_codec *c, const Param &par, const double *work, int norm_qm) {
+  return dwt_key(ws, c, par, work, norm_qm);
+}
+
+bool graphs_on() {
+  const char *e = std::getenv("BWC_AMD_GRAPH");
+  return !(e && e[0] == '0');
+}
+
+int graph_run(Workspace *ws, uint64_t key, const std::function<int()> &launch) {
+  for (Workspace::Graph &g : ws->graphs) {
+    if (g.key != key) continue;
+    if (!g.exec) {
+      hipGraph_t graph = nullptr;
+      HIP_OK(hipStreamBeginCapture(ws->stream, hipStreamCaptureModeThreadLocal));
+      const int rc = launch();
+      const hipError_t e = hipStreamEndCapture(ws->stream, &graph);
+      if (rc || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        if (!rc) std::snprintf(g_err, sizeof g_err, "graph capture: %s", hipGetErrorString(e));
+        return 1;
+      }
+      const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ei != hipSuccess) {
+        g.exec = nullptr;
+        std::snprintf(g_err, sizeof g_err, "graph instantiation: %s", hipGetErrorString(ei));
+        return 1;
+      }
+      ++ws->graph_records;
+    }
+    HIP_OK(hipGraphLaunch(g.exec, ws->stream));
+    ++ws->graph_replays;
+    return 0;
+  }
+  CHK(launch());
+  if (ws->graphs.size() >= 256) {
+    for (auto &g : ws->graphs)
+      if (g.exec) HIP_OK(hipGraphExecDestroy(g.exec));
+    ws->graphs.clear();
+  }
+  ws->graphs.push_back({key, nullptr});
+  return 0;
+}
+
 int dwt_inverse(Workspace *ws, const bwc_codec *c, const Param &par, double *work) {
   return dwt_run(ws, c, par, work, false, -1);
 }
@@ -2397,17 +2447,21 @@ static int fetch_rows(Workspace *ws, CblkRecords &rec, const std::vector<uint64_
 static int t1_code_dev(Workspace *ws, const bwc_codec *c, const Param &par, const uint8_t *dz, uint8_t *d_data,
                        const uint8_t *d_bp, const uint32_t *d_hist, const GenParams &gp, CblkRecords &rec,
                        std::vector<uint8_t> &zh, std::vector<uint64_t> &list, std::vector<uint32_t> &m,
-                       uint32_t hfl[4], bool records) {
+                       uint32_t hfl[4], bool records, const std::function<int()> *pre, uint64_t pre_key) {
   const uint64_t ncb = par.ncb;
   if (!ws->d_t1info) HIP_OK(hipMalloc((void **)&ws->d_t1info, 4 * sizeof(uint64_t)));
   CHK(kz_room(ws, ncb));
   uint64_t info[4] = {0, 0, 0, 0};
   PackLayout lo{};
   double tm = trace_mark(nullptr, 0);
-  for (int attempt = 0;; ++attempt) {
-    CHK(v2_buffers(ws, ncb, ws->cap_sym, ncb));
-    lo = pack_layout(ncb, ws->cap_pass);
-    CHK(pack_alloc(ws, lo, ncb));
+  CHK(v2_buffers(ws, ncb, ws->cap_sym, ncb));
+  lo = pack_layout(ncb, ws->cap_pass);
+  CHK(pack_alloc(ws, lo, ncb));
+  // the device chain: what comes before it (DWT, bitplanes, pass limits),
+  // the list, generation, MQ coding, [record packing], and the copies of the
+  // totals, flags, K, the pass limits, Z and Kmsbs to the host
+  auto chain = [&](bool with_pre) -> int {
+    if (with_pre && pre) CHK((*pre)());
     hipLaunchKernelGGL(t1_list_kernel, dim3(1), dim3(1024), 0, ws->stream, ncb, ws->d_K, dz, ws->d_cbbox,
                        ws->d_qoff, (uint64_t)ws->cap_sym, ws->cap_pass, ws->d_list, ws->d_symoff, ws->d_poff,
                        ws->d_t1info);
@@ -2425,7 +2479,32 @@ static int t1_code_dev(Workspace *ws, const bwc_codec *c, const Param &par, cons
       HIP_OK(hipMemcpyAsync(ws->h_kz + 2 * ncb, ws->d_Z, ncb, hipMemcpyDeviceToHost, ws->stream));
       HIP_OK(hipMemcpyAsync(ws->h_kz + 3 * ncb, ws->d_Kmsbs, ncb, hipMemcpyDeviceToHost, ws->stream));
     }
-    if (attempt == 0) tm = trace_mark("list + offsets + launch", tm);
+    return 0;
+  };
+  if (pre_key) {
+    // everything the chain's launches depend on
+    uint64_t h = pre_key ^ 0x9E3779B97F4A7C15ull;
+    auto mix = [&h](uint64_t v) { h = (h ^ v) * 0x100000001B3ull + 0x9E3779B97F4A7C15ull; };
+    const void *ptrs[] = {d_data, d_bp, d_hist, dz, ws->d_K, ws->d_Kmsbs, ws->d_Z, ws->d_U, ws->d_L, ws->d_S,
+                          ws->d_mse, ws->d_bfut, ws->d_cbbox, ws->d_cbsb, ws->d_cbmeta, ws->d_qoff, ws->d_doff,
+                          ws->d_bpoff, ws->d_snap, ws->d_flags, ws->d_t1info, ws->d_list, ws->d_symoff, ws->d_sym,
+                          ws->d_pcnt, ws->d_pmse, ws->d_poff, ws->d_rpack, ws->h_down, ws->h_kz, ws->d_feat,
+                          ws->d_tstat, ws->d_skipw, ws->d_zlim};
+    for (const void *p : ptrs) mix((uint64_t)(uintptr_t)p);
+    mix(ncb);
+    mix(ws->cap_sym);
+    mix(ws->cap_pass);
+    mix(lo.oF | (uint64_t)records << 62);
+    mix((uint64_t)gp.lds | (uint64_t)gp.maxcol << 32);
+    mix(gp.obuf_bytes);
+    mix((uint64_t)tuning().mq_lanes_full | (uint64_t)tuning().mq_lanes_limited << 16 |
+        (uint64_t)c->control.error_resilience << 32);
+    CHK(graph_run(ws, h, [&]() { return chain(true); }));
+  } else {
+    CHK(chain(true));
+  }
+  tm = trace_mark("list + offsets + launch", tm);
+  for (int attempt = 0;; ++attempt) {
     HIP_OK(hipStreamSynchronize(ws->stream));
     tm = trace_mark("records: wait for the GPU", tm);
     std::memcpy(info, ws->h_down + lo.oF + 16, 32);
@@ -2434,8 +2513,13 @@ static int t1_code_dev(Workspace *ws, const bwc_codec *c, const Param &par, cons
       std::snprintf(g_err, sizeof g_err, "tier-1 buffers too small after growing them");
       return 1;
     }
+    // the coding kernels did nothing: grow the buffers to the totals and run
+    // the chain from the list again (directly)
     CHK(v2_buffers(ws, ncb, info[1] + 64, ncb));
     ws->cap_pass = std::max<uint64_t>(ws->cap_pass, info[2]);
+    lo = pack_layout(ncb, ws->cap_pass);
+    CHK(pack_alloc(ws, lo, ncb));
+    CHK(chain(false));
   }
   std::memcpy(hfl, ws->h_down + lo.oF, 16);
   std::memcpy(rec.K.data(), ws->h_kz, ncb);
@@ -2518,7 +2602,6 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
   CHK(up_copy(ws, ws->d_cbmeta, meta.data(), ncb * sizeof(uint32_t)));
   CHK(up_copy(ws, ws->d_qoff, qoff.data(), (ncb + 1) * sizeof(uint64_t)));
   CHK(up_copy(ws, ws->d_doff, doff.data(), (ncb + 1) * sizeof(uint64_t)));
-  hipLaunchKernelGGL(init_flags_kernel, dim3(1), dim3(64), 0, ws->stream, ws->d_flags);
   const uint64_t W = par.box.len(0), H = par.box.len(1), D = par.box.len(2);
 
   // fast path: wavefront-per-codeblock symbol generation + few-lane MQ, when
@@ -2549,7 +2632,6 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
       if (!ws->d_tstat) HIP_OK(hipMalloc((void **)&ws->d_tstat, 4 * sizeof(float)));
       if (!ws->d_skipw) HIP_OK(hipMalloc((void **)&ws->d_skipw, kSkipModelSize * sizeof(float)));
       if (lim.model) CHK(up_copy(ws, ws->d_skipw, lim.model, kSkipModelSize * sizeof(float)));
-      hipLaunchKernelGGL(zero_f32_kernel, dim3(1), dim3(64), 0, ws->stream, ws->d_tstat, 4);
       d_feat = ws->d_feat;
     }
     const int nl = c->control.nLayers;
@@ -2563,29 +2645,53 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
       bs.cap_hist = cap;
       d_hist = bs.hist;
     }
-    tm0 = trace_mark("uploads + buffers", tm0);
-    hipLaunchKernelGGL(bitplane_kernel, dim3((uint32_t)std::min<uint64_t>(ncb, 65535)), dim3(256), 0, ws->stream,
-                       work, W, H, D, ws->d_cbbox, ws->d_cbsb, ws->d_bpoff, bs.bp, ws->d_K, ncb, ws->d_cbmeta,
-                       d_feat, ws->d_tstat, rate > 0 ? rate : 1.0f, d_hist);
-    HIP_OK(hipGetLastError());
     // pass limits on the device: the network's or the fixed ones
     const uint8_t *dz = nullptr;
     if (lim.model) {
-      char e[200];
-      if (skipnet_infer(ws->stream, ws->d_feat, ncb, ws->d_skipw, ws->d_tstat, ws->d_K, lim.margin, ws->d_zlim,
-                        nullptr, e, sizeof e)) {
-        std::snprintf(g_err, sizeof g_err, "%s", e);
-        return 1;
-      }
       dz = ws->d_zlim;
     } else if (lim.fixed) {
       CHK(up_copy(ws, ws->d_zlim, lim.fixed, ncb));
       dz = ws->d_zlim;
     }
+    tm0 = trace_mark("uploads + buffers", tm0);
+    // the device chain up to the codeblock list: the caller's launches (the
+    // DWT), the flag words, bitplanes and features, the network's limits
+    const float rate_arg = rate > 0 ? rate : 1.0f;
+    uint8_t *const bp = bs.bp;
+    const std::function<int()> pre = [&]() -> int {
+      if (lim.pre) CHK((*lim.pre)());
+      hipLaunchKernelGGL(init_flags_kernel, dim3(1), dim3(64), 0, ws->stream, ws->d_flags);
+      if (d_feat) hipLaunchKernelGGL(zero_f32_kernel, dim3(1), dim3(64), 0, ws->stream, ws->d_tstat, 4);
+      hipLaunchKernelGGL(bitplane_kernel, dim3((uint32_t)std::min<uint64_t>(ncb, 65535)), dim3(256), 0, ws->stream,
+                         work, W, H, D, ws->d_cbbox, ws->d_cbsb, ws->d_bpoff, bp, ws->d_K, ncb, ws->d_cbmeta,
+                         d_feat, ws->d_tstat, rate_arg, d_hist);
+      HIP_OK(hipGetLastError());
+      if (lim.model) {
+        char e[200];
+        if (skipnet_infer(ws->stream, ws->d_feat, ncb, ws->d_skipw, ws->d_tstat, ws->d_K, lim.margin, ws->d_zlim,
+                          nullptr, e, sizeof e)) {
+          std::snprintf(g_err, sizeof g_err, "%s", e);
+          return 1;
+        }
+      }
+      return 0;
+    };
+    // with a key from the caller (pipeline graphs), the key of this part
+    uint64_t key = 0;
+    if (lim.pre_key && !lim.features) {
+      key = lim.pre_key;
+      auto mix = [&key](uint64_t v) { key = (key ^ v) * 0x100000001B3ull + 0x9E3779B97F4A7C15ull; };
+      mix((uint64_t)(uintptr_t)work);
+      mix(W), mix(H), mix(D);
+      uint32_t rb;
+      std::memcpy(&rb, &rate_arg, 4);
+      mix(rb | (uint64_t)(lim.model != nullptr) << 32 | (uint64_t)(lim.fixed != nullptr) << 33 |
+          (uint64_t)(uint32_t)lim.margin << 34);
+    }
     // list, offsets, coding and record packing on the device, one wait
     std::vector<uint64_t> list;
     std::vector<uint32_t> m;
-    CHK(t1_code_dev(ws, c, par, dz, d_data, bs.bp, d_hist, gp, rec, zpred, list, m, hfl, lim.records));
+    CHK(t1_code_dev(ws, c, par, dz, d_data, bs.bp, d_hist, gp, rec, zpred, list, m, hfl, lim.records, &pre, key));
     if (lim.model) zlim = zpred.data();
     if (lim.out && zlim) std::memcpy(lim.out, zlim, ncb);
     if (lim.features) {
@@ -2598,6 +2704,8 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
         if (m[r]) rec.decisions[list[r]] = pc[list[r] * kMaxPasses + m[r] - 1];
     }
   } else {
+    if (lim.pre) CHK((*lim.pre)());
+    hipLaunchKernelGGL(init_flags_kernel, dim3(1), dim3(64), 0, ws->stream, ws->d_flags);
     cap = ws->cap_q;
     CHK(ensure((void **)&ws->d_q, &cap, qo * sizeof(uint64_t)));
     ws->cap_q = cap;

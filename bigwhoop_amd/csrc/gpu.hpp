@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <vector>
 
 #include "codec.hpp"
@@ -134,6 +135,15 @@ struct Workspace {
     int launches;
   };
   std::vector<DwtGraph> dwt_graphs;
+  // pipeline graphs: the forward DWT with tier-1 of one (tile, parameter),
+  // and the rate control of one tile, each keyed by everything its launch
+  // sequence depends on (exec null: seen once, recorded the next time)
+  struct Graph {
+    uint64_t key;
+    hipGraphExec_t exec;
+  };
+  std::vector<Graph> graphs;
+  uint64_t graph_records = 0, graph_replays = 0;   // pipeline graphs recorded / launched
   ~Workspace();
 };
 
@@ -171,7 +181,21 @@ struct PassLimits {
   // they stay on the device for rate control there and only K, Z and Kmsbs
   // come back
   bool records = true;
+  // launched first in the same device chain (the forward DWT), with the key
+  // of its launch sequence; with pipeline graphs on, the chain from it to
+  // tier-1's results is recorded once and replayed
+  const std::function<int()> *pre = nullptr;
+  uint64_t pre_key = 0;
 };
+// pipeline graphs (BWC_AMD_GRAPH, on unless set to 0)
+bool graphs_on();
+// the forward DWT's launches without a graph of its own, and their key
+int dwt_forward_direct(Workspace *ws, const bwc_codec *c, const Param &par, double *work, int norm_qm);
+uint64_t dwt_launch_key(const Workspace *ws, const bwc_codec *c, const Param &par, const double *work, int norm_qm);
+// run `launch` (launches and copies on ws->stream that neither allocate nor
+// wait) directly the first time `key` is seen, record it as a graph the
+// second time, replay that graph from then on
+int graph_run(Workspace *ws, uint64_t key, const std::function<int()> &launch);
 int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double *work, CblkRecords &rec,
               uint16_t *slope_min, uint16_t *slope_max, int slot, const PassLimits &lim = PassLimits());
 // features of the last t1_encode with a model (par.ncb x 16, feature 15

@@ -1221,77 +1221,100 @@ int rc_tile(Workspace *ws, bwc_codec *c, Job *job, uint64_t ti, SkipCheck *chk) 
   ProbeBufs pb{ptr<int16_t>(x.cur), ptr<uint32_t>(x.zl), ptr<uint32_t>(x.nb), ptr<uint32_t>(x.off),
                ptr<int64_t>(x.pbody), ptr<uint32_t>(x.pany)};
 
-  hipLaunchKernelGGL(rc_state_init_kernel, dim3(1), dim3(1), 0, s, st, (int64_t)hdr);
-  if (g.N) hipLaunchKernelGGL(rc_fill_u32_kernel, dim3(blocks(g.N, 256)), dim3(256), 0, s, dnodeval, g.N, 0xFFFFu);
-  if (G) {
-    hipLaunchKernelGGL(rc_init_kernel, dim3(blocks(G, 256)), dim3(256), 0, s, g.d_cb, G, dslots, dhoff, dhz, dhl,
-                       dhcnt, dhscan, dkms, dzg, dprev, dplen, dbeta, dbad, st);
-    hipLaunchKernelGGL(rc_msbs_cache_kernel, dim3(blocks(G, 256)), dim3(256), 0, s, g.d_cb, G, dkms, g.d_parent,
-                       dnodeval, stride, dmcw, dmcn);
-  }
-  HIP_RC(hipGetLastError());
-
-  // one evaluation of st->m (at most mcap) thresholds of the current layer
-  auto probe_round = [&](uint32_t mcap) -> int {
-    if (!G) return 0;
-    const uint64_t n = (uint64_t)mcap * G;
-    hipLaunchKernelGGL(rc_round_zero_kernel, dim3(blocks(mcap * capT / 32, 256 * 8)), dim3(256), 0, s, dbits,
-                       mcap * capT / 32 + 2, pb.body, pb.any, (uint64_t)mcap * NP);
-    hipLaunchKernelGGL(rc_cut_kernel, dim3(blocks(n, 256)), dim3(256), 0, s, st, mcap, g.d_cb, G, (uint32_t)NP,
-                       dslots, dhoff, dhz, dhl, dhcnt, dhscan, dzg, dprev, dplen, dbeta, dmcn, g.d_pk_first, pb);
-    const uint64_t nblk = (n + 1023) / 1024;
-    hipLaunchKernelGGL(rc_scan_local_kernel, dim3((uint32_t)nblk), dim3(256), 0, s, pb.nb, pb.off, n, dbsum);
-    hipLaunchKernelGGL(rc_scan_top_kernel, dim3(1), dim3(1024), 0, s, dbsum, nblk);
-    hipLaunchKernelGGL(rc_scan_add_kernel, dim3(blocks(n, 256)), dim3(256), 0, s, pb.off, n, dbsum);
-    hipLaunchKernelGGL(rc_write_kernel, dim3(blocks(n, 256)), dim3(256), 0, s, st, mcap, g.d_cb, G, dprev, dplen,
-                       dbeta, dmcn, dmcw, stride, g.d_pk_first, dpbase, capT, pb, dbits);
-    hipLaunchKernelGGL(rc_piece_kernel, dim3(npiece, mcap), dim3(256), 0, s, st, dbits, capT, dpbase, dpfirst,
-                       (uint32_t)NP, npiece, g.d_pk_first, G, pb, dpf_ex, dpf_cn);
-    hipLaunchKernelGGL(rc_packet_kernel, dim3((uint32_t)NP, mcap), dim3(64), 0, s, st, dpfirst, npiece,
-                       g.d_pk_first, G, (uint32_t)NP, pb, dpf_ex, dpf_cn, (int64_t *)nullptr);
-    HIP_RC(hipGetLastError());
-    return 0;
-  };
-
-  for (int l = 0; l < nl; ++l) {
-    const int64_t target = (int64_t)std::ceil(c->control.bitrate[l] * tsz / 8.0f);
-    hipLaunchKernelGGL(rc_layer_begin_kernel, dim3(1), dim3(1), 0, s, st, (uint32_t)l, target, depth);
-    for (int r = 0; r < rounds; ++r) {
-      if (probe_round(mmax)) return 1;
-      hipLaunchKernelGGL(rc_advance_kernel, dim3(1), dim3(1), 0, s, st, depth);
-    }
-    hipLaunchKernelGGL(rc_bisect_end_kernel, dim3(1), dim3(1), 0, s, st);
-    if (chk) {
-      hipLaunchKernelGGL(rc_setup_probes_kernel, dim3(1), dim3(1), 0, s, st, 1);
-      if (probe_round(2)) return 1;
-      hipLaunchKernelGGL(rc_check_kernel, dim3(blocks(G, 256)), dim3(256), 0, s, st, g.d_cb, G, dslots, dhoff, dhz,
-                         dhl, dhcnt, dhscan, dzg, chk->margin, dbad);
-    }
-    if (l == 0 && nl > 1 && G) {
-      // the layer's final estimate sees the MSB trees set (create_quality_layer
-      // sets them before create_packet, tier2.c:1040-1064)
-      hipLaunchKernelGGL(rc_msbs_set_kernel, dim3(blocks(G, 256)), dim3(256), 0, s, g.d_cb, G, dkms, g.d_parent,
-                         dnodeval);
+  // the launch sequence, up to the copies of the results into page-locked
+  // memory: recorded as a graph and replayed while everything it depends on
+  // stays the same (pipeline graphs on)
+  uint8_t *down = x.h_io + io_up;
+  auto launch_all = [&]() -> int {
+    hipLaunchKernelGGL(rc_state_init_kernel, dim3(1), dim3(1), 0, s, st, (int64_t)hdr);
+    if (g.N) hipLaunchKernelGGL(rc_fill_u32_kernel, dim3(blocks(g.N, 256)), dim3(256), 0, s, dnodeval, g.N, 0xFFFFu);
+    if (G) {
+      hipLaunchKernelGGL(rc_init_kernel, dim3(blocks(G, 256)), dim3(256), 0, s, g.d_cb, G, dslots, dhoff, dhz, dhl,
+                         dhcnt, dhscan, dkms, dzg, dprev, dplen, dbeta, dbad, st);
       hipLaunchKernelGGL(rc_msbs_cache_kernel, dim3(blocks(G, 256)), dim3(256), 0, s, g.d_cb, G, dkms, g.d_parent,
                          dnodeval, stride, dmcw, dmcn);
     }
-    const int est1 = l + 1 < nl;
-    if (est1) {
-      hipLaunchKernelGGL(rc_setup_probes_kernel, dim3(1), dim3(1), 0, s, st, 2);
-      if (probe_round(1)) return 1;
-    }
-    if (G)
-      hipLaunchKernelGGL(rc_commit_kernel, dim3(blocks(G, 256)), dim3(256), 0, s, st, g.d_cb, G, dslots, dhoff, dhz,
-                         dhl, dhcnt, dhscan, dzg, dprev, dplen, dbeta, nl, dcpo, dlcut);
-    hipLaunchKernelGGL(rc_layer_end_kernel, dim3(1), dim3(1), 0, s, st, est1);
     HIP_RC(hipGetLastError());
+
+    // one evaluation of st->m (at most mcap) thresholds of the current layer
+    auto probe_round = [&](uint32_t mcap) -> int {
+      if (!G) return 0;
+      const uint64_t n = (uint64_t)mcap * G;
+      hipLaunchKernelGGL(rc_round_zero_kernel, dim3(blocks(mcap * capT / 32, 256 * 8)), dim3(256), 0, s, dbits,
+                         mcap * capT / 32 + 2, pb.body, pb.any, (uint64_t)mcap * NP);
+      hipLaunchKernelGGL(rc_cut_kernel, dim3(blocks(n, 256)), dim3(256), 0, s, st, mcap, g.d_cb, G, (uint32_t)NP,
+                         dslots, dhoff, dhz, dhl, dhcnt, dhscan, dzg, dprev, dplen, dbeta, dmcn, g.d_pk_first, pb);
+      const uint64_t nblk = (n + 1023) / 1024;
+      hipLaunchKernelGGL(rc_scan_local_kernel, dim3((uint32_t)nblk), dim3(256), 0, s, pb.nb, pb.off, n, dbsum);
+      hipLaunchKernelGGL(rc_scan_top_kernel, dim3(1), dim3(1024), 0, s, dbsum, nblk);
+      hipLaunchKernelGGL(rc_scan_add_kernel, dim3(blocks(n, 256)), dim3(256), 0, s, pb.off, n, dbsum);
+      hipLaunchKernelGGL(rc_write_kernel, dim3(blocks(n, 256)), dim3(256), 0, s, st, mcap, g.d_cb, G, dprev, dplen,
+                         dbeta, dmcn, dmcw, stride, g.d_pk_first, dpbase, capT, pb, dbits);
+      hipLaunchKernelGGL(rc_piece_kernel, dim3(npiece, mcap), dim3(256), 0, s, st, dbits, capT, dpbase, dpfirst,
+                         (uint32_t)NP, npiece, g.d_pk_first, G, pb, dpf_ex, dpf_cn);
+      hipLaunchKernelGGL(rc_packet_kernel, dim3((uint32_t)NP, mcap), dim3(64), 0, s, st, dpfirst, npiece,
+                         g.d_pk_first, G, (uint32_t)NP, pb, dpf_ex, dpf_cn, (int64_t *)nullptr);
+      HIP_RC(hipGetLastError());
+      return 0;
+    };
+
+    for (int l = 0; l < nl; ++l) {
+      const int64_t target = (int64_t)std::ceil(c->control.bitrate[l] * tsz / 8.0f);
+      hipLaunchKernelGGL(rc_layer_begin_kernel, dim3(1), dim3(1), 0, s, st, (uint32_t)l, target, depth);
+      for (int r = 0; r < rounds; ++r) {
+        if (probe_round(mmax)) return 1;
+        hipLaunchKernelGGL(rc_advance_kernel, dim3(1), dim3(1), 0, s, st, depth);
+      }
+      hipLaunchKernelGGL(rc_bisect_end_kernel, dim3(1), dim3(1), 0, s, st);
+      if (chk) {
+        hipLaunchKernelGGL(rc_setup_probes_kernel, dim3(1), dim3(1), 0, s, st, 1);
+        if (probe_round(2)) return 1;
+        hipLaunchKernelGGL(rc_check_kernel, dim3(blocks(G, 256)), dim3(256), 0, s, st, g.d_cb, G, dslots, dhoff, dhz,
+                           dhl, dhcnt, dhscan, dzg, chk->margin, dbad);
+      }
+      if (l == 0 && nl > 1 && G) {
+        // the layer's final estimate sees the MSB trees set (create_quality_layer
+        // sets them before create_packet, tier2.c:1040-1064)
+        hipLaunchKernelGGL(rc_msbs_set_kernel, dim3(blocks(G, 256)), dim3(256), 0, s, g.d_cb, G, dkms, g.d_parent,
+                           dnodeval);
+        hipLaunchKernelGGL(rc_msbs_cache_kernel, dim3(blocks(G, 256)), dim3(256), 0, s, g.d_cb, G, dkms, g.d_parent,
+                           dnodeval, stride, dmcw, dmcn);
+      }
+      const int est1 = l + 1 < nl;
+      if (est1) {
+        hipLaunchKernelGGL(rc_setup_probes_kernel, dim3(1), dim3(1), 0, s, st, 2);
+        if (probe_round(1)) return 1;
+      }
+      if (G)
+        hipLaunchKernelGGL(rc_commit_kernel, dim3(blocks(G, 256)), dim3(256), 0, s, st, g.d_cb, G, dslots, dhoff, dhz,
+                           dhl, dhcnt, dhscan, dzg, dprev, dplen, dbeta, nl, dcpo, dlcut);
+      hipLaunchKernelGGL(rc_layer_end_kernel, dim3(1), dim3(1), 0, s, st, est1);
+      HIP_RC(hipGetLastError());
+    }
+    // results: cuts and L at the cuts per layer, the check's flags, the state
+    HIP_RC(hipMemcpyAsync(down, dcpo, G * nl * 2, hipMemcpyDeviceToHost, s));
+    HIP_RC(hipMemcpyAsync(down + G * nl * 2, dlcut, G * nl * 4, hipMemcpyDeviceToHost, s));
+    if (chk) HIP_RC(hipMemcpyAsync(down + G * nl * 6, dbad, G, hipMemcpyDeviceToHost, s));
+    HIP_RC(hipMemcpyAsync(x.h_st, st, sizeof(RcState), hipMemcpyDeviceToHost, s));
+    return 0;
+  };
+  if (graphs_on()) {
+    uint64_t key = 0xC6A4A7935BD1E995ull;
+    auto mix = [&key](uint64_t v) { key = (key ^ v) * 0x100000001B3ull + 0x9E3779B97F4A7C15ull; };
+    RcCtx::Buf *bs[] = {&x.slots, &x.hoff,  &x.hz,   &x.hl,          &x.hcnt, &x.hscan, &x.kms, &x.zg,
+                        &x.prev,  &x.plen,  &x.beta, &x.bad,         &x.nodeval, &x.mcw, &x.mcn, &x.cpo,
+                        &x.lcut,  &x.pbase, &x.piece_first, &x.cur,  &x.zl,   &x.nb,    &x.off, &x.pbody,
+                        &x.pany,  &x.bsum,  &x.bits, &x.pf_ex,       &x.pf_cn};
+    for (RcCtx::Buf *b : bs) mix((uint64_t)(uintptr_t)b->p);
+    const void *more[] = {x.d_st, x.h_st, down, g.d_cb, g.d_pk_first, g.d_parent};
+    for (const void *p : more) mix((uint64_t)(uintptr_t)p);
+    mix(G), mix(NP), mix(g.N), mix(capT), mix(npiece), mix(stride), mix((uint64_t)depth << 8 | (uint64_t)rounds);
+    mix((uint64_t)nl | (uint64_t)(chk ? chk->margin + 1 : 0) << 16), mix(hdr);
+    for (int l = 0; l < nl; ++l) mix((uint64_t)(int64_t)std::ceil(c->control.bitrate[l] * tsz / 8.0f));
+    if (graph_run(ws, key, launch_all)) return 1;
+  } else if (launch_all()) {
+    return 1;
   }
-  // results: cuts and L at the cuts per layer, the check's flags, the state
-  uint8_t *down = x.h_io + io_up;
-  HIP_RC(hipMemcpyAsync(down, dcpo, G * nl * 2, hipMemcpyDeviceToHost, s));
-  HIP_RC(hipMemcpyAsync(down + G * nl * 2, dlcut, G * nl * 4, hipMemcpyDeviceToHost, s));
-  if (chk) HIP_RC(hipMemcpyAsync(down + G * nl * 6, dbad, G, hipMemcpyDeviceToHost, s));
-  HIP_RC(hipMemcpyAsync(x.h_st, st, sizeof(RcState), hipMemcpyDeviceToHost, s));
   HIP_RC(hipStreamSynchronize(s));
   if (x.h_st->err) {
     std::snprintf(rc_err, sizeof rc_err, "device rate control: bisection state error %u", x.h_st->err);

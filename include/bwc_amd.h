@@ -161,6 +161,12 @@ int bwc_amd_cblk_cuts(bwc_codec *codec, uint64_t tile, uint8_t param, int16_t *c
    header bytes); at most cap entries written, the count returned (-1 for a
    codec that has not compressed). */
 int bwc_amd_rc_trace(bwc_codec *codec, uint32_t *layer, uint32_t *slope, int64_t *est, int cap);
+/* Pipeline graphs of a codec (BWC_AMD_GRAPH unset or 1; 0 turns them off):
+   the forward DWT with tier-1 of each (tile, parameter), and the rate
+   control of each tile, are recorded as HIP graphs the second time their
+   launch sequence repeats and replayed after that. counts[0] = graphs
+   recorded, counts[1] = graph launches, since the codec was created. */
+int bwc_amd_graph_counts(bwc_codec *codec, uint64_t counts[2]);
 /* Test entry point of the device rate control's header-size count: the
    bytes the packet-header writer emits for each of n bit strings (string i:
    nbits[i] bits, MSB first, from 32-bit big-endian word woff[i] of words;
