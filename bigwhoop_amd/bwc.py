@@ -112,6 +112,7 @@ def lib():
             "bwc_amd_cblk_cuts": (C.c_int, [vp, u64, u8, C.POINTER(C.c_int16)]),
             "bwc_amd_rc_trace": (C.c_int, [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(C.c_int64), C.c_int]),
             "bwc_amd_graph_counts": (C.c_int, [vp, C.POINTER(u64)]),
+            "bwc_amd_batch_queued": (u64, [vp]),
             "bwc_amd_stuffed_bytes": (C.c_int, [C.POINTER(u32), C.POINTER(u64), C.POINTER(u64), C.c_int,
                                                 C.POINTER(u64)]),
             "bwc_amd_collect_features": (C.c_int, [vp, C.c_int]),
@@ -383,6 +384,10 @@ class Codec:
         if lib().bwc_amd_cblk_cuts(self.h, tile, param, _p(out, C.c_int16)):
             raise RuntimeError("bwc_amd_cblk_cuts failed")
         return out
+
+    def batch_queued(self):
+        """Tier-1 calls of this codec coded through a batch's global codeblock queue."""
+        return int(lib().bwc_amd_batch_queued(self.h))
 
     def graph_counts(self):
         """(graphs recorded, graph launches) of this codec's pipeline graphs."""

@@ -621,6 +621,12 @@ size_t bwc_compress(bwc_codec *const codec, bwc_stream *const stream) {
   bwc_gl_ctrl &ctl = codec->control;
   const bool rc_dev = rc_on_device();
   const double t_start = now_ms();
+  // a call of a batch's global codeblock queue that ends before its tier-1
+  // joined the queue still counts as joined (with no codeblocks)
+  struct LeaveBatch {
+    gpu::Workspace *ws;
+    ~LeaveBatch() { gpu::batch_leave(ws); }
+  } leave_batch{ws};
   reset_for_encode(codec, job);
   std::fill(std::begin(job->skip_stats), std::end(job->skip_stats), 0);
   const bool on_dev = gpu::is_device_pointer(stream->inp) != 0;
@@ -649,7 +655,7 @@ size_t bwc_compress(bwc_codec *const codec, bwc_stream *const stream) {
       // With pipeline graphs the DWT is launched as the head of tier-1's
       // device chain (recorded with it; its stage time is then counted in
       // tier-1's), otherwise on its own.
-      const bool chain = gpu::graphs_on() && !job->collect_features;
+      const bool chain = gpu::graphs_on() && !job->collect_features && !ws->batch;
       const std::function<int()> dwt_launch_fn = [&]() {
         return gpu::dwt_forward_direct(ws, codec, par, ws->d_work, ctl.Qm);
       };
@@ -909,6 +915,38 @@ int bwc_amd_compress_batch(bwc_codec *const *codecs, bwc_stream *const *streams,
   workers = std::min(workers, n);
   int device = 0;
   (void)hipGetDevice(&device);
+  // Fields of one tile and one variable code their tier-1 through one global
+  // codeblock queue (BWC_AMD_BATCH=threads: each field on its own): every
+  // such field's compression runs on a thread of its own up to its codeblock
+  // list, the last to get there launches symbol generation and MQ coding
+  // once over all of them, and each goes on with its rate control.
+  const char *bm = std::getenv("BWC_AMD_BATCH");
+  const bool queue = !(bm && std::strcmp(bm, "threads") == 0);
+  std::vector<int> in_queue;
+  uint64_t items = 0;
+  for (int i = 0; queue && i < n; ++i) {
+    Job *job = job_of(codecs[i]);
+    if (job->plan.tiles.size() == 1 && codecs[i]->info.nPar == 1) {
+      in_queue.push_back(i);
+      items += job->plan.tiles[0].par[0].ncb;
+    }
+  }
+  gpu::BatchQueue *bq = nullptr;
+  if (in_queue.size() > 1 && in_queue.size() <= 512) {
+    bq = gpu::batch_create((int)in_queue.size(), items);
+    if (!bq) {
+      set_err(gpu::last_error());
+      return 1;
+    }
+    for (int i : in_queue) {
+      gpu::Workspace *ws = gpu::workspace_of(job_of(codecs[i]));
+      ws->batch = bq;
+      ws->batch_joined = false;
+    }
+    // every queued field needs a thread of its own until the launch
+    workers = std::max(workers, std::min(n, (int)in_queue.size() + (workers - 1)));
+    workers = std::max(workers, (int)in_queue.size());
+  }
   std::atomic<int> next{0};
   std::atomic<bool> failed{false};
   std::mutex em;
@@ -925,9 +963,39 @@ int bwc_amd_compress_batch(bwc_codec *const *codecs, bwc_stream *const *streams,
     }
   };
   std::vector<std::thread> th;
-  for (int w = 1; w < workers; ++w) th.emplace_back(work);
-  work();
-  for (auto &t : th) t.join();
+  if (bq) {
+    // queued fields first, one per thread, then the others on the same threads
+    std::vector<int> order = in_queue;
+    for (int i = 0; i < n; ++i)
+      if (std::find(in_queue.begin(), in_queue.end(), i) == in_queue.end()) order.push_back(i);
+    auto qwork = [&](int first) {
+      (void)hipSetDevice(device);
+      for (size_t k = (size_t)first; k < order.size(); k += (size_t)workers) {
+        const int i = order[k];
+        gpu::Workspace *ws = gpu::workspace_of(job_of(codecs[i]));
+        if (failed.load() && !(ws->batch && !ws->batch_joined)) continue;
+        sizes[i] = bwc_compress(codecs[i], streams[i]);
+        if (!sizes[i]) {
+          failed.store(true);
+          std::lock_guard<std::mutex> lk(em);
+          if (first_err.empty()) first_err = bwc_amd_last_error();
+        }
+      }
+    };
+    for (int w = 1; w < workers; ++w) th.emplace_back(qwork, w);
+    qwork(0);
+    for (auto &t : th) t.join();
+    for (int i : in_queue) {
+      gpu::Workspace *ws = gpu::workspace_of(job_of(codecs[i]));
+      ws->batch = nullptr;
+      ws->batch_joined = false;
+    }
+    if (gpu::batch_destroy(bq) && first_err.empty()) first_err = "batch queue failed";
+  } else {
+    for (int w = 1; w < workers; ++w) th.emplace_back(work);
+    work();
+    for (auto &t : th) t.join();
+  }
   if (!first_err.empty()) {
     std::snprintf(g_api_err, sizeof g_api_err, "%s", first_err.c_str());
     return 1;
@@ -1222,6 +1290,12 @@ int bwc_amd_stuffed_bytes(const uint32_t *words, const uint64_t *woff, const uin
     return 1;
   }
   return 0;
+}
+
+uint64_t bwc_amd_batch_queued(bwc_codec *codec) {
+  Job *job = job_of(codec);
+  if (!job || !job->ws) return 0;
+  return static_cast<const gpu::Workspace *>(job->ws)->batch_queued;
 }
 
 int bwc_amd_graph_counts(bwc_codec *codec, uint64_t counts[2]) {

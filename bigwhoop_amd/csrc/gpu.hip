@@ -18,6 +18,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <condition_variable>
 #include <mutex>
 
 #include "gpu.hpp"
@@ -592,7 +593,7 @@ __device__ __forceinline__ int mirror32(int p, int len) {
 // normalised as it is read, as normalize_kernel would have done in place.
 __global__ void __launch_bounds__(256) dwt97_x_kernel(const double *src, double *dst, int len, int par0, int half,
                                                       uint32_t n0, uint64_t s0, uint32_t n1, uint64_t s1,
-                                                      uint64_t s2, const double *norm, uint64_t shift) {
+                                                      uint64_t s2, const double *norm, uint64_t shift, int nt) {
   extern __shared__ double lds[];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int hp = (len + 16) / 2 + 1;
@@ -610,7 +611,8 @@ __global__ void __launch_bounds__(256) dwt97_x_kernel(const double *src, double 
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int e = e0 + 64 * u + lane, p = e - 8 + par0;
-        v[u] = (e < len + 16) ? s[((unsigned)p < (unsigned)len) ? p : mirror32(p, len)] : 0.0;
+        const double *q = &s[((unsigned)p < (unsigned)len) ? p : mirror32(p, len)];
+        v[u] = (e < len + 16) ? ((nt & 2) ? __builtin_nontemporal_load(q) : *q) : 0.0;
       }
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
@@ -643,7 +645,12 @@ __global__ void __launch_bounds__(256) dwt97_x_kernel(const double *src, double 
   __syncthreads();
   if (act) {
     double *d = dst + base;
-    for (int o = lane; o < len; o += 64) d[o] = (o < half) ? Ev[o + 4] : Od[o - half + 4 - par0];
+    if (nt & 1) {
+      for (int o = lane; o < len; o += 64)
+        __builtin_nontemporal_store((o < half) ? Ev[o + 4] : Od[o - half + 4 - par0], &d[o]);
+    } else {
+      for (int o = lane; o < len; o += 64) d[o] = (o < half) ? Ev[o + 4] : Od[o - half + 4 - par0];
+    }
   }
 }
 
@@ -657,7 +664,19 @@ __global__ void __launch_bounds__(256) dwt97_x_kernel(const double *src, double 
 // as zeros and, as each step reaches back at most two iterations, are exact
 // again after three, so only exact values are stored. Highpass outputs land
 // ahead of the read position, so the pass is out of place (src != dst).
-template <int U>
+// loads and stores of the streaming DWT passes, with the nontemporal hint when asked
+template <bool NT>
+__device__ __forceinline__ double st_ld(const double *p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <bool NT>
+__device__ __forceinline__ void st_st(double *p, double v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
+template <int U, int NT>
 __global__ void __launch_bounds__(256) dwt97_stream_kernel(const double *__restrict__ src, double *__restrict__ dst,
                                                            int len, int par0, int half, uint64_t sa, uint32_t n0,
                                                            uint64_t s0, uint32_t n1, uint64_t s1, uint64_t s2,
@@ -678,7 +697,7 @@ __global__ void __launch_bounds__(256) dwt97_stream_kernel(const double *__restr
   if (kA >= kB) return;
   auto ld = [&](int i) -> double {
     const int p = i + par0;
-    return s[(int64_t)(((unsigned)p < (unsigned)len) ? p : mirror32(p, len)) * (int64_t)sa];
+    return st_ld<(NT & 2) != 0>(&s[(int64_t)(((unsigned)p < (unsigned)len) ? p : mirror32(p, len)) * (int64_t)sa]);
   };
   // carried: x_{2k-2}, x_{2k-1}, a_{2k-3}, b_{2k-4}, c_{2k-5}
   const int k0 = kA - 4;
@@ -700,9 +719,10 @@ __global__ void __launch_bounds__(256) dwt97_stream_kernel(const double *__restr
       const double c3 = (i >= -1 && i <= last2) ? Lift97::KH * (am3 + Lift97::G * (bm4 + b2)) : am3;
       i = 2 * kk - 4;
       const double d4 = (i >= 0 && i <= last3) ? Lift97::KL * (bm4 + Lift97::D * (cm5 + c3)) : bm4;
-      if (kk >= kA && i >= ilo && i < ihi) d[(int64_t)(i >> 1) * (int64_t)sa] = d4;
+      if (kk >= kA && i >= ilo && i < ihi) st_st<(NT & 1) != 0>(&d[(int64_t)(i >> 1) * (int64_t)sa], d4);
       i = 2 * kk - 3;
-      if (kk >= kA && i >= ilo && i < ihi) d[(int64_t)(half + ((i + 2 * par0 - 1) >> 1)) * (int64_t)sa] = c3;
+      if (kk >= kA && i >= ilo && i < ihi)
+        st_st<(NT & 1) != 0>(&d[(int64_t)(half + ((i + 2 * par0 - 1) >> 1)) * (int64_t)sa], c3);
       xm2 = x0;
       xm1 = x1;
       am3 = a1;
@@ -756,10 +776,11 @@ static int dwt_passes(DwtCtx &x, const bwc_codec *c, const Param &par, double *w
   const char *dwt_env = std::getenv("BWC_AMD_DWT");
   const bool generic_only = dwt_env && dwt_env[0] == '0';
   // launch shape of the streaming passes: pair iterations unrolled per load
-  // batch, thread target 2^seg_log2 and minimum iterations per segment
+  // batch, thread target 2^seg_log2 and minimum iterations per segment,
+  // nontemporal hints
   struct {
-    int unroll, seg_log2, seg_min;
-  } tune = {tuning().dwt_unroll, tuning().dwt_seg_log2, tuning().dwt_seg_min};
+    int unroll, seg_log2, seg_min, nt;
+  } tune = {tuning().dwt_unroll, tuning().dwt_seg_log2, tuning().dwt_seg_min, tuning().dwt_nt};
   for (int step = 0; step < nd; ++step) {
     const int level = fwd ? step : nd - 1 - step;
     const Box &rb = par.res[nd - level].box;
@@ -831,7 +852,7 @@ static int dwt_passes(DwtCtx &x, const bwc_codec *c, const Param &par, double *w
         hipLaunchKernelGGL(dwt97_x_kernel, dim3((uint32_t)((sz[o[0]] + 3) / 4), (uint32_t)gy), dim3(256), xlds,
                            x.stream, cur, dst, (int)len, par0, half, (uint32_t)sz[o[0]], inc[o[0]],
                            (uint32_t)sz[o[1]], inc[o[1]], inc[o[2]], norm_pending ? x.norm : nullptr,
-                           (uint64_t)(norm_pending ? norm_qm : 0) << 52);
+                           (uint64_t)(norm_pending ? norm_qm : 0) << 52, tune.nt);
         HIP_OK(hipGetLastError());
         norm_pending = false;
         if (x_oop) std::swap(cur, oth);
@@ -848,13 +869,21 @@ static int dwt_passes(DwtCtx &x, const bwc_codec *c, const Param &par, double *w
         const int ksz = (kout + nseg - 1) / nseg;
         const uint64_t nthreads = nlines * (uint64_t)nseg;
         const uint32_t nblk = (uint32_t)((nthreads + 255) / 256);
-#define STREAM_LAUNCH(UU)                                                                                      \
-  hipLaunchKernelGGL(dwt97_stream_kernel<UU>, dim3(nblk), dim3(256), 0, x.stream, cur, oth, (int)len, par0, \
-                     half, inc[a], (uint32_t)sz[o[0]], inc[o[0]], (uint32_t)sz[o[1]], inc[o[1]], inc[o[2]],    \
+#define STREAM_LAUNCH(UU, NT)                                                                                      \
+  hipLaunchKernelGGL((dwt97_stream_kernel<UU, NT>), dim3(nblk), dim3(256), 0, x.stream, cur, oth, (int)len, par0, \
+                     half, inc[a], (uint32_t)sz[o[0]], inc[o[0]], (uint32_t)sz[o[1]], inc[o[1]], inc[o[2]],        \
                      (uint32_t)nlines, nseg, ksz)
-        if (tune.unroll == 2) STREAM_LAUNCH(2);
-        else if (tune.unroll == 8) STREAM_LAUNCH(8);
-        else STREAM_LAUNCH(4);
+#define STREAM_LAUNCH_U(NT)                  \
+  if (tune.unroll == 2) STREAM_LAUNCH(2, NT); \
+  else if (tune.unroll == 8) STREAM_LAUNCH(8, NT); \
+  else STREAM_LAUNCH(4, NT);
+        switch (tune.nt) {
+          case 1: STREAM_LAUNCH_U(1); break;
+          case 2: STREAM_LAUNCH_U(2); break;
+          case 3: STREAM_LAUNCH_U(3); break;
+          default: STREAM_LAUNCH_U(0); break;
+        }
+#undef STREAM_LAUNCH_U
 #undef STREAM_LAUNCH
         HIP_OK(hipGetLastError());
         std::swap(cur, oth);
@@ -925,7 +954,8 @@ static uint64_t dwt_key(const Workspace *ws, const bwc_codec *c, const Param &pa
   mix((uint64_t)(uintptr_t)ws->d_work2);
   mix((uint64_t)(uintptr_t)ws->d_norm);
   mix((uint64_t)(int64_t)norm_qm);
-  mix((uint64_t)tuning().dwt_unroll | (uint64_t)tuning().dwt_seg_log2 << 16 | (uint64_t)tuning().dwt_seg_min << 32);
+  mix((uint64_t)tuning().dwt_unroll | (uint64_t)tuning().dwt_seg_log2 << 16 | (uint64_t)tuning().dwt_seg_min << 32 |
+      (uint64_t)tuning().dwt_nt << 56);
   const char *d = std::getenv("BWC_AMD_DWT");
   mix(d && d[0] == '0');
   return h;
@@ -2367,6 +2397,8 @@ static int unpack_rows(Workspace *ws, CblkRecords &rec, const std::vector<uint64
   return 0;
 }
 
+static int batch_join(Workspace *ws, const T1Field *f, size_t lds, bool limited);
+
 // page-locked room for K, the pass limits, Z and Kmsbs of ncb codeblocks and
 // 16 bytes of flag words
 static int kz_room(Workspace *ws, uint64_t ncb) {
@@ -2460,13 +2492,23 @@ static int t1_code_dev(Workspace *ws, const bwc_codec *c, const Param &par, cons
   // the device chain: what comes before it (DWT, bitplanes, pass limits),
   // the list, generation, MQ coding, [record packing], and the copies of the
   // totals, flags, K, the pass limits, Z and Kmsbs to the host
-  auto chain = [&](bool with_pre) -> int {
+  auto chain = [&](bool with_pre, bool queued) -> int {
     if (with_pre && pre) CHK((*pre)());
     hipLaunchKernelGGL(t1_list_kernel, dim3(1), dim3(1024), 0, ws->stream, ncb, ws->d_K, dz, ws->d_cbbox,
                        ws->d_qoff, (uint64_t)ws->cap_sym, ws->cap_pass, ws->d_list, ws->d_symoff, ws->d_poff,
                        ws->d_t1info);
     HIP_OK(hipGetLastError());
-    CHK(v2_launch(ws, c, ncb, dz, d_data, d_bp, d_hist, gp, ws->d_t1info));
+    if (queued) {
+      // generation and MQ coding through the batch's global codeblock queue
+      T1Field f{ws->d_cbbox, ws->d_cbmeta, ws->d_cbsb,  ws->d_K,     ws->d_bpoff, d_bp,      ws->d_symoff,
+                ws->d_sym,   ws->d_pcnt,   ws->d_pmse,  ws->d_flags, dz,          ws->d_doff, d_data,
+                ws->d_Kmsbs, ws->d_Z,      ws->d_U,     ws->d_L,     ws->d_S,     ws->d_snap, ws->d_mse,
+                ws->d_bfut,  d_hist,       ws->d_t1info, ws->d_list, (int)c->control.error_resilience,
+                gp.maxcol,   gp.obuf_bytes};
+      CHK(batch_join(ws, &f, gp.lds, dz != nullptr));
+    } else {
+      CHK(v2_launch(ws, c, ncb, dz, d_data, d_bp, d_hist, gp, ws->d_t1info));
+    }
     if (records) {
       pack_launch(ws, lo, ncb, ws->d_t1info);
       HIP_OK(hipGetLastError());
@@ -2499,9 +2541,9 @@ static int t1_code_dev(Workspace *ws, const bwc_codec *c, const Param &par, cons
     mix(gp.obuf_bytes);
     mix((uint64_t)tuning().mq_lanes_full | (uint64_t)tuning().mq_lanes_limited << 16 |
         (uint64_t)c->control.error_resilience << 32);
-    CHK(graph_run(ws, h, [&]() { return chain(true); }));
+    CHK(graph_run(ws, h, [&]() { return chain(true, false); }));
   } else {
-    CHK(chain(true));
+    CHK(chain(true, ws->batch && !ws->batch_joined));
   }
   tm = trace_mark("list + offsets + launch", tm);
   for (int attempt = 0;; ++attempt) {
@@ -2519,7 +2561,7 @@ static int t1_code_dev(Workspace *ws, const bwc_codec *c, const Param &par, cons
     ws->cap_pass = std::max<uint64_t>(ws->cap_pass, info[2]);
     lo = pack_layout(ncb, ws->cap_pass);
     CHK(pack_alloc(ws, lo, ncb));
-    CHK(chain(false));
+    CHK(chain(false, false));
   }
   std::memcpy(hfl, ws->h_down + lo.oF, 16);
   std::memcpy(rec.K.data(), ws->h_kz, ncb);
@@ -2704,6 +2746,7 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
         if (m[r]) rec.decisions[list[r]] = pc[list[r] * kMaxPasses + m[r] - 1];
     }
   } else {
+    if (ws->batch && !ws->batch_joined) batch_leave(ws);
     if (lim.pre) CHK((*lim.pre)());
     hipLaunchKernelGGL(init_flags_kernel, dim3(1), dim3(64), 0, ws->stream, ws->d_flags);
     cap = ws->cap_q;
@@ -2816,6 +2859,180 @@ int gather(Workspace *ws, CblkRecords &rec, const std::vector<uint32_t> &len) {
   HIP_OK(hipMemcpyAsync(rec.data.data(), ws->d_packed, tot, hipMemcpyDeviceToHost, ws->stream));
   HIP_OK(hipStreamSynchronize(ws->stream));
   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// the global codeblock queue of a batch
+// ---------------------------------------------------------------------------
+struct BatchQueue {
+  std::mutex m;
+  std::condition_variable cv;
+  int expected = 0, joined = 0, err = 0;
+  bool launched = false;
+  std::vector<T1Field> fields;
+  std::vector<hipEvent_t> ready;   // each field's list built
+  size_t lds = 0;
+  bool limited = false;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  uint64_t max_items = 0;
+  T1Field *d_fields = nullptr;
+  uint64_t *d_items = nullptr;
+  uint32_t *d_misc = nullptr;   // [0] items, [1..2] queue counters, [4..67] bucket counts, [68..131] offsets
+};
+
+// work buckets: bucket of a codeblock = floor(log2) of its symbol-stream
+// bound (largest first in the queue)
+__global__ void batch_count_kernel(const T1Field *__restrict__ F, uint32_t *misc) {
+  const T1Field &f = F[blockIdx.y];
+  if (f.info[3]) return;
+  const uint64_t n = f.info[0];
+  for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < n; r += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t cb = f.list[r];
+    const uint64_t sz = f.symoff[cb + 1] - f.symoff[cb];
+    atomicAdd(&misc[4 + (63 - __clzll(sz | 1))], 1u);
+  }
+}
+__global__ void batch_offsets_kernel(uint32_t *misc) {
+  uint32_t o = 0;
+  for (int b = 63; b >= 0; --b) {
+    misc[68 + b] = o;
+    o += misc[4 + b];
+  }
+  misc[0] = o;
+  misc[1] = 0;
+  misc[2] = 0;
+}
+__global__ void batch_fill_kernel(const T1Field *__restrict__ F, uint32_t *misc, uint64_t *items) {
+  const T1Field &f = F[blockIdx.y];
+  if (f.info[3]) return;
+  const uint64_t n = f.info[0];
+  for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < n; r += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t cb = f.list[r];
+    const uint64_t sz = f.symoff[cb + 1] - f.symoff[cb];
+    const uint32_t k = atomicAdd(&misc[68 + (63 - __clzll(sz | 1))], 1u);
+    items[k] = (uint64_t)blockIdx.y << 32 | cb;
+  }
+}
+__global__ void batch_zero_kernel(uint32_t *misc, int n) {
+  for (int k = threadIdx.x; k < n; k += blockDim.x) misc[k] = 0;
+}
+
+BatchQueue *batch_create(int fields, uint64_t max_items) {
+  BatchQueue *q = new BatchQueue();
+  q->expected = fields;
+  q->max_items = max_items;
+  (void)hipGetDevice(&q->device);
+  if (hipStreamCreateWithFlags(&q->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&q->done, hipEventDisableTiming) != hipSuccess ||
+      hipMalloc((void **)&q->d_fields, std::max(1, fields) * sizeof(T1Field)) != hipSuccess ||
+      hipMalloc((void **)&q->d_items, std::max<uint64_t>(1, max_items) * sizeof(uint64_t)) != hipSuccess ||
+      hipMalloc((void **)&q->d_misc, 256 * sizeof(uint32_t)) != hipSuccess) {
+    std::snprintf(g_err, sizeof g_err, "batch queue: %s", hipGetErrorString(hipGetLastError()));
+    (void)batch_destroy(q);
+    return nullptr;
+  }
+  return q;
+}
+
+// the last field to join: the queue and the two launches on the batch stream
+static int batch_launch(BatchQueue *q) {
+  const int nf = (int)q->fields.size();
+  HIP_OK(hipSetDevice(q->device));
+  for (hipEvent_t e : q->ready) HIP_OK(hipStreamWaitEvent(q->stream, e, 0));
+  if (!nf) return 0;
+  HIP_OK(hipMemcpyAsync(q->d_fields, q->fields.data(), nf * sizeof(T1Field), hipMemcpyHostToDevice, q->stream));
+  hipLaunchKernelGGL(batch_zero_kernel, dim3(1), dim3(256), 0, q->stream, q->d_misc, 256);
+  hipLaunchKernelGGL(batch_count_kernel, dim3(64, nf), dim3(256), 0, q->stream, q->d_fields, q->d_misc);
+  hipLaunchKernelGGL(batch_offsets_kernel, dim3(1), dim3(1), 0, q->stream, q->d_misc);
+  hipLaunchKernelGGL(batch_fill_kernel, dim3(64, nf), dim3(256), 0, q->stream, q->d_fields, q->d_misc, q->d_items);
+  HIP_OK(hipGetLastError());
+  // persistent grids: symbol generation as many workgroups as fit (the
+  // largest per-codeblock LDS of the fields), MQ coding four wavefronts per
+  // SIMD with the lanes per wavefront of the codings' kind
+  const uint32_t per_cu = (uint32_t)std::max<size_t>(1, std::min<size_t>(16, (160 * 1024) / std::max<size_t>(1, q->lds)));
+  const uint32_t ggen = (uint32_t)std::min<uint64_t>(q->max_items, 256ull * per_cu);
+  hipLaunchKernelGGL(t1_gen_queue_kernel, dim3(std::max(1u, ggen)), dim3(64), q->lds, q->stream, q->d_fields,
+                     q->d_items, q->d_misc, q->d_misc + 1);
+  HIP_OK(hipGetLastError());
+  int lpw = q->limited ? tuning().mq_lanes_limited : tuning().mq_lanes_full;
+#define MQQ(N)                                                                                                   \
+  hipLaunchKernelGGL(t1_mq_queue_kernel<N>,                                                                       \
+                     dim3((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((q->max_items + N - 1) / N, 4096))), \
+                     dim3(64), 0, q->stream, q->d_fields, q->d_items, q->d_misc, q->d_misc + 1)
+  switch (lpw) {
+    case 1: MQQ(1); break;
+    case 2: MQQ(2); break;
+    case 4: MQQ(4); break;
+    case 8: MQQ(8); break;
+    case 32: MQQ(32); break;
+    default: MQQ(16); break;
+  }
+#undef MQQ
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+static int batch_arrive(Workspace *ws, const T1Field *f, size_t lds, bool limited) {
+  BatchQueue *q = ws->batch;
+  ws->batch_joined = true;
+  hipEvent_t e = nullptr;
+  if (f) {
+    HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIP_OK(hipEventRecord(e, ws->stream));
+  }
+  int err = 0;
+  {
+    std::unique_lock<std::mutex> lk(q->m);
+    if (f) {
+      q->fields.push_back(*f);
+      q->ready.push_back(e);
+      q->lds = std::max(q->lds, lds);
+      q->limited |= limited;
+    }
+    if (++q->joined == q->expected) {
+      if (batch_launch(q)) q->err = 1;
+      if (hipEventRecord(q->done, q->stream) != hipSuccess) q->err = 1;
+      q->launched = true;
+      q->cv.notify_all();
+    } else {
+      q->cv.wait(lk, [q] { return q->launched; });
+    }
+    err = q->err;
+  }
+  if (err) {
+    std::snprintf(g_err, sizeof g_err, "batch queue launch failed");
+    return 1;
+  }
+  if (f) {
+    HIP_OK(hipStreamWaitEvent(ws->stream, q->done, 0));
+    ++ws->batch_queued;
+  }
+  return 0;
+}
+
+static int batch_join(Workspace *ws, const T1Field *f, size_t lds, bool limited) {
+  return batch_arrive(ws, f, lds, limited);
+}
+
+void batch_leave(Workspace *ws) {
+  if (ws->batch && !ws->batch_joined) (void)batch_arrive(ws, nullptr, 0, false);
+}
+
+int batch_destroy(BatchQueue *q) {
+  if (!q) return 0;
+  int rc = q->err;
+  if (q->stream && hipStreamSynchronize(q->stream) != hipSuccess) rc = 1;
+  for (hipEvent_t e : q->ready)
+    if (e) (void)hipEventDestroy(e);
+  if (q->done) (void)hipEventDestroy(q->done);
+  if (q->stream) (void)hipStreamDestroy(q->stream);
+  if (q->d_fields) (void)hipFree(q->d_fields);
+  if (q->d_items) (void)hipFree(q->d_items);
+  if (q->d_misc) (void)hipFree(q->d_misc);
+  delete q;
+  return rc;
 }
 
 // ---------------------------------------------------------------------------

@@ -94,6 +94,11 @@ struct Workspace {
   std::vector<RecSlot> rec_slots;
   // device rate control (pcrd.hip): its buffers and per-tile tables
   void *rc = nullptr;
+  // the batch whose global codeblock queue codes this workspace's next
+  // tier-1 call (bwc_amd_compress_batch), and whether the call has joined it
+  struct BatchQueue *batch = nullptr;
+  bool batch_joined = false;
+  uint64_t batch_queued = 0;   // tier-1 calls coded through a batch's queue
   uint8_t *d_U = nullptr, *d_zlim = nullptr;
   // pass-limit network: features (16 per codeblock), tile statistics,
   // weights on the device
@@ -227,6 +232,17 @@ int rc_tile(Workspace *ws, bwc_codec *c, Job *job, uint64_t tile, SkipCheck *chk
 // triples, at most cap; returns the count
 int rc_trace(Workspace *ws, uint32_t *layer, uint32_t *slope, int64_t *est, int cap);
 void rc_release(Workspace *ws);
+// A batch's global codeblock queue: the tier-1 calls of `fields` workspaces
+// (each with batch set) join it after their codeblock lists are built; the
+// last to join launches one symbol-generation and one MQ launch over every
+// field's codeblocks (largest codings first) on the batch's stream, and all
+// go on once those are queued. max_items bounds the codeblocks of all fields.
+BatchQueue *batch_create(int fields, uint64_t max_items);
+// a workspace whose call will not join (an error before tier-1, or a field
+// coded on the general tier-1 path): counted as joined with no codeblocks
+void batch_leave(Workspace *ws);
+// wait for the batch's launches and free it; nonzero when a launch failed
+int batch_destroy(BatchQueue *q);
 const char *rc_last_error();
 // the header writer's stuffed byte count of n bit strings, by the device
 // rate control's kernels (string i: nbits[i] bits from word woff[i])

@@ -28,6 +28,7 @@ const Knob kKnobs[] = {
     {"dwt_unroll", "BWC_AMD_DWT_UNROLL", &Tuning::dwt_unroll, 2, 8, true},
     {"dwt_seg_log2", "BWC_AMD_DWT_SEG_LOG2", &Tuning::dwt_seg_log2, 8, 24, false},
     {"dwt_seg_min", "BWC_AMD_DWT_SEG_MIN", &Tuning::dwt_seg_min, 2, 1 << 20, false},
+    {"dwt_nt", "BWC_AMD_DWT_NT", &Tuning::dwt_nt, 0, 3, false},
     {"mq_lanes_full", "BWC_AMD_MQ_LANES_FULL", &Tuning::mq_lanes_full, 1, 32, true},
     {"mq_lanes_limited", "BWC_AMD_MQ_LANES_LIMITED", &Tuning::mq_lanes_limited, 1, 32, true},
 };

@@ -11,6 +11,9 @@ struct Tuning {
   // batch (2, 4 or 8), thread target 2^dwt_seg_log2 per pass, minimum pair
   // iterations per line segment
   int dwt_unroll = 4, dwt_seg_log2 = 17, dwt_seg_min = 8;
+  // forward 9/7 passes: bit 0 stores, bit 1 loads of the streaming kernels
+  // with the nontemporal hint (data read or written once per pass)
+  int dwt_nt = 3;
   // codeblocks per MQ wavefront (1, 2, 4, 8, 16 or 32): complete codings and
   // codings to a pass limit
   int mq_lanes_full = 16, mq_lanes_limited = 4;

@@ -161,6 +161,11 @@ int bwc_amd_cblk_cuts(bwc_codec *codec, uint64_t tile, uint8_t param, int16_t *c
    header bytes); at most cap entries written, the count returned (-1 for a
    codec that has not compressed). */
 int bwc_amd_rc_trace(bwc_codec *codec, uint32_t *layer, uint32_t *slope, int64_t *est, int cap);
+/* Tier-1 calls of this codec whose symbol generation and MQ coding ran in
+   the global codeblock queue of a bwc_amd_compress_batch call (fields of one
+   tile and one variable; BWC_AMD_BATCH=threads codes each field on its
+   own). */
+uint64_t bwc_amd_batch_queued(bwc_codec *codec);
 /* Pipeline graphs of a codec (BWC_AMD_GRAPH unset or 1; 0 turns them off):
    the forward DWT with tier-1 of each (tile, parameter), and the rate
    control of each tile, are recorded as HIP graphs the second time their

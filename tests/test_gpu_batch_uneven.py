@@ -6,7 +6,9 @@ min(5, floor(log2(shortest axis)) - 2), the workload's codeblock shapes),
 with element counts log-uniform in [1e4, 1e6] instead of [1e5, 1e8].
 Every codestream of one concurrent bwc_amd_compress_batch call, plain and
 with the learned pass limits (network and margin from skipnet.choose), must
-equal the oracle's for that field."""
+equal the oracle's for that field, with the fields' tier-1 coded through the
+batch's global codeblock queue (one generation and one MQ launch over all
+fields) and with each field coded on its own (BWC_AMD_BATCH=threads)."""
 import numpy as np
 import pytest
 
@@ -36,9 +38,14 @@ def fields_and_oracle():
     return fields, ref
 
 
+@pytest.mark.parametrize("mode", ["queue", "threads"])
 @pytest.mark.parametrize("learned", [False, True])
-def test_uneven_batch_matches_oracle(fields_and_oracle, learned):
+def test_uneven_batch_matches_oracle(fields_and_oracle, learned, mode, monkeypatch):
     fields, ref = fields_and_oracle
+    if mode == "threads":
+        monkeypatch.setenv("BWC_AMD_BATCH", "threads")
+    else:
+        monkeypatch.delenv("BWC_AMD_BATCH", raising=False)
     codecs = []
     try:
         for f, x in zip(SPEC, fields):
@@ -52,6 +59,11 @@ def test_uneven_batch_matches_oracle(fields_and_oracle, learned):
         sizes = bwc.compress_batch(codecs, [x.ctypes.data for x in fields], workers=8)
         for i, (c, n) in enumerate(zip(codecs, sizes)):
             assert c.out[:n].tobytes() == ref[i], (i, SPEC[i]["shape"], SPEC[i]["dtype"], SPEC[i]["rate"])
+            assert c.batch_queued() == (1 if mode == "queue" else 0), i
+        # a second batch call on the same codecs
+        sizes = bwc.compress_batch(codecs, [x.ctypes.data for x in fields], workers=8)
+        for i, (c, n) in enumerate(zip(codecs, sizes)):
+            assert c.out[:n].tobytes() == ref[i], i
     finally:
         for c in codecs:
             c.close()
