@@ -932,8 +932,9 @@ int bwc_amd_compress_batch(bwc_codec *const *codecs, bwc_stream *const *streams,
     }
   }
   gpu::BatchQueue *bq = nullptr;
+  const int slots = workers;
   if (in_queue.size() > 1 && in_queue.size() <= 512) {
-    bq = gpu::batch_create((int)in_queue.size(), items);
+    bq = gpu::batch_create((int)in_queue.size(), items, slots);
     if (!bq) {
       set_err(gpu::last_error());
       return 1;
@@ -943,8 +944,8 @@ int bwc_amd_compress_batch(bwc_codec *const *codecs, bwc_stream *const *streams,
       ws->batch = bq;
       ws->batch_joined = false;
     }
-    // every queued field needs a thread of its own until the launch
-    workers = std::max(workers, std::min(n, (int)in_queue.size() + (workers - 1)));
+    // every queued field needs a thread of its own until the launch; after it
+    // `workers` of them go on at a time
     workers = std::max(workers, (int)in_queue.size());
   }
   std::atomic<int> next{0};

@@ -2867,7 +2867,7 @@ int gather(Workspace *ws, CblkRecords &rec, const std::vector<uint32_t> &len) {
 struct BatchQueue {
   std::mutex m;
   std::condition_variable cv;
-  int expected = 0, joined = 0, err = 0;
+  int expected = 0, joined = 0, err = 0, slots = 1;
   bool launched = false;
   std::vector<T1Field> fields;
   std::vector<hipEvent_t> ready;   // each field's list built
@@ -2919,9 +2919,10 @@ __global__ void batch_zero_kernel(uint32_t *misc, int n) {
   for (int k = threadIdx.x; k < n; k += blockDim.x) misc[k] = 0;
 }
 
-BatchQueue *batch_create(int fields, uint64_t max_items) {
+BatchQueue *batch_create(int fields, uint64_t max_items, int slots) {
   BatchQueue *q = new BatchQueue();
   q->expected = fields;
+  q->slots = std::max(1, slots);
   q->max_items = max_items;
   (void)hipGetDevice(&q->device);
   if (hipStreamCreateWithFlags(&q->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -3000,6 +3001,11 @@ static int batch_arrive(Workspace *ws, const T1Field *f, size_t lds, bool limite
       q->cv.wait(lk, [q] { return q->launched; });
     }
     err = q->err;
+    if (f) {
+      q->cv.wait(lk, [q] { return q->slots > 0; });
+      --q->slots;
+      ws->batch_slot = true;
+    }
   }
   if (err) {
     std::snprintf(g_err, sizeof g_err, "batch queue launch failed");
@@ -3018,6 +3024,12 @@ static int batch_join(Workspace *ws, const T1Field *f, size_t lds, bool limited)
 
 void batch_leave(Workspace *ws) {
   if (ws->batch && !ws->batch_joined) (void)batch_arrive(ws, nullptr, 0, false);
+  if (ws->batch && ws->batch_slot) {
+    std::lock_guard<std::mutex> lk(ws->batch->m);
+    ++ws->batch->slots;
+    ws->batch_slot = false;
+    ws->batch->cv.notify_all();
+  }
 }
 
 int batch_destroy(BatchQueue *q) {

@@ -97,7 +97,7 @@ struct Workspace {
   // the batch whose global codeblock queue codes this workspace's next
   // tier-1 call (bwc_amd_compress_batch), and whether the call has joined it
   struct BatchQueue *batch = nullptr;
-  bool batch_joined = false;
+  bool batch_joined = false, batch_slot = false;
   uint64_t batch_queued = 0;   // tier-1 calls coded through a batch's queue
   uint8_t *d_U = nullptr, *d_zlim = nullptr;
   // pass-limit network: features (16 per codeblock), tile statistics,
@@ -237,9 +237,12 @@ void rc_release(Workspace *ws);
 // last to join launches one symbol-generation and one MQ launch over every
 // field's codeblocks (largest codings first) on the batch's stream, and all
 // go on once those are queued. max_items bounds the codeblocks of all fields.
-BatchQueue *batch_create(int fields, uint64_t max_items);
-// a workspace whose call will not join (an error before tier-1, or a field
-// coded on the general tier-1 path): counted as joined with no codeblocks
+// After the launches at most `slots` of the joined calls go on at a time
+// (the host side of rate control, packets and assembly).
+BatchQueue *batch_create(int fields, uint64_t max_items, int slots);
+// at the end of a batch call's compression: a call that never joined (an
+// error before tier-1, or a field coded on the general tier-1 path) counts
+// as joined with no codeblocks; a call that joined gives back its slot
 void batch_leave(Workspace *ws);
 // wait for the batch's launches and free it; nonzero when a launch failed
 int batch_destroy(BatchQueue *q);
