@@ -90,6 +90,13 @@ Workspace::~Workspace() {
   if (h_up) (void)hipHostFree(h_up);
   if (h_down) (void)hipHostFree(h_down);
   if (h_kz) (void)hipHostFree(h_kz);
+  for (int b = 0; b < kStageBufs; ++b) {
+    if (ev_stage[b]) {
+      (void)hipEventSynchronize(ev_stage[b]);
+      (void)hipEventDestroy(ev_stage[b]);
+    }
+    if (h_stage[b]) (void)hipHostFree(h_stage[b]);
+  }
   for (auto &e : ev)
     if (e) (void)hipEventDestroy(e);
   if (copy_stream) (void)hipStreamSynchronize(copy_stream);
@@ -335,13 +342,51 @@ int fill_minmax(Workspace *ws, const bwc_codec *c, const Tile &tl, int param, co
       // contiguous when d == nZ)
       const uint64_t run = w * h * d;
       const bool whole = d == nZ;
+      // Pageable memory: the host's threads copy each chunk into one of the
+      // page-locked bounce buffers (in parallel, one chunk after another),
+      // and the DMA of a chunk runs while the next is copied. BWC_AMD_STAGE=0
+      // hands the pageable source to hipMemcpyAsync instead.
+      const char *se = std::getenv("BWC_AMD_STAGE");
+      const bool bounce = !(se && se[0] == '0');
+      if (bounce && ws->cap_stage < kFillChunkBytes) {
+        for (int b = 0; b < Workspace::kStageBufs; ++b) {
+          if (ws->h_stage[b]) HIP_OK(hipHostFree(ws->h_stage[b]));
+          ws->h_stage[b] = nullptr;
+        }
+        ws->cap_stage = 0;
+        for (int b = 0; b < Workspace::kStageBufs; ++b) {
+          HIP_OK(hipHostMalloc((void **)&ws->h_stage[b], kFillChunkBytes));
+          if (!ws->ev_stage[b]) HIP_OK(hipEventCreateWithFlags(&ws->ev_stage[b], hipEventDisableTiming));
+        }
+        ws->cap_stage = kFillChunkBytes;
+      }
+      const int hthreads = rc_thread_count();
+      int nb = 0;
       for (uint64_t t = 0; t < (whole ? 1 : dt); ++t) {
         const uint64_t len = whole ? n : run;
         const uint64_t sbase = poff + nX * nY * (z0 + nZ * t);
         for (uint64_t o = 0; o < len; o += cvals) {
           const uint64_t m = std::min(cvals, len - o);
-          HIP_OK(hipMemcpyAsync(dst + (t * run + o) * prec, src + (sbase + o) * prec, m * prec,
-                                hipMemcpyHostToDevice, ws->copy_stream));
+          if (bounce) {
+            const int b = nb++ % Workspace::kStageBufs;
+            // the buffer's last DMA has finished reading it
+            HIP_OK(hipEventSynchronize(ws->ev_stage[b]));
+            const char *from = src + (sbase + o) * prec;
+            uint8_t *to = ws->h_stage[b];
+            const uint64_t bytes = m * prec;
+            const size_t parts = (size_t)std::max<uint64_t>(1, std::min<uint64_t>(16, bytes >> 22));
+            const std::function<void(size_t)> cp = [&](size_t k) {
+              const uint64_t a = (bytes * k / parts) & ~63ull, e = k + 1 == parts ? bytes : (bytes * (k + 1) / parts) & ~63ull;
+              std::memcpy(to + a, from + a, e - a);
+            };
+            if (parts > 1 && hthreads > 1) host_pool_run(std::min(hthreads, 16) - 1, parts, cp);
+            else cp(0);
+            HIP_OK(hipMemcpyAsync(dst + (t * run + o) * prec, to, bytes, hipMemcpyHostToDevice, ws->copy_stream));
+            HIP_OK(hipEventRecord(ws->ev_stage[b], ws->copy_stream));
+          } else {
+            HIP_OK(hipMemcpyAsync(dst + (t * run + o) * prec, src + (sbase + o) * prec, m * prec,
+                                  hipMemcpyHostToDevice, ws->copy_stream));
+          }
           chunks.push_back({t * run + o, m});
           if (ws->ev_copy.size() < chunks.size()) {
             hipEvent_t e;

@@ -23,6 +23,13 @@ struct Workspace {
   hipStream_t copy_stream = nullptr;
   std::vector<hipEvent_t> ev_copy;
   hipEvent_t ev_fill = nullptr;
+  // pageable host fields: page-locked bounce buffers the host threads copy
+  // chunks into (several at once) while earlier chunks cross PCIe; ev_stage
+  // marks the end of the last copy out of each
+  static constexpr int kStageBufs = 4;
+  uint8_t *h_stage[kStageBufs] = {};
+  size_t cap_stage = 0;
+  hipEvent_t ev_stage[kStageBufs] = {};
   // field staging (host input copied here) and the tile working buffer (f64)
   void *d_input = nullptr;
   size_t input_cap = 0;

@@ -78,3 +78,26 @@ def test_graphs_off_gives_the_same_bytes(monkeypatch):
         assert (counts[1] > 0) == (g == "1"), counts
     assert outs[0] == outs[1]
     assert outs[0][0] == ol.oracle_compress(x, shape, **kw)
+
+
+@pytest.mark.parametrize("skip", ["plain", "learned"])
+def test_graph_replays_with_changing_fields(skip, monkeypatch):
+    """One codec, graphs recorded on one field and replayed on others: each
+    call's codestream equals the oracle's for the field it was given."""
+    monkeypatch.delenv("BWC_AMD_GRAPH", raising=False)
+    monkeypatch.delenv("BWC_AMD_RC", raising=False)
+    shape, kw = (80, 72, 48), dict(rate="1,3", decomp=(4, 4, 3, 0), cb=(5, 5, 3, 0))
+    xs = [grf(shape, slope=s, seed=20 + k) for k, s in enumerate((-1.2, -5.0 / 3.0, -2.6))]
+    refs = [ol.oracle_compress(x, shape, **kw) for x in xs]
+    c = bwc.Codec(shape, **kw)
+    try:
+        if skip == "learned":
+            c.set_skip_model(skipnet.wide_weights())
+            c.set_pass_skip(2, 3)
+        for k in range(9):
+            i = (k * 2) % 3 if k > 2 else 1
+            print("changing fields", skip, k, i, file=sys.stderr, flush=True)
+            assert c.compress(xs[i]) == refs[i], (k, i)
+        assert c.graph_counts()[1] > 0
+    finally:
+        c.close()
