@@ -2232,7 +2232,10 @@ static GenParams gen_params(const Param &par) {
     const uint32_t P = gw <= 64 ? std::max(1u, std::min(64 / gw, nsl)) : 1u;
     g.obuf_bytes = std::max(g.obuf_bytes, P * (cw * ch * 2 + cw * ns * 3 + 16));
   }
-  g.lds = (size_t)g.maxcol * 2 + (size_t)((g.maxcol + 1) / 2) * 8 + (size_t)g.obuf_bytes;
+  // column states, the sign plane and the current plane (each to 4 bytes),
+  // then the pass-output buffers
+  const size_t hq = ((g.maxcol + 1) / 2 + 3) & ~3u;
+  g.lds = (((size_t)g.maxcol * 2 + 3) & ~(size_t)3) + 2 * hq + (size_t)g.obuf_bytes;
   return g;
 }
 
