@@ -2213,8 +2213,9 @@ __global__ void __launch_bounds__(1024) t1_list_kernel(uint64_t ncb, const uint8
 }
 
 // LDS of the symbol generator: per codeblock it packs P = 64 / pow2ceil(width)
-// slices side by side, each with its own pass-output buffer; the buffers take
-// the largest, over the codeblocks, of P times the codeblock's per-slice bound
+// slices side by side, each after the first with its own pass-output buffer;
+// the buffers take the largest, over the codeblocks, of P - 1 times the
+// codeblock's per-slice bound
 struct GenParams {
   uint32_t maxcol = 0, obuf_bytes = 0;
   size_t lds = 0;
@@ -2230,7 +2231,7 @@ static GenParams gen_params(const Param &par) {
     while (gw < cw) gw <<= 1;
     const uint32_t nsl = (b[6] - b[2]) * (b[7] - b[3]);
     const uint32_t P = gw <= 64 ? std::max(1u, std::min(64 / gw, nsl)) : 1u;
-    g.obuf_bytes = std::max(g.obuf_bytes, P * (cw * ch * 2 + cw * ns * 3 + 16));
+    g.obuf_bytes = std::max(g.obuf_bytes, (P - 1) * (cw * ch * 2 + cw * ns * 3 + 16));
   }
   // column states, the sign plane and the current plane (each to 4 bytes),
   // then the pass-output buffers
