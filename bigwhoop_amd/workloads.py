@@ -4,6 +4,7 @@
     python -m bigwhoop_amd.workloads layers512     # 512^3 f64, 4 layers at 1, 2, 4, 8 bpv
     python -m bigwhoop_amd.workloads turb4d        # 256^3 x 16 x 5 variables, 2 bpv
     python -m bigwhoop_amd.workloads batch64       # 64 uneven fields (1-4 axes, f32/f64)
+    python -m bigwhoop_amd.workloads modes64       # 64^3 f64, four modes + noise, 4 bpv
 
 Each prints one JSON line per case: input GB/s of the plain encoder (every
 pass coded) and of the learned pass-skip encoder on the same field, the
@@ -20,6 +21,10 @@ the run fails otherwise). Fields are synthesised in HBM with torch's FFT
   background; in time the complex spectral coefficients follow a
   first-order AR(0.95) process, so the phases evolve smoothly. 5 spatial and
   2 temporal DWT levels, codeblocks 16x16x8x4, 2 bits per value.
+* modes64: the reference's CPU-runnable case: 64^3 float64, the sum of four
+  seeded sinusoidal modes (|k| <= 8) plus N(0, 1e-3) noise, 3 levels per
+  axis, codeblocks 16x16x16, one layer at 4 bits per value (made on the host
+  and copied to HBM).
 """
 from __future__ import annotations
 
@@ -27,6 +32,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 def _top_rate(rate: str) -> float:
     return max(float(r) for r in rate.replace("-", ",").replace(" ", ",").split(",") if r)
@@ -327,6 +334,13 @@ def main(argv):
             f = grf_device(n)
             compare("512^3 k^-5/3, layers 1,2,4,8 bpv", (n, n, n), 1, f.data_ptr(), n ** 3 * 8,
                     dict(rate="1,2,4,8", decomp=(5, 5, 5, 0), cb=(5, 5, 3, 0)))
+            del f
+        elif w == "modes64":
+            from bigwhoop_amd.fields import modes_field
+
+            f = torch.from_numpy(np.ascontiguousarray(modes_field((64, 64, 64), seed=1)).reshape(-1)).to("cuda")
+            compare("64^3 four modes + noise at 4 bpv", (64, 64, 64), 1, f.data_ptr(), f.numel() * 8,
+                    dict(rate="4", decomp=(3, 3, 3, 0), cb=(4, 4, 4, 0)), steps=10)
             del f
         elif w == "batch64":
             batch64()
