@@ -24,6 +24,7 @@ from __future__ import annotations
 import argparse
 import itertools
 import json
+import os
 import statistics
 import sys
 import time
@@ -32,7 +33,7 @@ from pathlib import Path
 DEFAULT_OUT = Path(__file__).resolve().parent / "data" / "tuning_gfx950.txt"
 SETTINGS = dict(rate="2", decomp=(5, 5, 5, 0), cb=(5, 5, 3, 0))
 DWT_CANDIDATES = dict(dwt_unroll=(2, 4, 8), dwt_seg_log2=(15, 16, 17, 18, 19), dwt_seg_min=(4, 8, 16))
-MQ_LIMITED = (2, 4, 8, 16)
+MQ_LIMITED = (2, 3, 4, 5, 6, 8)
 MQ_FULL = (8, 16, 32)
 STAGE_DWT, STAGE_T1 = 2, 4   # indices into bwc_amd_last_timings
 
@@ -52,6 +53,10 @@ def autotune(n=512, out=DEFAULT_OUT, reps=3, log=print):
     import torch
     from bigwhoop_amd import bwc, skipnet
     from bigwhoop_amd.fields import grf_device
+
+    # stage times of direct launches: with pipeline graphs the DWT runs inside
+    # tier-1's graph and has no stage time of its own
+    os.environ["BWC_AMD_GRAPH"] = "0"
 
     field = grf_device(n)
     ptr = field.data_ptr()

@@ -2288,7 +2288,10 @@ static int v2_launch(Workspace *ws, const bwc_codec *c, uint64_t nl, const uint8
   switch (lpw) {
     case 1: MQ_LAUNCH(1); break;
     case 2: MQ_LAUNCH(2); break;
+    case 3: MQ_LAUNCH(3); break;
     case 4: MQ_LAUNCH(4); break;
+    case 5: MQ_LAUNCH(5); break;
+    case 6: MQ_LAUNCH(6); break;
     case 8: MQ_LAUNCH(8); break;
     case 32: MQ_LAUNCH(32); break;
     default: MQ_LAUNCH(16); break;
@@ -3014,7 +3017,10 @@ static int batch_launch(BatchQueue *q) {
   switch (lpw) {
     case 1: MQQ(1); break;
     case 2: MQQ(2); break;
+    case 3: MQQ(3); break;
     case 4: MQQ(4); break;
+    case 5: MQQ(5); break;
+    case 6: MQQ(6); break;
     case 8: MQQ(8); break;
     case 32: MQQ(32); break;
     default: MQQ(16); break;

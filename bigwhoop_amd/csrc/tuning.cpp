@@ -3,6 +3,7 @@
 
 #include <dlfcn.h>
 
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -22,15 +23,19 @@ struct Knob {
   int Tuning::*field;
   int lo, hi;
   bool pow2;
+  uint64_t only;   // when nonzero: the valid values, bit v for value v
 };
 
+// the MQ kernel's instantiations: 1-6, 8, 16 or 32 codeblocks per wavefront
+constexpr uint64_t kMqLanes = 0x7Eull | 1ull << 8 | 1ull << 16 | 1ull << 32;
+
 const Knob kKnobs[] = {
-    {"dwt_unroll", "BWC_AMD_DWT_UNROLL", &Tuning::dwt_unroll, 2, 8, true},
-    {"dwt_seg_log2", "BWC_AMD_DWT_SEG_LOG2", &Tuning::dwt_seg_log2, 8, 24, false},
-    {"dwt_seg_min", "BWC_AMD_DWT_SEG_MIN", &Tuning::dwt_seg_min, 2, 1 << 20, false},
-    {"dwt_nt", "BWC_AMD_DWT_NT", &Tuning::dwt_nt, 0, 3, false},
-    {"mq_lanes_full", "BWC_AMD_MQ_LANES_FULL", &Tuning::mq_lanes_full, 1, 32, true},
-    {"mq_lanes_limited", "BWC_AMD_MQ_LANES_LIMITED", &Tuning::mq_lanes_limited, 1, 32, true},
+    {"dwt_unroll", "BWC_AMD_DWT_UNROLL", &Tuning::dwt_unroll, 2, 8, true, 0},
+    {"dwt_seg_log2", "BWC_AMD_DWT_SEG_LOG2", &Tuning::dwt_seg_log2, 8, 24, false, 0},
+    {"dwt_seg_min", "BWC_AMD_DWT_SEG_MIN", &Tuning::dwt_seg_min, 2, 1 << 20, false, 0},
+    {"dwt_nt", "BWC_AMD_DWT_NT", &Tuning::dwt_nt, 0, 3, false, 0},
+    {"mq_lanes_full", "BWC_AMD_MQ_LANES_FULL", &Tuning::mq_lanes_full, 1, 32, true, 0},
+    {"mq_lanes_limited", "BWC_AMD_MQ_LANES_LIMITED", &Tuning::mq_lanes_limited, 1, 32, false, kMqLanes},
 };
 
 const Knob *find(const char *name) {
@@ -41,6 +46,7 @@ const Knob *find(const char *name) {
 
 bool valid(const Knob &k, int v) {
   if (v < k.lo || v > k.hi) return false;
+  if (k.only) return v < 64 && ((k.only >> v) & 1);
   return !k.pow2 || (v & (v - 1)) == 0;
 }
 

@@ -11,7 +11,7 @@ import pytest
 from bigwhoop_amd import bwc
 
 ROOT = Path(__file__).resolve().parents[1]
-DEFAULTS = {"dwt_unroll": 4, "dwt_seg_log2": 17, "dwt_seg_min": 8, "mq_lanes_full": 16, "mq_lanes_limited": 4}
+DEFAULTS = {"dwt_unroll": 4, "dwt_seg_log2": 17, "dwt_seg_min": 8, "mq_lanes_full": 16, "mq_lanes_limited": 5}
 
 
 def run_py(code, env):
@@ -31,7 +31,7 @@ def test_set_get_and_ranges():
     try:
         for name, good, bad in [("dwt_unroll", [2, 4, 8], [1, 3, 16]), ("dwt_seg_log2", [8, 17, 24], [7, 25]),
                                 ("dwt_seg_min", [2, 100], [0, 1]), ("mq_lanes_full", [1, 2, 32], [0, 3, 64]),
-                                ("mq_lanes_limited", [4, 16], [5, 33])]:
+                                ("mq_lanes_limited", [3, 5, 6, 16], [7, 12, 33])]:
             for v in good:
                 bwc.set_tuning(name, v)
                 assert bwc.get_tuning(name) == v
