@@ -777,6 +777,139 @@ __global__ void __launch_bounds__(256) dwt97_stream_kernel(const double *__restr
   }
 }
 
+// X and Y passes of one level in one kernel, for boxes whose X lines fit
+// LDS: the box is read once and written once for both axes. A workgroup of
+// 256 threads takes one (Z, T) plane and one segment of its Y iterations.
+// Per batch, the four wavefronts X-transform 2U rows of the plane (the Y
+// pipeline's next working indices, mirrored into the plane) in LDS with
+// dwt97_x_kernel's steps; then every thread advances the Y pipeline of its
+// X-transformed positions o = t, t + 256, ... over those rows with
+// dwt97_stream_kernel's steps and stores the outputs. Every coefficient is
+// the same sequence of binary64 operations as in the two separate passes.
+// Row slot r of LDS: Ev = lds + 2 hp r, Od = Ev + hp, as in dwt97_x_kernel.
+template <int U, int NT, int CMAX>
+__global__ void __launch_bounds__(256) dwt97_xy_kernel(const double *__restrict__ src, double *__restrict__ dst,
+                                                       int lenx, int par0x, int halfx, int leny, int par0y, int halfy,
+                                                       uint64_t sy, uint32_t n2, uint64_t s2, uint64_t s3, int nseg,
+                                                       int ksz, const double *norm, uint64_t shift) {
+  extern __shared__ double lds[];
+  const int hp = (lenx + 16) / 2 + 1;
+  const uint32_t plane = blockIdx.x / (uint32_t)nseg, seg = blockIdx.x % (uint32_t)nseg;
+  const uint64_t base = (uint64_t)(plane % n2) * s2 + (uint64_t)(plane / n2) * s3;
+  const double *s = src + base;
+  double *d = dst + base;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const double alpha = norm ? norm[2] : 0, beta = norm ? norm[3] : 0;
+  const int last0 = -3 + 2 * ((leny + 7) / 2 - 1), last1 = -2 + 2 * ((leny + 5) / 2 - 1);
+  const int last2 = -1 + 2 * ((leny + 3) / 2 - 1), last3 = 2 * ((leny + 1) / 2 - 1);
+  const int ilo = -par0y, ihi = leny - par0y;
+  const int kend = (ihi + 5) / 2;
+  const int kA = 1 + (int)seg * ksz, kB = min(kend, kA + ksz);
+  if (kA >= kB) return;   // the whole workgroup
+  // rows with Y working indices i0 .. i0 + nrows - 1 into slots 0 .. nrows - 1,
+  // X-transformed; slot r belongs to wavefront r % 4
+  auto xrows = [&](int i0, int nrows) {
+    for (int r = wv; r < nrows; r += 4) {
+      const int p = i0 + r + par0y;
+      const double *row = s + (uint64_t)(((unsigned)p < (unsigned)leny) ? p : mirror32(p, leny)) * sy;
+      double *Ev = lds + (size_t)2 * hp * r, *Od = Ev + hp;
+      for (int e0 = 0; e0 < lenx + 16; e0 += 64 * 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int e = e0 + 64 * u + lane, q = e - 8 + par0x;
+          const double *a = &row[((unsigned)q < (unsigned)lenx) ? q : mirror32(q, lenx)];
+          v[u] = (e < lenx + 16) ? st_ld<(NT & 2) != 0>(a) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int e = e0 + 64 * u + lane;
+          if (e >= lenx + 16) break;
+          const double x = norm ? norm_value(v[u], alpha, beta, shift) : v[u];
+          if (e & 1) Od[e >> 1] = x;
+          else Ev[e >> 1] = x;
+        }
+      }
+    }
+    __syncthreads();
+    for (int r = wv; r < nrows; r += 4) {
+      double *Ev = lds + (size_t)2 * hp * r, *Od = Ev + hp;
+      for (int m = lane; m < (lenx + 7) / 2; m += 64) Od[2 + m] += Lift97::A * (Ev[2 + m] + Ev[3 + m]);
+    }
+    __syncthreads();
+    for (int r = wv; r < nrows; r += 4) {
+      double *Ev = lds + (size_t)2 * hp * r, *Od = Ev + hp;
+      for (int m = lane; m < (lenx + 5) / 2; m += 64) Ev[3 + m] += Lift97::B * (Od[2 + m] + Od[3 + m]);
+    }
+    __syncthreads();
+    for (int r = wv; r < nrows; r += 4) {
+      double *Ev = lds + (size_t)2 * hp * r, *Od = Ev + hp;
+      for (int m = lane; m < (lenx + 3) / 2; m += 64) {
+        const double t = Od[3 + m] + Lift97::G * (Ev[3 + m] + Ev[4 + m]);
+        Od[3 + m] = Lift97::KH * t;
+      }
+    }
+    __syncthreads();
+    for (int r = wv; r < nrows; r += 4) {
+      double *Ev = lds + (size_t)2 * hp * r, *Od = Ev + hp;
+      for (int m = lane; m < (lenx + 1) / 2; m += 64) {
+        const double t = Ev[4 + m] + Lift97::D * (Od[3 + m] + Od[4 + m]);
+        Ev[4 + m] = Lift97::KL * t;
+      }
+    }
+    __syncthreads();
+  };
+  // X output o of slot r
+  auto xval = [&](int r, int o) -> double {
+    const double *Ev = lds + (size_t)2 * hp * r, *Od = Ev + hp;
+    return (o < halfx) ? Ev[o + 4] : Od[o - halfx + 4 - par0x];
+  };
+  // carried per column: x_{2k-2}, x_{2k-1}, a_{2k-3}, b_{2k-4}, c_{2k-5}
+  double xm2[CMAX], xm1[CMAX], am3[CMAX], bm4[CMAX], cm5[CMAX];
+  const int k0 = kA - 4;
+  xrows(2 * k0 - 2, 2);
+#pragma unroll
+  for (int c = 0; c < CMAX; ++c) {
+    const int o = threadIdx.x + 256 * c;
+    xm2[c] = o < lenx ? xval(0, o) : 0.0;
+    xm1[c] = o < lenx ? xval(1, o) : 0.0;
+    am3[c] = bm4[c] = cm5[c] = 0.0;
+  }
+  __syncthreads();
+  for (int k = k0; k < kB; k += U) {
+    xrows(2 * k, 2 * U);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int kk = k + u;
+      if (kk >= kB) break;
+#pragma unroll
+      for (int c = 0; c < CMAX; ++c) {
+        const int o = threadIdx.x + 256 * c;
+        if (o >= lenx) continue;
+        const double x0 = xval(2 * u, o), x1 = xval(2 * u + 1, o);
+        int i = 2 * kk - 1;
+        const double a1 = (i >= -3 && i <= last0) ? xm1[c] + Lift97::A * (xm2[c] + x0) : xm1[c];
+        i = 2 * kk - 2;
+        const double b2 = (i >= -2 && i <= last1) ? xm2[c] + Lift97::B * (am3[c] + a1) : xm2[c];
+        i = 2 * kk - 3;
+        const double c3 = (i >= -1 && i <= last2) ? Lift97::KH * (am3[c] + Lift97::G * (bm4[c] + b2)) : am3[c];
+        i = 2 * kk - 4;
+        const double d4 = (i >= 0 && i <= last3) ? Lift97::KL * (bm4[c] + Lift97::D * (cm5[c] + c3)) : bm4[c];
+        if (kk >= kA && i >= ilo && i < ihi) st_st<(NT & 1) != 0>(&d[(uint64_t)(i >> 1) * sy + o], d4);
+        i = 2 * kk - 3;
+        if (kk >= kA && i >= ilo && i < ihi)
+          st_st<(NT & 1) != 0>(&d[(uint64_t)(halfy + ((i + 2 * par0y - 1) >> 1)) * sy + o], c3);
+        xm2[c] = x0;
+        xm1[c] = x1;
+        am3[c] = a1;
+        bm4[c] = b2;
+        cm5[c] = c3;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // copy the box [0,n0) x [0,n1) x [0,n2) x [0,n3) (strides 1, s1, s2, s3)
 __global__ void box_copy_kernel(const double *__restrict__ src, double *__restrict__ dst, uint32_t n0, uint32_t n1,
                                 uint32_t n2, uint64_t s1, uint64_t s2, uint64_t s3, uint64_t total) {
@@ -825,7 +958,9 @@ static int dwt_passes(DwtCtx &x, const bwc_codec *c, const Param &par, double *w
   // nontemporal hints
   struct {
     int unroll, seg_log2, seg_min, nt;
-  } tune = {tuning().dwt_unroll, tuning().dwt_seg_log2, tuning().dwt_seg_min, tuning().dwt_nt};
+    bool no_xy;
+  } tune = {tuning().dwt_unroll, tuning().dwt_seg_log2, tuning().dwt_seg_min, tuning().dwt_nt,
+            tuning().dwt_xy == 0};
   for (int step = 0; step < nd; ++step) {
     const int level = fwd ? step : nd - 1 - step;
     const Box &rb = par.res[nd - level].box;
@@ -843,6 +978,10 @@ static int dwt_passes(DwtCtx &x, const bwc_codec *c, const Param &par, double *w
     // is evened out by running the X pass out of place, else by a copy back
     const int nstrided = (int)streamed(0) + (int)streamed(1) + (int)streamed(2) + (int)streamed(3);
     const bool x_oop = (nstrided & 1) && x_lds(0);
+    // X and Y in one pass (dwt97_xy_kernel) when Y streams, X fits the LDS
+    // kernel with at most 1024 samples, and the streamed passes are even in
+    // number (the fused pass is out of place and counts as Y's)
+    const bool xy = x_lds(0) && streamed(1) && !(nstrided & 1) && sz[0] <= 1024 && !tune.no_xy;
     double *cur = work, *oth = x.work2;
     if (nstrided && x.work2_cap < par.box.volume() * sizeof(double)) {
       if (!x.grow) {
@@ -892,6 +1031,43 @@ static int dwt_passes(DwtCtx &x, const bwc_codec *c, const Param &par, double *w
       // other one along the strided axes
       const int half = par0 ? (int)(len / 2) : (int)((len + 1) / 2);
       const size_t xlds = (size_t)4 * 2 * ((len + 16) / 2 + 1) * sizeof(double);
+      if (xy && a == 1) continue;   // done with X
+      if (xy && a == 0) {
+        const int leny = (int)sz[1], par0y = (int)(rb.lo[1] & 1);
+        const int halfy = par0y ? leny / 2 : (leny + 1) / 2;
+        const uint64_t planes = sz[2] * sz[3];
+        // Y iterations split into segments until about four workgroups per
+        // CU of work are there, keeping >= 16 pair iterations each
+        const int kout = (leny - par0y + 5) / 2 - 1;
+        int nseg = 1;
+        while (planes * nseg < 1024 && kout / (nseg * 2) >= 16) nseg *= 2;
+        const int ksz = (kout + nseg - 1) / nseg;
+        constexpr int XU = 4;
+        const size_t xylds = (size_t)2 * XU * 2 * ((len + 16) / 2 + 1) * sizeof(double);
+        const uint64_t shift = (uint64_t)(norm_pending ? norm_qm : 0) << 52;
+        const double *nrm = norm_pending ? x.norm : nullptr;
+#define XY_LAUNCH(NT, CM)                                                                                           \
+  hipLaunchKernelGGL((dwt97_xy_kernel<XU, NT, CM>), dim3((uint32_t)(planes * nseg)), dim3(256), xylds, x.stream, cur, \
+                     oth, (int)len, par0, half, leny, par0y, halfy, inc[1], (uint32_t)sz[2], inc[2], inc[3], nseg,   \
+                     ksz, nrm, shift)
+#define XY_LAUNCH_C(NT)                   \
+  if (len <= 256) XY_LAUNCH(NT, 1);       \
+  else if (len <= 512) XY_LAUNCH(NT, 2);  \
+  else XY_LAUNCH(NT, 4);
+        switch (tune.nt) {
+          case 1: XY_LAUNCH_C(1); break;
+          case 2: XY_LAUNCH_C(2); break;
+          case 3: XY_LAUNCH_C(3); break;
+          default: XY_LAUNCH_C(0); break;
+        }
+#undef XY_LAUNCH_C
+#undef XY_LAUNCH
+        HIP_OK(hipGetLastError());
+        norm_pending = false;
+        std::swap(cur, oth);
+        ++x.launches;
+        continue;
+      }
       if (x_lds(a)) {
         double *dst = x_oop ? oth : cur;
         hipLaunchKernelGGL(dwt97_x_kernel, dim3((uint32_t)((sz[o[0]] + 3) / 4), (uint32_t)gy), dim3(256), xlds,
@@ -1000,7 +1176,7 @@ static uint64_t dwt_key(const Workspace *ws, const bwc_codec *c, const Param &pa
   mix((uint64_t)(uintptr_t)ws->d_norm);
   mix((uint64_t)(int64_t)norm_qm);
   mix((uint64_t)tuning().dwt_unroll | (uint64_t)tuning().dwt_seg_log2 << 16 | (uint64_t)tuning().dwt_seg_min << 32 |
-      (uint64_t)tuning().dwt_nt << 56);
+      (uint64_t)tuning().dwt_nt << 56 | (uint64_t)tuning().dwt_xy << 60);
   const char *d = std::getenv("BWC_AMD_DWT");
   mix(d && d[0] == '0');
   return h;
@@ -3365,6 +3541,15 @@ static int process_setup() {
     HIP_OK(hipFuncSetAttribute((const void *)dwt_axis_kernel<1, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
     HIP_OK(hipFuncSetAttribute((const void *)dwt_axis_kernel<1, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
     HIP_OK(hipFuncSetAttribute((const void *)dwt97_x_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
+#define XY_ATTR(NT)                                                                                                 \
+  HIP_OK(hipFuncSetAttribute((const void *)dwt97_xy_kernel<4, NT, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                             lds_max));                                                                          \
+  HIP_OK(hipFuncSetAttribute((const void *)dwt97_xy_kernel<4, NT, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                             lds_max));                                                                          \
+  HIP_OK(hipFuncSetAttribute((const void *)dwt97_xy_kernel<4, NT, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                             lds_max));
+    XY_ATTR(0) XY_ATTR(1) XY_ATTR(2) XY_ATTR(3)
+#undef XY_ATTR
     tables = true;
   }
   return 0;
