@@ -784,15 +784,19 @@ __global__ void __launch_bounds__(256) dwt97_stream_kernel(const double *__restr
 // pipeline's next working indices, mirrored into the plane) in LDS with
 // dwt97_x_kernel's steps; then every thread advances the Y pipeline of its
 // X-transformed positions o = t, t + 256, ... over those rows with
-// dwt97_stream_kernel's steps and stores the outputs. Every coefficient is
-// the same sequence of binary64 operations as in the two separate passes.
-// Row slot r of LDS: Ev = lds + 2 hp r, Od = Ev + hp, as in dwt97_x_kernel.
+// dwt97_stream_kernel's steps and stores the outputs. The next batch's rows
+// are loaded into registers while the current batch's Y steps run. Every
+// coefficient is the same sequence of binary64 operations as in the two
+// separate passes. Row slot r of LDS: Ev = lds + 2 hp r, Od = Ev + hp, as in
+// dwt97_x_kernel; slot r is loaded by wavefront r % 4.
 template <int U, int NT, int CMAX>
 __global__ void __launch_bounds__(256) dwt97_xy_kernel(const double *__restrict__ src, double *__restrict__ dst,
                                                        int lenx, int par0x, int halfx, int leny, int par0y, int halfy,
                                                        uint64_t sy, uint32_t n2, uint64_t s2, uint64_t s3, int nseg,
                                                        int ksz, const double *norm, uint64_t shift) {
   extern __shared__ double lds[];
+  constexpr int NV = (256 * CMAX + 16 + 63) / 64;   // samples of an extended row per lane
+  constexpr int SPW = (2 * U + 3) / 4;              // row slots per wavefront
   const int hp = (lenx + 16) / 2 + 1;
   const uint32_t plane = blockIdx.x / (uint32_t)nseg, seg = blockIdx.x % (uint32_t)nseg;
   const uint64_t base = (uint64_t)(plane % n2) * s2 + (uint64_t)(plane / n2) * s3;
@@ -806,29 +810,38 @@ __global__ void __launch_bounds__(256) dwt97_xy_kernel(const double *__restrict_
   const int kend = (ihi + 5) / 2;
   const int kA = 1 + (int)seg * ksz, kB = min(kend, kA + ksz);
   if (kA >= kB) return;   // the whole workgroup
-  // rows with Y working indices i0 .. i0 + nrows - 1 into slots 0 .. nrows - 1,
-  // X-transformed; slot r belongs to wavefront r % 4
-  auto xrows = [&](int i0, int nrows) {
-    for (int r = wv; r < nrows; r += 4) {
+  // rows with Y working indices i0 .. i0 + nrows - 1 into registers: sample
+  // e = 64 u + lane of the extended row (X position e - 8 + par0x, mirrored)
+  double pf[SPW][NV];
+  auto fetch = [&](int i0, int nrows) {
+#pragma unroll
+    for (int j = 0; j < SPW; ++j) {
+      const int r = wv + 4 * j;
+      if (r >= nrows) break;
       const int p = i0 + r + par0y;
       const double *row = s + (uint64_t)(((unsigned)p < (unsigned)leny) ? p : mirror32(p, leny)) * sy;
+#pragma unroll
+      for (int u = 0; u < NV; ++u) {
+        const int e = 64 * u + lane, q = e - 8 + par0x;
+        pf[j][u] = (e < lenx + 16) ? st_ld<(NT & 2) != 0>(&row[((unsigned)q < (unsigned)lenx) ? q : mirror32(q, lenx)])
+                                   : 0.0;
+      }
+    }
+  };
+  // the fetched rows into their slots, normalised when asked, X-transformed
+  auto commit = [&](int nrows) {
+#pragma unroll
+    for (int j = 0; j < SPW; ++j) {
+      const int r = wv + 4 * j;
+      if (r >= nrows) break;
       double *Ev = lds + (size_t)2 * hp * r, *Od = Ev + hp;
-      for (int e0 = 0; e0 < lenx + 16; e0 += 64 * 8) {
-        double v[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int e = e0 + 64 * u + lane, q = e - 8 + par0x;
-          const double *a = &row[((unsigned)q < (unsigned)lenx) ? q : mirror32(q, lenx)];
-          v[u] = (e < lenx + 16) ? st_ld<(NT & 2) != 0>(a) : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int e = e0 + 64 * u + lane;
-          if (e >= lenx + 16) break;
-          const double x = norm ? norm_value(v[u], alpha, beta, shift) : v[u];
-          if (e & 1) Od[e >> 1] = x;
-          else Ev[e >> 1] = x;
-        }
+      for (int u = 0; u < NV; ++u) {
+        const int e = 64 * u + lane;
+        if (e >= lenx + 16) break;
+        const double x = norm ? norm_value(pf[j][u], alpha, beta, shift) : pf[j][u];
+        if (e & 1) Od[e >> 1] = x;
+        else Ev[e >> 1] = x;
       }
     }
     __syncthreads();
@@ -867,7 +880,8 @@ __global__ void __launch_bounds__(256) dwt97_xy_kernel(const double *__restrict_
   // carried per column: x_{2k-2}, x_{2k-1}, a_{2k-3}, b_{2k-4}, c_{2k-5}
   double xm2[CMAX], xm1[CMAX], am3[CMAX], bm4[CMAX], cm5[CMAX];
   const int k0 = kA - 4;
-  xrows(2 * k0 - 2, 2);
+  fetch(2 * k0 - 2, 2);
+  commit(2);
 #pragma unroll
   for (int c = 0; c < CMAX; ++c) {
     const int o = threadIdx.x + 256 * c;
@@ -875,9 +889,11 @@ __global__ void __launch_bounds__(256) dwt97_xy_kernel(const double *__restrict_
     xm1[c] = o < lenx ? xval(1, o) : 0.0;
     am3[c] = bm4[c] = cm5[c] = 0.0;
   }
+  fetch(2 * k0, 2 * U);
   __syncthreads();
   for (int k = k0; k < kB; k += U) {
-    xrows(2 * k, 2 * U);
+    commit(2 * U);
+    if (k + U < kB) fetch(2 * (k + U), 2 * U);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int kk = k + u;
@@ -979,9 +995,12 @@ static int dwt_passes(DwtCtx &x, const bwc_codec *c, const Param &par, double *w
     const int nstrided = (int)streamed(0) + (int)streamed(1) + (int)streamed(2) + (int)streamed(3);
     const bool x_oop = (nstrided & 1) && x_lds(0);
     // X and Y in one pass (dwt97_xy_kernel) when Y streams, X fits the LDS
-    // kernel with at most 1024 samples, and the streamed passes are even in
-    // number (the fused pass is out of place and counts as Y's)
-    const bool xy = x_lds(0) && streamed(1) && !(nstrided & 1) && sz[0] <= 1024 && !tune.no_xy;
+    // kernel with at most 1024 samples, the streamed passes are even in
+    // number (the fused pass is out of place and counts as Y's) and the box
+    // holds at least 2^24 values (smaller ones are latency-bound, and the
+    // separate passes run more workgroups on them)
+    const bool xy = x_lds(0) && streamed(1) && !(nstrided & 1) && sz[0] <= 1024 &&
+                    sz[0] * sz[1] * sz[2] * sz[3] >= (1ull << 24) && !tune.no_xy;
     double *cur = work, *oth = x.work2;
     if (nstrided && x.work2_cap < par.box.volume() * sizeof(double)) {
       if (!x.grow) {

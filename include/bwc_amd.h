@@ -57,9 +57,12 @@ int bwc_amd_compress_batch(bwc_codec *const *codecs, bwc_stream *const *streams,
    changes a result. Names: "dwt_unroll" (2, 4, 8: pair iterations per load
    batch of the strided DWT passes), "dwt_seg_log2" (8-24: thread target
    2^n of those passes), "dwt_seg_min" (>= 2: minimum pair iterations per
-   line segment), "mq_lanes_full" and "mq_lanes_limited" (1, 2, 4, 8, 16, 32:
-   codeblocks per MQ wavefront for complete codings and for codings to a pass
-   limit). Values come from the defaults, then the tuning file the autotuner
+   line segment), "dwt_nt" (0-3: nontemporal hint on the streaming DWT
+   passes' stores, bit 0, and loads, bit 1), "dwt_xy" (0, 1: X and Y passes
+   of a level fused into one kernel where it applies), "mq_lanes_full" (1, 2,
+   4, 8, 16, 32) and "mq_lanes_limited" (1-6, 8, 16, 32): codeblocks per MQ
+   wavefront for complete codings and for codings to a pass limit. Values
+   come from the defaults, then the tuning file the autotuner
    writes (python -m bigwhoop_amd.autotune; BWC_AMD_TUNING names another
    file), then environment variables (BWC_AMD_DWT_UNROLL, ...), then these
    calls. bwc_amd_tuning_source gives the file read ("" when none). Setting a
