@@ -975,8 +975,9 @@ static int dwt_passes(DwtCtx &x, const bwc_codec *c, const Param &par, double *w
   struct {
     int unroll, seg_log2, seg_min, nt;
     bool no_xy;
+    int xy_min;
   } tune = {tuning().dwt_unroll, tuning().dwt_seg_log2, tuning().dwt_seg_min, tuning().dwt_nt,
-            tuning().dwt_xy == 0};
+            tuning().dwt_xy == 0, tuning().dwt_xy_min_log2};
   for (int step = 0; step < nd; ++step) {
     const int level = fwd ? step : nd - 1 - step;
     const Box &rb = par.res[nd - level].box;
@@ -997,10 +998,10 @@ static int dwt_passes(DwtCtx &x, const bwc_codec *c, const Param &par, double *w
     // X and Y in one pass (dwt97_xy_kernel) when Y streams, X fits the LDS
     // kernel with at most 1024 samples, the streamed passes are even in
     // number (the fused pass is out of place and counts as Y's) and the box
-    // holds at least 2^24 values (smaller ones are latency-bound, and the
-    // separate passes run more workgroups on them)
+    // holds at least 2^dwt_xy_min_log2 values (2^24 by default: smaller ones
+    // are latency-bound, and the separate passes run more workgroups on them)
     const bool xy = x_lds(0) && streamed(1) && !(nstrided & 1) && sz[0] <= 1024 &&
-                    sz[0] * sz[1] * sz[2] * sz[3] >= (1ull << 24) && !tune.no_xy;
+                    sz[0] * sz[1] * sz[2] * sz[3] >= (1ull << tune.xy_min) && !tune.no_xy;
     double *cur = work, *oth = x.work2;
     if (nstrided && x.work2_cap < par.box.volume() * sizeof(double)) {
       if (!x.grow) {
@@ -1196,6 +1197,7 @@ static uint64_t dwt_key(const Workspace *ws, const bwc_codec *c, const Param &pa
   mix((uint64_t)(int64_t)norm_qm);
   mix((uint64_t)tuning().dwt_unroll | (uint64_t)tuning().dwt_seg_log2 << 16 | (uint64_t)tuning().dwt_seg_min << 32 |
       (uint64_t)tuning().dwt_nt << 56 | (uint64_t)tuning().dwt_xy << 60);
+  mix((uint64_t)tuning().dwt_xy_min_log2);
   const char *d = std::getenv("BWC_AMD_DWT");
   mix(d && d[0] == '0');
   return h;

@@ -14,8 +14,9 @@ struct Tuning {
   // forward 9/7 passes: bit 0 stores, bit 1 loads of the streaming kernels
   // with the nontemporal hint (data read or written once per pass)
   int dwt_nt = 3;
-  // 1: X and Y passes of a level fused into one kernel where possible
-  int dwt_xy = 1;
+  // 1: X and Y passes of a level fused into one kernel where possible, on
+  // boxes of at least 2^dwt_xy_min_log2 values
+  int dwt_xy = 1, dwt_xy_min_log2 = 24;
   // codeblocks per MQ wavefront: complete codings (1, 2, 4, 8, 16 or 32) and
   // codings to a pass limit (1-6, 8, 16 or 32)
   int mq_lanes_full = 16, mq_lanes_limited = 5;

@@ -59,7 +59,8 @@ int bwc_amd_compress_batch(bwc_codec *const *codecs, bwc_stream *const *streams,
    2^n of those passes), "dwt_seg_min" (>= 2: minimum pair iterations per
    line segment), "dwt_nt" (0-3: nontemporal hint on the streaming DWT
    passes' stores, bit 0, and loads, bit 1), "dwt_xy" (0, 1: X and Y passes
-   of a level fused into one kernel where it applies), "mq_lanes_full" (1, 2,
+   of a level fused into one kernel where it applies), "dwt_xy_min_log2"
+   (0-40: the fusion only on boxes of at least 2^n values), "mq_lanes_full" (1, 2,
    4, 8, 16, 32) and "mq_lanes_limited" (1-6, 8, 16, 32): codeblocks per MQ
    wavefront for complete codings and for codings to a pass limit. Values
    come from the defaults, then the tuning file the autotuner

@@ -60,7 +60,9 @@ def test_dwt_bit_exact(case, path, monkeypatch):
 
 
 # line lengths from 2 upwards, odd and even, tile offsets that make the
-# lowpass start on odd positions, 1-4 transformed axes
+# lowpass start on odd positions, 1-4 transformed axes; X lines up to 1024
+# samples and an untransformed axis between two transformed ones for the
+# fused X-Y kernel
 DWT_SHAPES = [
     dict(shape=(37, 5, 9), decomp=(3, 2, 3, 0)),
     dict(shape=(3, 70, 2), decomp=(1, 4, 1, 0)),
@@ -69,13 +71,27 @@ DWT_SHAPES = [
     dict(shape=(21, 19, 17, 6), decomp=(2, 2, 2, 1)),
     dict(shape=(45, 33, 29), decomp=(3, 3, 3, 0), tiles=(24, 24, 24, 1)),
     dict(shape=(600, 7, 3), decomp=(4, 1, 1, 0)),
+    dict(shape=(301, 40, 3), decomp=(3, 3, 1, 0), tiles=(150, 24, 3, 1)),
+    dict(shape=(1024, 11, 2), decomp=(1, 2, 1, 0)),
+    dict(shape=(96, 150, 5, 4), decomp=(3, 3, 0, 1)),
 ]
 
 
-@pytest.mark.parametrize("path", ["fast", "generic"])
+def _dwt_path(path, monkeypatch):
+    """fast: LDS X pass, streaming strided passes, X and Y fused on large
+    boxes only; xy: the fused X-Y kernel on every box it applies to (3-D
+    boxes with X lines of at most 1024 samples); generic: the in-place LDS
+    kernel only"""
+    monkeypatch.setenv("BWC_AMD_DWT", "0" if path == "generic" else "1")
+    before = bwc.get_tuning("dwt_xy_min_log2")
+    bwc.set_tuning("dwt_xy_min_log2", 0 if path == "xy" else 24)
+    return before
+
+
+@pytest.mark.parametrize("path", ["fast", "xy", "generic"])
 @pytest.mark.parametrize("spec", DWT_SHAPES, ids=[str(s["shape"]) for s in DWT_SHAPES])
 def test_dwt_shapes_bit_exact(spec, path, monkeypatch):
-    monkeypatch.setenv("BWC_AMD_DWT", "1" if path == "fast" else "0")
+    before = _dwt_path(path, monkeypatch)
     _say("dwt shapes", path, spec)
     from bigwhoop_amd.fields import grf
     shape = spec["shape"]
@@ -92,6 +108,7 @@ def test_dwt_shapes_bit_exact(spec, path, monkeypatch):
             o.dwt_forward(w, t, 0)
             assert np.array_equal(g.view(np.uint64), w.view(np.uint64)), (t, spec)
     finally:
+        bwc.set_tuning("dwt_xy_min_log2", before)
         c.close()
         o.close()
 
