@@ -785,7 +785,7 @@ __global__ void __launch_bounds__(256) dwt97_stream_kernel(const double *__restr
 // dwt97_x_kernel's steps; then every thread advances the Y pipeline of its
 // X-transformed positions o = t, t + 256, ... over those rows with
 // dwt97_stream_kernel's steps and stores the outputs. The next batch's rows
-// are loaded into registers while the current batch's Y steps run. Every
+// are loaded into registers while the current batch's X and Y steps run. Every
 // coefficient is the same sequence of binary64 operations as in the two
 // separate passes. Row slot r of LDS: Ev = lds + 2 hp r, Od = Ev + hp, as in
 // dwt97_x_kernel; slot r is loaded by wavefront r % 4.
@@ -828,8 +828,8 @@ __global__ void __launch_bounds__(256) dwt97_xy_kernel(const double *__restrict_
       }
     }
   };
-  // the fetched rows into their slots, normalised when asked, X-transformed
-  auto commit = [&](int nrows) {
+  // the fetched rows into their slots, normalised when asked
+  auto stage = [&](int nrows) {
 #pragma unroll
     for (int j = 0; j < SPW; ++j) {
       const int r = wv + 4 * j;
@@ -844,6 +844,9 @@ __global__ void __launch_bounds__(256) dwt97_xy_kernel(const double *__restrict_
         else Ev[e >> 1] = x;
       }
     }
+  };
+  // the staged rows X-transformed
+  auto lift = [&](int nrows) {
     __syncthreads();
     for (int r = wv; r < nrows; r += 4) {
       double *Ev = lds + (size_t)2 * hp * r, *Od = Ev + hp;
@@ -881,7 +884,8 @@ __global__ void __launch_bounds__(256) dwt97_xy_kernel(const double *__restrict_
   double xm2[CMAX], xm1[CMAX], am3[CMAX], bm4[CMAX], cm5[CMAX];
   const int k0 = kA - 4;
   fetch(2 * k0 - 2, 2);
-  commit(2);
+  stage(2);
+  lift(2);
 #pragma unroll
   for (int c = 0; c < CMAX; ++c) {
     const int o = threadIdx.x + 256 * c;
@@ -892,8 +896,9 @@ __global__ void __launch_bounds__(256) dwt97_xy_kernel(const double *__restrict_
   fetch(2 * k0, 2 * U);
   __syncthreads();
   for (int k = k0; k < kB; k += U) {
-    commit(2 * U);
+    stage(2 * U);
     if (k + U < kB) fetch(2 * (k + U), 2 * U);
+    lift(2 * U);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int kk = k + u;
