@@ -954,6 +954,7 @@ int bwc_amd_compress_batch(bwc_codec *const *codecs, bwc_stream *const *streams,
   std::string first_err;
   auto work = [&] {
     (void)hipSetDevice(device);
+    gpu::set_thread_graphs(false);
     for (int i; !failed.load() && (i = next.fetch_add(1)) < n;) {
       sizes[i] = bwc_compress(codecs[i], streams[i]);
       if (!sizes[i]) {
@@ -971,6 +972,7 @@ int bwc_amd_compress_batch(bwc_codec *const *codecs, bwc_stream *const *streams,
       if (std::find(in_queue.begin(), in_queue.end(), i) == in_queue.end()) order.push_back(i);
     auto qwork = [&](int first) {
       (void)hipSetDevice(device);
+      gpu::set_thread_graphs(false);
       for (size_t k = (size_t)first; k < order.size(); k += (size_t)workers) {
         const int i = order[k];
         gpu::Workspace *ws = gpu::workspace_of(job_of(codecs[i]));
@@ -985,6 +987,7 @@ int bwc_amd_compress_batch(bwc_codec *const *codecs, bwc_stream *const *streams,
     };
     for (int w = 1; w < workers; ++w) th.emplace_back(qwork, w);
     qwork(0);
+    gpu::set_thread_graphs(true);
     for (auto &t : th) t.join();
     for (int i : in_queue) {
       gpu::Workspace *ws = gpu::workspace_of(job_of(codecs[i]));
@@ -995,6 +998,7 @@ int bwc_amd_compress_batch(bwc_codec *const *codecs, bwc_stream *const *streams,
   } else {
     for (int w = 1; w < workers; ++w) th.emplace_back(work);
     work();
+    gpu::set_thread_graphs(true);
     for (auto &t : th) t.join();
   }
   if (!first_err.empty()) {

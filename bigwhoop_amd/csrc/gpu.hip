@@ -1257,7 +1257,13 @@ uint64_t dwt_launch_key(const Workspace *ws, const bwc_codec *c, const Param &pa
   return dwt_key(ws, c, par, work, norm_qm);
 }
 
+namespace {
+thread_local bool t_graphs_off = false;
+}
+void set_thread_graphs(bool on) { t_graphs_off = !on; }
+
 bool graphs_on() {
+  if (t_graphs_off) return false;
   const char *e = std::getenv("BWC_AMD_GRAPH");
   return !(e && e[0] == '0');
 }
@@ -2031,17 +2037,38 @@ __global__ void __launch_bounds__(256) bitplane_kernel(const double *__restrict_
     // K first: x86_to_u64 is monotone on quotients in [0, 2^64), maps NaN to
     // 2^63 (bit length 64) and larger quotients to 0, so K is the bit length
     // of the largest quotient below 2^64, or 64 with a NaN
+    // Division by the step is monotone, so the largest quotient is the
+    // thread's largest magnitude divided once; only when that reaches 2^64
+    // does the thread look for the largest quotient below it value by value.
+    // Two columns per iteration keep eight loads in flight.
     double dmax = 0;
     bool nan = false;
+    uint64_t amax = 0;   // largest magnitude bits of the thread's non-NaN values
 #pragma unroll 1
-    for (uint32_t k = threadIdx.x; k < kend; k += blockDim.x) {
-      uint64_t raw[4];
+    for (uint32_t k = threadIdx.x; k < kend; k += 2 * blockDim.x) {
+      uint64_t raw[8];
       bp_load(work, g, k, raw);
+      bp_load(work, g, k + blockDim.x, raw + 4);   // past the codeblock: zeros
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const double d = bp_quot(raw[r], g.step);
-        nan |= d != d;
-        dmax = d < 18446744073709551616.0 && d > dmax ? d : dmax;
+      for (int r = 0; r < 8; ++r) {
+        const uint64_t a = raw[r] & 0x7FFFFFFFFFFFFFFFull;
+        const bool isn = a > 0x7FF0000000000000ull;
+        nan |= isn;
+        amax = (!isn && a > amax) ? a : amax;
+      }
+    }
+    dmax = __longlong_as_double((long long)amax) / g.step;
+    if (!(dmax < 18446744073709551616.0)) {
+      dmax = 0;
+#pragma unroll 1
+      for (uint32_t k = threadIdx.x; k < kend; k += blockDim.x) {
+        uint64_t raw[4];
+        bp_load(work, g, k, raw);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const double d = bp_quot(raw[r], g.step);
+          dmax = d < 18446744073709551616.0 && d > dmax ? d : dmax;
+        }
       }
     }
     for (int o = 32; o > 0; o >>= 1) {
@@ -2994,7 +3021,8 @@ int t1_encode(Workspace *ws, const bwc_codec *c, const Param &par, const double 
       // decisions per codeblock: the symbol position where its last coded
       // pass ends (measurement only)
       std::vector<uint32_t> pc((size_t)ncb * kMaxPasses);
-      HIP_OK(hipMemcpy(pc.data(), ws->d_pcnt, pc.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+      HIP_OK(hipMemcpyAsync(pc.data(), ws->d_pcnt, pc.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, ws->stream));
+      HIP_OK(hipStreamSynchronize(ws->stream));
       rec.decisions.assign(ncb, 0);
       for (size_t r = 0; r < list.size(); ++r)
         if (m[r]) rec.decisions[list[r]] = pc[list[r] * kMaxPasses + m[r] - 1];

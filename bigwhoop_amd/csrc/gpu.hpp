@@ -201,6 +201,10 @@ struct PassLimits {
 };
 // pipeline graphs (BWC_AMD_GRAPH, on unless set to 0)
 bool graphs_on();
+// pipeline graphs for the calling thread's compressions (on unless set off;
+// the batch's threads run without them, so that no thread captures while
+// the others launch)
+void set_thread_graphs(bool on);
 // the forward DWT's launches without a graph of its own, and their key
 int dwt_forward_direct(Workspace *ws, const bwc_codec *c, const Param &par, double *work, int norm_qm);
 uint64_t dwt_launch_key(const Workspace *ws, const bwc_codec *c, const Param &par, const double *work, int norm_qm);

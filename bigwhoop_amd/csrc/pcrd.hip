@@ -965,7 +965,7 @@ T *ptr(RcCtx::Buf &b) {
 // packet order of tile ti: parameters, resolutions, precincts; within a
 // packet every subband's precinct codeblocks in turn. Each precinct-subband
 // MSB tree gets its nodes at a running offset (leaves first).
-int build_geo(bwc_codec *c, Job *job, uint64_t ti, RcGeo &g) {
+int build_geo(bwc_codec *c, Job *job, uint64_t ti, RcGeo &g, hipStream_t s) {
   Tile &tl = job->plan.tiles[ti];
   g.cb.clear();
   g.pk_first.clear();
@@ -1001,9 +1001,13 @@ int build_geo(bwc_codec *c, Job *job, uint64_t ti, RcGeo &g) {
   HIP_RC(hipMalloc((void **)&g.d_cb, std::max<size_t>(1, g.G) * sizeof(RcCb)));
   HIP_RC(hipMalloc((void **)&g.d_pk_first, (g.NP + 1) * sizeof(uint32_t)));
   HIP_RC(hipMalloc((void **)&g.d_parent, std::max<size_t>(1, g.N) * sizeof(int32_t)));
-  if (g.G) HIP_RC(hipMemcpy(g.d_cb, g.cb.data(), g.G * sizeof(RcCb), hipMemcpyHostToDevice));
-  HIP_RC(hipMemcpy(g.d_pk_first, g.pk_first.data(), (g.NP + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
-  if (g.N) HIP_RC(hipMemcpy(g.d_parent, parent.data(), g.N * sizeof(int32_t), hipMemcpyHostToDevice));
+  // copies on the call's own (non-blocking) stream: a copy on the legacy
+  // stream would depend on every blocking stream, which another thread's
+  // graph capture forbids
+  if (g.G) HIP_RC(hipMemcpyAsync(g.d_cb, g.cb.data(), g.G * sizeof(RcCb), hipMemcpyHostToDevice, s));
+  HIP_RC(hipMemcpyAsync(g.d_pk_first, g.pk_first.data(), (g.NP + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  if (g.N) HIP_RC(hipMemcpyAsync(g.d_parent, parent.data(), g.N * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  HIP_RC(hipStreamSynchronize(s));
   return 0;
 }
 
@@ -1119,7 +1123,7 @@ int rc_tile(Workspace *ws, bwc_codec *c, Job *job, uint64_t ti, SkipCheck *chk) 
     x.geo_of = job;
   }
   RcGeo &g = x.geo[ti];
-  if (!g.d_pk_first && build_geo(c, job, ti, g)) return 1;
+  if (!g.d_pk_first && build_geo(c, job, ti, g, ws->stream)) return 1;
   if (!x.d_st) {
     HIP_RC(hipMalloc((void **)&x.d_st, sizeof(RcState)));
     HIP_RC(hipHostMalloc((void **)&x.h_st, sizeof(RcState)));

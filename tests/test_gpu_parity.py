@@ -175,3 +175,21 @@ def test_headline_settings_match_oracle_at_scale(n):
         ref = ol.oracle_compress(x, (n, n, n), **kw)
         assert len(got) == len(ref)
         assert got == ref
+
+
+@pytest.mark.parametrize("qm", [50, 56, 60, 62])
+def test_large_qm_matches_oracle(qm):
+    """Quantised magnitudes near and past 2^64 (the bitplane kernel's bit
+    count takes the largest quotient below 2^64): codestream against the
+    oracle at large Qm."""
+    from bigwhoop_amd.fields import grf
+    shape = (40, 36, 20)
+    kw = dict(rate="4", decomp=(3, 3, 2, 0), cb=(4, 4, 3, 0), qm=qm)
+    _say("large qm", qm)
+    x = grf(shape, seed=3).reshape(-1)
+    ref = ol.oracle_compress(x, shape, **kw)
+    c = bwc.Codec(shape, **kw)
+    try:
+        assert c.compress(x) == ref
+    finally:
+        c.close()
