@@ -777,6 +777,91 @@ __global__ void __launch_bounds__(256) dwt97_stream_kernel(const double *__restr
   }
 }
 
+// Inverse 9/7 along a strided axis, one thread per line segment as in
+// dwt97_stream_kernel (consecutive lanes on consecutive lines). The line is
+// the interleaving of its lowpass half [0, half) and highpass half [half,
+// len) (fill_inverse_buffer, dwt.c:169-272), extended by whole-point
+// symmetry; the reference's inverse steps (dwt.c:428-465) on working index i
+// (position i + par0) are
+//   st0 (even i in [-4, m0]): e = KH x_i - D (x_{i-1} + x_{i+1})
+//   st1 (odd i in [-5, m1]):  o = KL x_i - G (e_{i-1} + e_{i+1})
+//   st2 (even i in [-4, m2]): f = e_i - B (o_{i-1} + o_{i+1})
+//   st3 (odd i in [-3, m3]):  g = o_i - A (f_{i-1} + f_{i+1})
+// with positions outside a step's range left as they are. After reading
+// x_{2k} and x_{2k+1} the thread finishes e_{2k}, o_{2k-1}, f_{2k-2} and
+// g_{2k-3}, and stores f_{2k-2} and g_{2k-3} at their positions. Segments
+// start four iterations early with zero carried values, which are exact
+// again after three. Outputs land where lowpass and highpass inputs may
+// still be unread, so the pass is out of place.
+template <int U, int NT>
+__global__ void __launch_bounds__(256) idwt97_stream_kernel(const double *__restrict__ src, double *__restrict__ dst,
+                                                            int len, int par0, int half, uint64_t sa, uint32_t n0,
+                                                            uint64_t s0, uint32_t n1, uint64_t s1, uint64_t s2,
+                                                            uint32_t nlines, int nseg, int ksz) {
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= nlines * (uint32_t)nseg) return;
+  const uint32_t l = tid % nlines, seg = tid / nlines;
+  const uint32_t i0 = l % n0, r = l / n0;
+  const uint32_t i1 = r % n1, i2 = r / n1;
+  const uint64_t base = (uint64_t)i0 * s0 + (uint64_t)i1 * s1 + (uint64_t)i2 * s2;
+  const double *s = src + base;
+  double *d = dst + base;
+  const int m0 = -4 + 2 * ((len + 9) / 2 - 1), m1 = -5 + 2 * ((len + 9) / 2 - 1);
+  const int m2 = -4 + 2 * ((len + 7) / 2 - 1), m3 = -3 + 2 * ((len + 5) / 2 - 1);
+  const int ilo = -par0, ihi = len - par0;   // output working indices [ilo, ihi)
+  const int kend = (ihi + 2) / 2 + 1;        // past the last k with an output
+  const int kA = 1 + (int)seg * ksz, kB = min(kend, kA + ksz);
+  if (kA >= kB) return;
+  // interleaved sample at working index i (position mirrored into the line)
+  auto ld = [&](int i) -> double {
+    int p = i + par0;
+    p = ((unsigned)p < (unsigned)len) ? p : mirror32(p, len);
+    const int q = p - par0;
+    const int from = (q >= 0 && !(q & 1)) ? (q >> 1) : (half + ((p + par0 - 1) >> 1));
+    return st_ld<(NT & 2) != 0>(&s[(int64_t)from * (int64_t)sa]);
+  };
+  // carried: x_{2k-1}, e_{2k-2}, o_{2k-3}, f_{2k-4}
+  const int k0 = kA - 4;
+  double xm1 = ld(2 * k0 - 1), em2 = 0, om3 = 0, fm4 = 0;
+  for (int k = k0; k < kB; k += U) {
+    double xv[2 * U];
+#pragma unroll
+    for (int u = 0; u < 2 * U; ++u) xv[u] = ld(2 * k + u);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int kk = k + u;
+      if (kk >= kB) break;
+      const double x0 = xv[2 * u], x1 = xv[2 * u + 1];
+      int i = 2 * kk;
+      double e0 = x0;
+      if (i >= -4 && i <= m0) {
+        e0 = Lift97::KH * x0;
+        e0 -= Lift97::D * (xm1 + x1);
+      }
+      i = 2 * kk - 1;
+      double o1 = xm1;
+      if (i >= -5 && i <= m1) {
+        o1 = Lift97::KL * xm1;
+        o1 -= Lift97::G * (em2 + e0);
+      }
+      i = 2 * kk - 2;
+      const double f2 = (i >= -4 && i <= m2) ? em2 - Lift97::B * (om3 + o1) : em2;
+      i = 2 * kk - 3;
+      const double g3 = (i >= -3 && i <= m3) ? om3 - Lift97::A * (fm4 + f2) : om3;
+      if (kk >= kA) {
+        i = 2 * kk - 2;
+        if (i >= ilo && i < ihi) st_st<(NT & 1) != 0>(&d[(int64_t)(i + par0) * (int64_t)sa], f2);
+        i = 2 * kk - 3;
+        if (i >= ilo && i < ihi) st_st<(NT & 1) != 0>(&d[(int64_t)(i + par0) * (int64_t)sa], g3);
+      }
+      xm1 = x1;
+      em2 = e0;
+      om3 = o1;
+      fm4 = f2;
+    }
+  }
+}
+
 // X and Y passes of one level in one kernel, for boxes whose X lines fit
 // LDS: the box is read once and written once for both axes. A workgroup of
 // 256 threads takes one (Z, T) plane and one segment of its Y iterations.
@@ -981,8 +1066,9 @@ static int dwt_passes(DwtCtx &x, const bwc_codec *c, const Param &par, double *w
     int unroll, seg_log2, seg_min, nt;
     bool no_xy;
     int xy_min;
+    bool no_inv;
   } tune = {tuning().dwt_unroll, tuning().dwt_seg_log2, tuning().dwt_seg_min, tuning().dwt_nt,
-            tuning().dwt_xy == 0, tuning().dwt_xy_min_log2};
+            tuning().dwt_xy == 0, tuning().dwt_xy_min_log2, tuning().dwt_inv_stream == 0};
   for (int step = 0; step < nd; ++step) {
     const int level = fwd ? step : nd - 1 - step;
     const Box &rb = par.res[nd - level].box;
@@ -1008,7 +1094,15 @@ static int dwt_passes(DwtCtx &x, const bwc_codec *c, const Param &par, double *w
     const bool xy = x_lds(0) && streamed(1) && !(nstrided & 1) && sz[0] <= 1024 &&
                     sz[0] * sz[1] * sz[2] * sz[3] >= (1ull << tune.xy_min) && !tune.no_xy;
     double *cur = work, *oth = x.work2;
-    if (nstrided && x.work2_cap < par.box.volume() * sizeof(double)) {
+    // inverse 9/7 along the strided axes on the streaming kernel, when the
+    // level has an even number of such passes (each moves the box to the
+    // other buffer; X stays on the in-place kernel)
+    auto istream = [&](int a) {
+      return !fwd && !generic_only && a != 0 && kern[a] == bwc_dwt_9_7 && active(a) && !tune.no_inv;
+    };
+    const int ninv = (int)istream(1) + (int)istream(2) + (int)istream(3);
+    const bool inv = ninv > 0 && !(ninv & 1);
+    if ((nstrided || inv) && x.work2_cap < par.box.volume() * sizeof(double)) {
       if (!x.grow) {
         std::snprintf(g_err, sizeof g_err, "DWT second buffer of %zu bytes is smaller than the %llu the box needs",
                       x.work2_cap, (unsigned long long)(par.box.volume() * sizeof(double)));
@@ -1136,6 +1230,29 @@ static int dwt_passes(DwtCtx &x, const bwc_codec *c, const Param &par, double *w
         ++x.launches;
         continue;
       }
+      if (inv && istream(a)) {
+        const uint64_t nlines = sz[o[0]] * sz[o[1]] * sz[o[2]];
+        const int kout = (int)((len - par0 + 2) / 2);
+        int nseg = 1;
+        while ((uint64_t)nseg * 2 * nlines <= (1ull << tune.seg_log2) && kout / (nseg * 2) >= tune.seg_min) nseg *= 2;
+        const int ksz = (kout + nseg - 1) / nseg;
+        const uint32_t nblk = (uint32_t)((nlines * (uint64_t)nseg + 255) / 256);
+#define ISTREAM_LAUNCH(NT)                                                                                           \
+  hipLaunchKernelGGL((idwt97_stream_kernel<4, NT>), dim3(nblk), dim3(256), 0, x.stream, cur, oth, (int)len, par0,    \
+                     half, inc[a], (uint32_t)sz[o[0]], inc[o[0]], (uint32_t)sz[o[1]], inc[o[1]], inc[o[2]],           \
+                     (uint32_t)nlines, nseg, ksz)
+        switch (tune.nt) {
+          case 1: ISTREAM_LAUNCH(1); break;
+          case 2: ISTREAM_LAUNCH(2); break;
+          case 3: ISTREAM_LAUNCH(3); break;
+          default: ISTREAM_LAUNCH(0); break;
+        }
+#undef ISTREAM_LAUNCH
+        HIP_OK(hipGetLastError());
+        std::swap(cur, oth);
+        ++x.launches;
+        continue;
+      }
       if (lds > 160 * 1024) {
         std::snprintf(g_err, sizeof g_err, "DWT line of %lld samples exceeds the LDS line buffer", (long long)len);
         return 1;
@@ -1202,7 +1319,7 @@ static uint64_t dwt_key(const Workspace *ws, const bwc_codec *c, const Param &pa
   mix((uint64_t)(int64_t)norm_qm);
   mix((uint64_t)tuning().dwt_unroll | (uint64_t)tuning().dwt_seg_log2 << 16 | (uint64_t)tuning().dwt_seg_min << 32 |
       (uint64_t)tuning().dwt_nt << 56 | (uint64_t)tuning().dwt_xy << 60);
-  mix((uint64_t)tuning().dwt_xy_min_log2);
+  mix((uint64_t)tuning().dwt_xy_min_log2 | (uint64_t)tuning().dwt_inv_stream << 8);
   const char *d = std::getenv("BWC_AMD_DWT");
   mix(d && d[0] == '0');
   return h;

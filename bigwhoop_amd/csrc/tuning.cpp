@@ -36,6 +36,7 @@ const Knob kKnobs[] = {
     {"dwt_nt", "BWC_AMD_DWT_NT", &Tuning::dwt_nt, 0, 3, false, 0},
     {"dwt_xy", "BWC_AMD_DWT_XY", &Tuning::dwt_xy, 0, 1, false, 0},
     {"dwt_xy_min_log2", "BWC_AMD_DWT_XY_MIN_LOG2", &Tuning::dwt_xy_min_log2, 0, 40, false, 0},
+    {"dwt_inv_stream", "BWC_AMD_DWT_INV_STREAM", &Tuning::dwt_inv_stream, 0, 1, false, 0},
     {"mq_lanes_full", "BWC_AMD_MQ_LANES_FULL", &Tuning::mq_lanes_full, 1, 32, true, 0},
     {"mq_lanes_limited", "BWC_AMD_MQ_LANES_LIMITED", &Tuning::mq_lanes_limited, 1, 32, false, kMqLanes},
 };

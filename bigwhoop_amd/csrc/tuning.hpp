@@ -17,6 +17,8 @@ struct Tuning {
   // 1: X and Y passes of a level fused into one kernel where possible, on
   // boxes of at least 2^dwt_xy_min_log2 values
   int dwt_xy = 1, dwt_xy_min_log2 = 24;
+  // 1: inverse 9/7 along strided axes on the streaming kernel
+  int dwt_inv_stream = 1;
   // codeblocks per MQ wavefront: complete codings (1, 2, 4, 8, 16 or 32) and
   // codings to a pass limit (1-6, 8, 16 or 32)
   int mq_lanes_full = 16, mq_lanes_limited = 5;

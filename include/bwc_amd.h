@@ -60,7 +60,9 @@ int bwc_amd_compress_batch(bwc_codec *const *codecs, bwc_stream *const *streams,
    line segment), "dwt_nt" (0-3: nontemporal hint on the streaming DWT
    passes' stores, bit 0, and loads, bit 1), "dwt_xy" (0, 1: X and Y passes
    of a level fused into one kernel where it applies), "dwt_xy_min_log2"
-   (0-40: the fusion only on boxes of at least 2^n values), "mq_lanes_full" (1, 2,
+   (0-40: the fusion only on boxes of at least 2^n values), "dwt_inv_stream"
+   (0, 1: inverse 9/7 along strided axes on the streaming kernel),
+   "mq_lanes_full" (1, 2,
    4, 8, 16, 32) and "mq_lanes_limited" (1-6, 8, 16, 32): codeblocks per MQ
    wavefront for complete codings and for codings to a pass limit. Values
    come from the defaults, then the tuning file the autotuner

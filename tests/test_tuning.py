@@ -32,7 +32,8 @@ def test_set_get_and_ranges():
         for name, good, bad in [("dwt_unroll", [2, 4, 8], [1, 3, 16]), ("dwt_seg_log2", [8, 17, 24], [7, 25]),
                                 ("dwt_seg_min", [2, 100], [0, 1]), ("mq_lanes_full", [1, 2, 32], [0, 3, 64]),
                                 ("mq_lanes_limited", [3, 5, 6, 16], [7, 12, 33]), ("dwt_nt", [0, 3], [-1, 4]),
-                                ("dwt_xy", [0, 1], [2]), ("dwt_xy_min_log2", [0, 24, 40], [-1, 41])]:
+                                ("dwt_xy", [0, 1], [2]), ("dwt_xy_min_log2", [0, 24, 40], [-1, 41]),
+                                ("dwt_inv_stream", [0, 1], [2])]:
             for v in good:
                 bwc.set_tuning(name, v)
                 assert bwc.get_tuning(name) == v
