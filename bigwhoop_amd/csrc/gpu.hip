@@ -494,11 +494,11 @@ __global__ void dwt_axis_kernel(double *__restrict__ data, int64_t len, int par0
     if (FWD) {
       v = base[g * s0 + p * sa];
     } else {
-      // inverse: interleave lowpass [0,half) and highpass [half,len)
-      // (fill_inverse_buffer, dwt.c:169-272): sample p comes from
-      // low[(p - par0)/2] when p - par0 is even, else high[(p + par0 - 1)/2]
-      const int64_t q = p - par0;
-      const int64_t src = (q >= 0 && (q & 1) == 0) ? (q >> 1) : (half + ((p + par0 - 1) >> 1));
+      // inverse: interleave lowpass [0,half) and highpass [half,len) as
+      // fill_inverse_buffer does (dwt.c:169-272): position p of the line
+      // holds low[p/2] when p is even and high[(p-1)/2] when p is odd, for
+      // either origin parity (an odd origin only shifts the lifting by one)
+      const int64_t src = !(p & 1) ? (p >> 1) : (half + ((p - 1) >> 1));
       v = base[g * s0 + src * sa];
     }
     lds[g * pitch + DWT_E + p] = v;
@@ -777,6 +777,74 @@ __global__ void __launch_bounds__(256) dwt97_stream_kernel(const double *__restr
   }
 }
 
+// Inverse 9/7 along X (contiguous lines), one wavefront per line as in
+// dwt97_x_kernel: the extended interleaved line (idwt97_stream_kernel's
+// sample at each working index) split into even and odd working positions in
+// LDS (Ev[j] holds i = 2j - 8, Od[j] holds i = 2j - 7), the four inverse
+// steps over their ranges, and the line written back in natural order.
+// src and dst may be the same buffer.
+__global__ void __launch_bounds__(256) idwt97_x_kernel(const double *src, double *dst, int len, int par0, int half,
+                                                       uint32_t n0, uint64_t s0, uint32_t n1, uint64_t s1,
+                                                       uint64_t s2) {
+  extern __shared__ double lds[];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int hp = (len + 16) / 2 + 1;
+  double *Ev = lds + (size_t)wv * 2 * hp, *Od = Ev + hp;
+  const uint32_t i0 = blockIdx.x * 4 + wv;
+  const uint32_t i1 = blockIdx.y % n1, i2 = blockIdx.y / n1;
+  const bool act = i0 < n0;
+  const uint64_t base = (uint64_t)i0 * s0 + (uint64_t)i1 * s1 + (uint64_t)i2 * s2;
+  if (act) {
+    const double *s = src + base;
+    for (int e0 = 0; e0 < len + 16; e0 += 64 * 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + 64 * u + lane;
+        int p = e - 8 + par0;
+        p = ((unsigned)p < (unsigned)len) ? p : mirror32(p, len);
+        const int from = !(p & 1) ? (p >> 1) : (half + ((p - 1) >> 1));
+        v[u] = (e < len + 16) ? s[from] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + 64 * u + lane;
+        if (e >= len + 16) break;
+        if (e & 1) Od[e >> 1] = v[u];
+        else Ev[e >> 1] = v[u];
+      }
+    }
+  }
+  __syncthreads();
+  if (act)
+    for (int m = lane; m < (len + 9) / 2; m += 64) {
+      double t = Lift97::KH * Ev[2 + m];
+      t -= Lift97::D * (Od[1 + m] + Od[2 + m]);
+      Ev[2 + m] = t;
+    }
+  __syncthreads();
+  if (act)
+    for (int m = lane; m < (len + 9) / 2; m += 64) {
+      double t = Lift97::KL * Od[1 + m];
+      t -= Lift97::G * (Ev[1 + m] + Ev[2 + m]);
+      Od[1 + m] = t;
+    }
+  __syncthreads();
+  if (act)
+    for (int m = lane; m < (len + 7) / 2; m += 64) Ev[2 + m] -= Lift97::B * (Od[1 + m] + Od[2 + m]);
+  __syncthreads();
+  if (act)
+    for (int m = lane; m < (len + 5) / 2; m += 64) Od[2 + m] -= Lift97::A * (Ev[2 + m] + Ev[3 + m]);
+  __syncthreads();
+  if (act) {
+    double *d = dst + base;
+    for (int p = lane; p < len; p += 64) {
+      const int e = p - par0 + 8;
+      d[p] = (e & 1) ? Od[e >> 1] : Ev[e >> 1];
+    }
+  }
+}
+
 // Inverse 9/7 along a strided axis, one thread per line segment as in
 // dwt97_stream_kernel (consecutive lanes on consecutive lines). The line is
 // the interleaving of its lowpass half [0, half) and highpass half [half,
@@ -816,8 +884,7 @@ __global__ void __launch_bounds__(256) idwt97_stream_kernel(const double *__rest
   auto ld = [&](int i) -> double {
     int p = i + par0;
     p = ((unsigned)p < (unsigned)len) ? p : mirror32(p, len);
-    const int q = p - par0;
-    const int from = (q >= 0 && !(q & 1)) ? (q >> 1) : (half + ((p + par0 - 1) >> 1));
+    const int from = !(p & 1) ? (p >> 1) : (half + ((p - 1) >> 1));
     return st_ld<(NT & 2) != 0>(&s[(int64_t)from * (int64_t)sa]);
   };
   // carried: x_{2k-1}, e_{2k-2}, o_{2k-3}, f_{2k-4}
@@ -1227,6 +1294,15 @@ static int dwt_passes(DwtCtx &x, const bwc_codec *c, const Param &par, double *w
 #undef STREAM_LAUNCH
         HIP_OK(hipGetLastError());
         std::swap(cur, oth);
+        ++x.launches;
+        continue;
+      }
+      if (!fwd && !generic_only && a == 0 && kern[0] == bwc_dwt_9_7 && !tune.no_inv && xlds <= 160 * 1024) {
+        // inverse X on the wavefront-per-line LDS kernel, in place
+        hipLaunchKernelGGL(idwt97_x_kernel, dim3((uint32_t)((sz[o[0]] + 3) / 4), (uint32_t)gy), dim3(256), xlds,
+                           x.stream, cur, cur, (int)len, par0, half, (uint32_t)sz[o[0]], inc[o[0]],
+                           (uint32_t)sz[o[1]], inc[o[1]], inc[o[2]]);
+        HIP_OK(hipGetLastError());
         ++x.launches;
         continue;
       }
@@ -3712,6 +3788,7 @@ static int process_setup() {
     HIP_OK(hipFuncSetAttribute((const void *)dwt_axis_kernel<1, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
     HIP_OK(hipFuncSetAttribute((const void *)dwt_axis_kernel<1, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
     HIP_OK(hipFuncSetAttribute((const void *)dwt97_x_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
+    HIP_OK(hipFuncSetAttribute((const void *)idwt97_x_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
 #define XY_ATTR(NT)                                                                                                 \
   HIP_OK(hipFuncSetAttribute((const void *)dwt97_xy_kernel<4, NT, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, \
                              lds_max));                                                                          \

@@ -45,6 +45,15 @@ LAYOUTS = [
     dict(shape=(57, 33, 21), rate="0.75,3", decomp=(3, 2, 2, 0), cb=(4, 4, 4, 0)),
     dict(shape=(40, 40, 16, 6), rate="4", decomp=(3, 3, 2, 1), cb=(4, 4, 3, 1)),
     dict(shape=(48, 40, 24), rate="1.5,6", decomp=(3, 3, 2, 0), cb=(4, 4, 3, 0), tiles=(32, 24, 16, 0), erres=True),
+    # odd line lengths down to 2, tile origins that put the lowpass on odd
+    # positions (lines [45, 69), [23, 35) and [3, 5) of the second tile),
+    # segments of the streaming inverse passes. An odd origin with an odd
+    # length is left out: the reference decoder then reads one sample of its
+    # working buffer that it never filled (dwt.c:169-272), so its output
+    # there depends on the lines its thread decoded before.
+    dict(shape=(37, 29, 9), rate="3", decomp=(3, 2, 3, 0), cb=(4, 4, 3, 0)),
+    dict(shape=(21, 69, 11), rate="2", decomp=(1, 4, 2, 0), cb=(4, 4, 2, 0), tiles=(21, 45, 11, 0)),
+    dict(shape=(9, 130, 67), rate="2", decomp=(2, 5, 5, 0), cb=(3, 5, 4, 0)),
 ]
 
 
