@@ -530,7 +530,9 @@ def header_info(cs: bytes) -> dict:
     """bwc_open_header on a codestream: dimensions, parameters, precisions,
     layers and their rates."""
     L = lib()
-    buf = np.frombuffer(cs, np.uint8).copy()
+    # bwc_open_header only reads the buffer, so the codestream is passed in
+    # place (a copy of a large codestream would cost more than the parse)
+    buf = np.frombuffer(cs, np.uint8)
     h = L.bwc_open_header(buf.ctypes.data_as(C.c_void_p))
     if not h:
         raise RuntimeError("bwc_open_header failed: " + last_error())
